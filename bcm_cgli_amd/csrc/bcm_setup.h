@@ -1,0 +1,88 @@
+// Host-side case construction for the BCM solver: cube lattice, cell flags,
+// immersed-boundary (IB) tables, interface tables, domain-boundary cube lists
+// and the initial state, all in the reference's Array.h layouts.
+#pragma once
+
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "bcm_layout.h"
+
+namespace bcm {
+
+struct CaseParams {
+  int lattice[3];            // cubes along x, y, z
+  double cube_size;          // edge length of one cube
+  double origin[3];          // lower corner of the box
+  double sphere_center[3];
+  double sphere_diameter;
+  int ico_level;             // icosphere subdivision level (20 * 4^L triangles)
+  double mach, reynolds;
+  uint64_t seed;             // initial density noise seed
+  double noise_amp;          // relative density noise amplitude (1e-3)
+};
+
+// Immersed-boundary table of one list (minus or plus) in the 1-based flat
+// format of BCM_Reading_IBM.cpp:48-75: GCindex[4n+1], IPsur[4n+1], weight[8n+1].
+struct IBTable {
+  int n = 0;
+  std::vector<int> gcindex;   // size 4n+1
+  std::vector<int> ipsur;     // size 4n+1
+  std::vector<double> weight; // size 8n+1
+};
+
+// The 15 interface lists of BCM_Interface_table.cpp:20-308, 1-based.
+enum IfaceList {
+  L_X_EQ, L_Y_EQ, L_Z_EQ,
+  L_X_BS_PLUS, L_X_SB_PLUS, L_X_BS_MINUS, L_X_SB_MINUS,
+  L_Y_BS_PLUS, L_Y_SB_PLUS, L_Y_BS_MINUS, L_Y_SB_MINUS,
+  L_Z_BS_PLUS, L_Z_SB_PLUS, L_Z_BS_MINUS, L_Z_SB_MINUS,
+  L_COUNT
+};
+
+struct Triangle {
+  double n[3];      // facet normal as written to the STL
+  double v[3][3];   // vertices
+};
+
+struct Case {
+  CaseParams p;
+  Physics phys;
+  int ncube = 0;          // number of cubes + 1 (index 0 unused)
+  int n_wallcube = 0;     // number of wall cubes + 1
+  std::vector<double> cube_size, Xcube, Ycube, Zcube;   // [ncube]
+  std::vector<double> Xcnt, Ycnt, Zcnt;                  // [ncube][20]
+  std::vector<int> csl;                                  // [ncube]
+  std::vector<int> adj_number;                           // [ncube][5][7]
+  std::vector<int> wallcube;                             // [n_wallcube]
+  std::vector<int> fws;                                  // [ncube][20][20][20]
+  std::vector<int> fws_grid;   // flags as read from the grid file (before setup passes)
+  std::vector<int> nadj;       // [L_COUNT] list lengths
+  std::vector<std::vector<int>> adj;   // [L_COUNT][ncube], 1-based lists
+  std::vector<int> bc_lists[6];        // Xbc_l, Xbc_u, Ybc_l, Ybc_u, Zbc_l, Zbc_u; 1-based
+  std::vector<Triangle> stl;           // all triangles in file order
+  IBTable ib_minus, ib_plus;
+  int n_ghost_candidates = 0;          // ghost cells examined by the IB setup
+  // State arrays, AoS [ncube][20][20][20][5].
+  std::vector<double> U1_, U1, U1q;
+};
+
+// Builds the whole case.  Returns 0 on success, non-zero with a message in err.
+int build_case(const CaseParams& p, Case& cs, std::string& err);
+
+// Individual passes, exposed for tests.
+void compute_physics(const CaseParams& p, Physics& ph);
+void make_icosphere(int level, const double center[3], double radius,
+                    std::vector<Triangle>& tris);
+double noise_unit(uint64_t seed, int cube, int i, int j, int k);
+void interface_tables(int ncube, const int* csl, const int* adj_number,
+                      int* nadj, int* const* lists);
+void host_interface_copy(int ncube, const int* const* lists, const int* nadj,
+                         const int* adj_number, double* U);
+
+// Writers for the on-disk inputs the reference reads.
+int write_bcmgrid(const Case& cs, const std::string& path);
+int write_stl(const Case& cs, const std::string& path);
+
+}  // namespace bcm
