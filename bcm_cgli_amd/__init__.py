@@ -1,0 +1,215 @@
+"""MI355X-native per-iteration solver path of the BCM_CGLI Building-Cube solver.
+
+Python binding (ctypes) of ``libbcm_cgli.so``, whose C ABI is declared in
+``include/bcm_cgli.h``.  The compute path is HIP on the GPU; this module only
+moves arguments and numpy views across the boundary and raises if the native
+library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .configs import CONFIGS, case_params  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbcm_cgli.so")
+
+STATE_NAMES = ("U1_", "U1", "U1q", "U1p1", "U1p2", "Rku1", "Residual1")
+PARTS = {"bc": 0, "ghost": 1, "interface": 2, "flux": 3, "update": 4, "shift": 5, "stage": 6}
+
+
+class BCMError(RuntimeError):
+    pass
+
+
+class CaseParamsC(ctypes.Structure):
+    _fields_ = [
+        ("lattice", ctypes.c_int * 3),
+        ("cube_size", ctypes.c_double),
+        ("origin", ctypes.c_double * 3),
+        ("sphere_center", ctypes.c_double * 3),
+        ("sphere_diameter", ctypes.c_double),
+        ("ico_level", ctypes.c_int),
+        ("mach", ctypes.c_double),
+        ("reynolds", ctypes.c_double),
+        ("seed", ctypes.c_ulonglong),
+        ("noise_amp", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads the native library; raises BCMError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise BCMError(f"native library {path} is missing; run __graft_entry__.build()")
+    lib = ctypes.CDLL(path)
+    c_int, c_double, c_void_p, c_char_p = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_char_p
+    P = ctypes.POINTER
+    lib.bcm_last_error.restype = c_char_p
+    lib.bcm_case_build.argtypes = [P(CaseParamsC), P(c_void_p)]
+    lib.bcm_case_free.argtypes = [c_void_p]
+    lib.bcm_case_free.restype = None
+    lib.bcm_case_info.argtypes = [c_void_p, P(c_int), P(c_double)]
+    lib.bcm_case_array.argtypes = [c_void_p, c_char_p, P(ctypes.c_longlong)]
+    lib.bcm_case_array.restype = c_void_p
+    lib.bcm_case_write_inputs.argtypes = [c_void_p, c_char_p]
+    lib.bcm_engine_create.argtypes = [c_int, P(c_void_p)]
+    lib.bcm_engine_destroy.argtypes = [c_void_p]
+    lib.bcm_engine_destroy.restype = None
+    lib.bcm_engine_load_case.argtypes = [c_void_p, c_void_p]
+    lib.bcm_engine_set_time.argtypes = [c_void_p, c_double, c_double, c_double]
+    lib.bcm_engine_run.argtypes = [c_void_p, c_int, c_int, c_void_p, c_int]
+    lib.bcm_engine_part.argtypes = [c_void_p, c_int, c_int]
+    lib.bcm_engine_get_state.argtypes = [c_void_p, c_int, c_void_p]
+    lib.bcm_engine_set_state.argtypes = [c_void_p, c_int, c_void_p]
+    lib.bcm_engine_get_er.argtypes = [c_void_p, c_void_p]
+    lib.bcm_engine_sync.argtypes = [c_void_p]
+    lib.bcm_engine_time_part.argtypes = [c_void_p, c_int, c_int, c_int, P(c_double)]
+    lib.bcm_engine_ghost_stats.argtypes = [c_void_p, c_void_p]
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise BCMError(_lib.bcm_last_error().decode())
+
+
+class Case:
+    """A flow case built on the host in the reference's array layouts."""
+
+    def __init__(self, params):
+        lib = load_library()
+        self.params = dict(params)
+        p = CaseParamsC()
+        for d in range(3):
+            p.lattice[d] = int(params["lattice"][d])
+            p.origin[d] = float(params["origin"][d])
+            p.sphere_center[d] = float(params["sphere_center"][d])
+        p.cube_size = float(params["cube_size"])
+        p.sphere_diameter = float(params["sphere_diameter"])
+        p.ico_level = int(params["ico_level"])
+        p.mach = float(params["mach"])
+        p.reynolds = float(params["reynolds"])
+        p.seed = int(params["seed"])
+        p.noise_amp = float(params["noise_amp"])
+        h = ctypes.c_void_p()
+        _check(lib.bcm_case_build(ctypes.byref(p), ctypes.byref(h)))
+        self._h = h
+        info = (ctypes.c_int * 16)()
+        phys = (ctypes.c_double * 8)()
+        _check(lib.bcm_case_info(h, info, phys))
+        keys = ("ncube", "n_wallcube", "nbc_minus", "nbc_plus", "nXbc_l", "nXbc_u", "nYbc_l",
+                "nYbc_u", "nZbc_l", "nZbc_u", "nadjX_eq", "nadjY_eq", "nadjZ_eq", "n_triangles",
+                "n_ghost_candidates")
+        self.info = {k: info[n] for n, k in enumerate(keys)}
+        self.phys = dict(zip(("rho0", "U0", "Uin", "V0", "W0", "P0", "T0", "mu_L"), list(phys)))
+        self.ncube = self.info["ncube"]
+
+    def array(self, name):
+        """Numpy view of a case array (owned by the case)."""
+        n = ctypes.c_longlong()
+        ptr = _lib.bcm_case_array(self._h, name.encode(), ctypes.byref(n))
+        if not ptr:
+            raise BCMError(_lib.bcm_last_error().decode())
+        dbl = name in ("U1_", "U1", "U1q", "cube_size", "Xcube", "Ycube", "Zcube",
+                       "ib_minus_w", "ib_plus_w")
+        ct = ctypes.c_double if dbl else ctypes.c_int
+        buf = (ct * n.value).from_address(ptr)
+        a = np.ctypeslib.as_array(buf)
+        if name in ("U1_", "U1", "U1q"):
+            a = a.reshape(self.ncube, 20, 20, 20, 5)
+        elif name == "FWS":
+            a = a.reshape(self.ncube, 20, 20, 20)
+        elif name == "adj_number":
+            a = a.reshape(self.ncube, 5, 7)
+        return a
+
+    def write_inputs(self, directory):
+        _check(_lib.bcm_case_write_inputs(self._h, directory.encode()))
+
+    def cells(self):
+        return (self.ncube - 1) * 4096
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.bcm_case_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Engine:
+    """One device-resident case on one GPU."""
+
+    def __init__(self, device=0):
+        lib = load_library()
+        h = ctypes.c_void_p()
+        _check(lib.bcm_engine_create(int(device), ctypes.byref(h)))
+        self._h = h
+        self.case = None
+
+    def load_case(self, case):
+        _check(_lib.bcm_engine_load_case(self._h, case._h))
+        self.case = case
+
+    def set_time(self, deltaT=1e-4, deltaTau=1e-4 / 200.0, e=0.1):
+        _check(_lib.bcm_engine_set_time(self._h, deltaT, deltaTau, e))
+
+    def run(self, n_steps, n_inner, record_er=True, use_graph=True):
+        er = np.zeros((max(n_steps, 1) * n_inner, 7)) if record_er else None
+        ptr = er.ctypes.data if record_er else None
+        _check(_lib.bcm_engine_run(self._h, int(n_steps), int(n_inner), ptr, int(bool(use_graph))))
+        return er[: n_steps * n_inner] if record_er else None
+
+    def part(self, name, RK=1):
+        _check(_lib.bcm_engine_part(self._h, PARTS[name], int(RK)))
+
+    def get_state(self, name):
+        out = np.empty((self.case.ncube, 20, 20, 20, 5))
+        _check(_lib.bcm_engine_get_state(self._h, STATE_NAMES.index(name), out.ctypes.data))
+        return out
+
+    def set_state(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.size == self.case.ncube * 40000
+        _check(_lib.bcm_engine_set_state(self._h, STATE_NAMES.index(name), a.ctypes.data))
+
+    def er(self):
+        out = np.zeros(8)
+        _check(_lib.bcm_engine_get_er(self._h, out.ctypes.data))
+        return out
+
+    def sync(self):
+        _check(_lib.bcm_engine_sync(self._h))
+
+    def time_part(self, name, RK=1, reps=10):
+        ms = ctypes.c_double()
+        _check(_lib.bcm_engine_time_part(self._h, PARTS[name], int(RK), int(reps), ctypes.byref(ms)))
+        return ms.value
+
+    def ghost_stats(self):
+        s = (ctypes.c_int * 3)()
+        _check(_lib.bcm_engine_ghost_stats(self._h, s))
+        return {"entries": s[0], "cubes": s[1], "levels": s[2]}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.bcm_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

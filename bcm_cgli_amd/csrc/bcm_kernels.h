@@ -1,0 +1,51 @@
+// Device data layout and kernel launch interface of the BCM engine.
+//
+// State arrays live in HBM as structure-of-arrays per cube:
+//   [cube][var][i][j][k], 20^3 doubles per (cube, var) plane, k fastest.
+// A wavefront of 64 lanes covers four 16-cell k-rows of one i-plane, so every
+// stencil load is a coalesced row segment.  Flags are int8 [cube][i][j][k].
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bcm {
+
+constexpr int PLANE = 8000;               // 20^3 cells of one variable of one cube
+constexpr int CUBE_STRIDE = 5 * PLANE;    // doubles per cube in a state array
+
+struct DevPhys {
+  double rho0, U0, Uin, V0, W0, P0, T0, mu_L;
+};
+
+// One IB ghost-fill entry in level order (see the ghost schedule in the engine).
+struct GhostEntry {
+  int cube;
+  int16_t gi, gj, gk, pi, pj, pk;   // ghost cell and interpolation base corner
+  double w[8];
+};
+
+// Source cubes for the six face halos of a cube (0 = no equal-size neighbour).
+struct HaloSrc {
+  int src[6];   // -X, +X, -Y, +Y, -Z, +Z
+};
+
+void launch_bc(hipStream_t s, double* U, const int* cubes, const uint8_t* masks, int n,
+               DevPhys ph);
+void launch_ghost(hipStream_t s, double* U, const GhostEntry* e, const int* cube_level_off,
+                  const int* level_off, const int* cubes, int ncubes_with_ghosts, DevPhys ph);
+void launch_halo_edge(hipStream_t s, double* U, const HaloSrc* src, int ncube);
+void launch_flux(hipStream_t s, const double* U, const double* U1, const double* U1q,
+                 const int8_t* fws, const double* cube_size, double* Rku, double* Res,
+                 int ncube, DevPhys ph, double deltaT, double deltaTau, double e);
+void launch_update(hipStream_t s, int RK, double* U, double* U1p1, double* U1p2,
+                   const double* Rku, const double* Res, const int8_t* fws, int ncube,
+                   double* partials);
+void launch_reduce(hipStream_t s, const double* partials, int nblocks, double* er,
+                   int mpi_ncube, DevPhys ph);
+void launch_shift(hipStream_t s, const double* U, double* U1, double* U1q, int ncube);
+void launch_aos_to_soa(hipStream_t s, const double* aos, double* soa, int ncube);
+void launch_soa_to_aos(hipStream_t s, const double* soa, double* aos, int ncube);
+int update_blocks(int ncube);
+
+}  // namespace bcm

@@ -1,0 +1,822 @@
+// CDNA4 (gfx950) kernels of the BCM per-iteration hot path, FP64.
+//
+// Every kernel evaluates the reference expressions in the reference order and
+// the library is built with -ffp-contract=off, so results are bitwise those
+// of the reference's FP64 CPU path (checked against the oracle in tests/).
+// Citations are file:line in the reference tree; "Flux.cpp" is
+// BCM_Flux_XYZ_Viscous_Runge_kutta.cpp.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "bcm_kernels.h"
+
+namespace bcm {
+namespace {
+
+constexpr double KG = 1.4;          // K
+constexpr double KM1 = 1.4 - 1;     // (K-1), folded exactly as the reference folds it
+constexpr double RG = 287.0;        // R
+constexpr double CVG = 717.5;       // Cv
+constexpr double PRG = 0.72;        // Pr_L
+constexpr int WG = 256;
+
+__device__ __forceinline__ size_t sidx(int c, int v, int i, int j, int k) {
+  return (size_t)c * CUBE_STRIDE + (size_t)v * PLANE + i * 400 + j * 20 + k;
+}
+__device__ __forceinline__ int fidx(int c, int i, int j, int k) {
+  return c * PLANE + i * 400 + j * 20 + k;
+}
+// max(a,b) of the reference: (((a)>(b))?(a):(b)), NaN falls to b.
+__device__ __forceinline__ double rmax(double a, double b) { return (a > b) ? a : b; }
+
+// ---------------------------------------------------------------------------
+// Preconditioned Roe-type flux at one face (Flux.cpp:611-797; X :1061-1247,
+// Y :1520-1695).  D is the face-normal axis.
+template <int D>
+__device__ __forceinline__ void roe_flux(const double (&L)[5], const double (&R)[5], double e,
+                                         double (&F)[5], double& dP) {
+  const double lr = L[0];
+  const double lq0 = L[1] / lr, lq1 = L[2] / lr, lq2 = L[3] / lr;
+  const double lVV = lq0 * lq0 + lq1 * lq1 + lq2 * lq2;
+  const double lP = (L[4] - 0.5 * lr * lVV) * KM1;
+  const double lC = KG * lP / lr;
+  const double lH = 0.5 * lVV + lC / KM1;
+  const double rr = R[0];
+  const double rq0 = R[1] / rr, rq1 = R[2] / rr, rq2 = R[3] / rr;
+  const double rVV = rq0 * rq0 + rq1 * rq1 + rq2 * rq2;
+  const double rP = (R[4] - 0.5 * rr * rVV) * KM1;
+  const double rC = KG * rP / rr;
+  const double rH = 0.5 * rVV + rC / KM1;
+
+  const double t5 = sqrt(lr), t6 = sqrt(rr), t4 = t5 + t6;
+  const double rho = sqrt(lr * rr);
+  const double q0 = (t5 * lq0 + t6 * rq0) / t4;
+  const double q1 = (t5 * lq1 + t6 * rq1) / t4;
+  const double q2 = (t5 * lq2 + t6 * rq2) / t4;
+  const double VV = q0 * q0 + q1 * q1 + q2 * q2;
+  const double H = (t5 * lH + t6 * rH) / t4;
+  const double C = (H - 0.5 * VV) * KM1;
+  const double beta = rmax(VV / C, e);
+  const double un = D == 0 ? q0 : (D == 1 ? q1 : q2);
+  const double lun = D == 0 ? lq0 : (D == 1 ? lq1 : lq2);
+  const double run = D == 0 ? rq0 : (D == 1 ? rq1 : rq2);
+  const double temp = 0.5 * (1 + beta) * un;
+  const double S = 0.5 * sqrt(4 * beta * C + un * un * (1 - beta) * (1 - beta));
+  const double temp1 = (rP - lP) / rho / beta / C;
+  const double tS = temp / S;
+  const double temp2 = tS * (run - lun);
+  const double temp3 = 0.5 * (1 - beta) * un * temp / S;
+  const double an = fabs(un);
+  const double deltaU = (S - temp3 - beta * an) * temp1 + temp2;
+  const double deltaP = tS * (rP - lP) + (S - an + temp3) * rho * (run - lun);
+
+  // Dissipation and the averaged physical flux; Ep = 1 multiplies exactly.
+  const double fav0 = an * (R[0] - L[0]) + deltaU * rho;
+  double fav1 = an * (R[1] - L[1]) + deltaU * rho * q0;
+  double fav2 = an * (R[2] - L[2]) + deltaU * rho * q1;
+  double fav3 = an * (R[3] - L[3]) + deltaU * rho * q2;
+  if (D == 0) fav1 = fav1 + deltaP;
+  if (D == 1) fav2 = fav2 + deltaP;
+  if (D == 2) fav3 = fav3 + deltaP;
+  const double fav4 = an * (R[4] - L[4]) + deltaU * rho * H + deltaP * un;
+  F[0] = 0.5 * ((lr * lun + rr * run) - fav0);
+  F[1] = D == 0 ? 0.5 * ((lr * lq0 * lq0 + lP + rr * rq0 * rq0 + rP) - fav1)
+                : 0.5 * ((lr * lq0 * lun + rr * rq0 * run) - fav1);
+  F[2] = D == 1 ? 0.5 * ((lr * lq1 * lq1 + lP + rr * rq1 * rq1 + rP) - fav2)
+                : 0.5 * ((lr * lq1 * lun + rr * rq1 * run) - fav2);
+  F[3] = D == 2 ? 0.5 * ((lr * lq2 * lq2 + lP + rr * rq2 * rq2 + rP) - fav3)
+                : 0.5 * ((lr * lq2 * lun + rr * rq2 * run) - fav3);
+  F[4] = 0.5 * ((lun * (3.5 * lP + 0.5 * lr * lVV) + run * (3.5 * rP + 0.5 * rr * rVV)) - fav4);
+  dP = 0.5 * (lP + rP);
+}
+
+// Inviscid flux difference of one cell along axis D (Flux.cpp:367-806 for Z):
+// upwind-biased face states of 5th, 3rd or 1st order depending on the cell
+// position and the neighbour flags, then the two face fluxes.
+template <int D>
+__device__ __forceinline__ void axis_flux(const double* __restrict__ U,
+                                          const int8_t* __restrict__ fws, int c, int i, int j,
+                                          int k, double e, double (&Rf)[5], double& dPb,
+                                          double& dPf) {
+  constexpr int st = D == 0 ? 400 : (D == 1 ? 20 : 1);
+  const int pos = D == 0 ? i : (D == 1 ? j : k);
+  const size_t b = sidx(c, 0, i, j, k);
+  const int8_t* fb = fws + fidx(c, i, j, k);
+  const int fm2 = fb[-2 * st], fm1 = fb[-st], fp1 = fb[st], fp2 = fb[2 * st];
+  double s[7][5];
+#pragma unroll
+  for (int o = -2; o <= 2; ++o)
+#pragma unroll
+    for (int v = 0; v < 5; ++v) s[o + 3][v] = U[b + (size_t)v * PLANE + o * st];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    s[0][v] = pos > 2 ? U[b + (size_t)v * PLANE - 3 * st] : 0.0;
+    s[6][v] = pos < 17 ? U[b + (size_t)v * PLANE + 3 * st] : 0.0;
+  }
+  const double c6 = 1. / 6, c60 = 1. / 60;
+  double ML[5], MR[5], MLi[5], MRi[5];
+  if (fp1 == 1 && fm1 == 1) {
+    if (pos == 2 || fm2 != 1) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        ML[v] = c6 * (-s[1][v] + 5 * s[2][v] + 2 * s[3][v]);
+        MR[v] = c6 * (2 * s[2][v] + 5 * s[3][v] - s[4][v]);
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        ML[v] = c60 * (2 * s[0][v] - 13 * s[1][v] + 47 * s[2][v] + 27 * s[3][v] - 3 * s[4][v]);
+        MR[v] = c60 * (-3 * s[1][v] + 27 * s[2][v] + 47 * s[3][v] - 13 * s[4][v] + 2 * s[5][v]);
+      }
+    }
+    if (pos == 17 || fp2 != 1) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        MLi[v] = c6 * (-s[2][v] + 5 * s[3][v] + 2 * s[4][v]);
+        MRi[v] = c6 * (2 * s[3][v] + 5 * s[4][v] - s[5][v]);
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        MLi[v] = c60 * (2 * s[1][v] - 13 * s[2][v] + 47 * s[3][v] + 27 * s[4][v] - 3 * s[5][v]);
+        MRi[v] = c60 * (-3 * s[2][v] + 27 * s[3][v] + 47 * s[4][v] - 13 * s[5][v] + 2 * s[6][v]);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      ML[v] = s[2][v];
+      MR[v] = s[3][v];
+      MLi[v] = s[3][v];
+      MRi[v] = s[4][v];
+    }
+  }
+  double Fb[5], Ff[5];
+  roe_flux<D>(ML, MR, e, Fb, dPb);
+  roe_flux<D>(MLi, MRi, e, Ff, dPf);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) Rf[v] = Ff[v] - Fb[v];
+}
+
+struct Prim {
+  double u, v, w, T;
+};
+
+// Neighbour primitives as the viscous block computes them (T = P/(rho*R)).
+__device__ __forceinline__ Prim nb_prim(const double* __restrict__ U, size_t b) {
+  const double r = U[b];
+  Prim p;
+  p.u = U[b + PLANE] / r;
+  p.v = U[b + 2 * PLANE] / r;
+  p.w = U[b + 3 * PLANE] / r;
+  const double P = (U[b + 4 * PLANE] - 0.5 * r * (p.u * p.u + p.v * p.v + p.w * p.w)) * KM1;
+  p.T = P / (r * RG);
+  return p;
+}
+
+// Slot of a neighbour offset (a,b,c) in {-1,0,1}^3 in a 27-entry table.
+__device__ __forceinline__ constexpr int nslot(int a, int b, int c) {
+  return (a + 1) * 9 + (b + 1) * 3 + (c + 1);
+}
+
+// Viscous flux through the face of axis D on side SIDE (-1 backward,
+// +1 forward), Flux.cpp:1946-2272.
+template <int D, int SIDE>
+__device__ __forceinline__ void visc_face(const Prim (&q)[27], const Prim& me, double T0,
+                                          double mu_L, double inv, double (&L)[5]) {
+  constexpr int o0 = D == 0 ? SIDE : 0, o1 = D == 1 ? SIDE : 0, o2 = D == 2 ? SIDE : 0;
+  const Prim f = q[nslot(o0, o1, o2)];
+  const double Tf = 0.5 * (me.T + f.T);
+  const double temp = Tf / T0;
+  const double mu = mu_L * temp * sqrt(temp) * (T0 + 110.) / (Tf + 110.);
+  double g[3][4];
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    if (t == D) {
+      if (SIDE < 0) {
+        g[t][0] = (me.u - f.u) * inv;
+        g[t][1] = (me.v - f.v) * inv;
+        g[t][2] = (me.w - f.w) * inv;
+        g[t][3] = (me.T - f.T) * inv;
+      } else {
+        g[t][0] = (f.u - me.u) * inv;
+        g[t][1] = (f.v - me.v) * inv;
+        g[t][2] = (f.w - me.w) * inv;
+        g[t][3] = (f.T - me.T) * inv;
+      }
+    } else {
+      const int p0 = t == 0 ? 1 : 0, p1 = t == 1 ? 1 : 0, p2 = t == 2 ? 1 : 0;
+      const Prim& A = q[nslot(p0, p1, p2)];
+      const Prim& B = q[nslot(p0 + o0, p1 + o1, p2 + o2)];
+      const Prim& Cm = q[nslot(-p0, -p1, -p2)];
+      const Prim& Dm = q[nslot(-p0 + o0, -p1 + o1, -p2 + o2)];
+      g[t][0] = (A.u + B.u - Cm.u - Dm.u) / 4 * inv;
+      g[t][1] = (A.v + B.v - Cm.v - Dm.v) / 4 * inv;
+      g[t][2] = (A.w + B.w - Cm.w - Dm.w) / 4 * inv;
+      g[t][3] = (A.T + B.T - Cm.T - Dm.T) / 4 * inv;
+    }
+  }
+  const double uf = 0.5 * (me.u + f.u), vf = 0.5 * (me.v + f.v), wf = 0.5 * (me.w + f.w);
+  const double div = g[0][0] + g[1][1] + g[2][2];
+  double sg[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m)
+    sg[m] = (m == D) ? mu * (2 * g[D][D] - 2.0 / 3.0 * div) : mu * (g[m][D] + g[D][m]);
+  L[1] = sg[0];
+  L[2] = sg[1];
+  L[3] = sg[2];
+  L[4] = uf * sg[0] + vf * sg[1] + wf * sg[2] + mu * CVG * KG * g[D][3] / (PRG);
+}
+
+// ---------------------------------------------------------------------------
+// Flux, residual and preconditioned stage increment of one interior cell
+// (Flux.cpp:331-2398).  Grid: (16 i-planes, cubes), 256 lanes = 16 j x 16 k.
+__global__ __launch_bounds__(WG) void k_flux(const double* __restrict__ U,
+                                             const double* __restrict__ U1,
+                                             const double* __restrict__ U1q,
+                                             const int8_t* __restrict__ fws,
+                                             const double* __restrict__ cube_size,
+                                             double* __restrict__ Rku, double* __restrict__ Res,
+                                             DevPhys ph, double deltaT, double deltaTau,
+                                             double e) {
+  const int c = blockIdx.y + 1;
+  const int i = blockIdx.x + 2;
+  const int j = (threadIdx.x >> 4) + 2;
+  const int k = (threadIdx.x & 15) + 2;
+  const double dx = cube_size[c] / 16;
+  const double dy = dx, dz = dx;
+  const double inv = 1. / dx;
+  const size_t b = sidx(c, 0, i, j, k);
+  const int IF = fws[fidx(c, i, j, k)];
+
+  double u[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) u[v] = U[b + (size_t)v * PLANE];
+
+  // Inviscid part.
+  double Rx[5], Ry[5], Rz[5], dPxb, dPxf, dPyb, dPyf, dPzb, dPzf;
+  axis_flux<2>(U, fws, c, i, j, k, e, Rz, dPzb, dPzf);
+  axis_flux<0>(U, fws, c, i, j, k, e, Rx, dPxb, dPxf);
+  axis_flux<1>(U, fws, c, i, j, k, e, Ry, dPyb, dPyf);
+  double Rf[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) Rf[v] = -(Rx[v] / dx + Ry[v] / dy + Rz[v] / dz);
+
+  // Viscous part: primitives of the 6 face and 12 edge neighbours.
+  Prim q[27];
+#pragma unroll
+  for (int a = -1; a <= 1; ++a)
+#pragma unroll
+    for (int bb = -1; bb <= 1; ++bb)
+#pragma unroll
+      for (int cc = -1; cc <= 1; ++cc) {
+        const int nz = (a != 0) + (bb != 0) + (cc != 0);
+        if (nz == 1 || nz == 2) q[nslot(a, bb, cc)] = nb_prim(U, b + a * 400 + bb * 20 + cc);
+      }
+  Prim me;
+  const double rho = u[0];
+  me.u = u[1] / u[0];
+  me.v = u[2] / u[0];
+  me.w = u[3] / u[0];
+  {
+    const double VV = me.u * me.u + me.v * me.v + me.w * me.w;
+    const double P = (u[4] - 0.5 * rho * VV) * KM1;
+    me.T = P / rho / RG;
+  }
+  double Lxb[5], Lxf[5], Lyb[5], Lyf[5], Lzb[5], Lzf[5];
+  visc_face<0, -1>(q, me, ph.T0, ph.mu_L, inv, Lxb);
+  visc_face<0, +1>(q, me, ph.T0, ph.mu_L, inv, Lxf);
+  visc_face<1, -1>(q, me, ph.T0, ph.mu_L, inv, Lyb);
+  visc_face<1, +1>(q, me, ph.T0, ph.mu_L, inv, Lyf);
+  visc_face<2, -1>(q, me, ph.T0, ph.mu_L, inv, Lzb);
+  visc_face<2, +1>(q, me, ph.T0, ph.mu_L, inv, Lzf);
+  double vF[5];
+#pragma unroll
+  for (int v = 1; v < 5; ++v)
+    vF[v] = (Lxf[v] - Lxb[v]) * inv + (Lyf[v] - Lyb[v]) * inv + (Lzf[v] - Lzb[v]) * inv;
+
+  // Residual with the dual-time source.
+  double Rk[5];
+  {
+    const double uu0 = U1[b], uq0 = U1q[b];
+    Rk[0] = -(3 * u[0] - 4 * uu0 + uq0) / (2 * deltaT) + Rf[0];
+  }
+#pragma unroll
+  for (int v = 1; v < 5; ++v) {
+    const double uu = U1[b + (size_t)v * PLANE], uq = U1q[b + (size_t)v * PLANE];
+    const double Rp = -(3 * u[v] - 4 * uu + uq) / (2 * deltaT);
+    Rk[v] = Rp + Rf[v] + vF[v];
+  }
+
+  // Closed-form preconditioner D = M (Gamma + 3 dtau/(2 dT) M)^-1 (Flux.cpp:2295-2343).
+  const double U_ = u[1] / u[0], V_ = u[2] / u[0], W_ = u[3] / u[0];
+  const double VV = U_ * U_ + V_ * V_ + W_ * W_;
+  const double P = (u[4] - 0.5 * rho * VV) * KM1;
+  const double C = KG * P / rho;
+  const double T = P / rho;
+  const double H = 0.5 * VV + C / KM1;
+  const double beta = rmax(VV / C, e);
+  const double temp = KG * T * (2 * deltaT + 3 * deltaTau) * (2 * deltaT + 3 * beta * deltaTau);
+  const double temp2 = KM1 * (beta - 1) * deltaT * deltaT;
+  const double temp3 = H - H * KG - VV + KG * (T + VV);
+  double out[5];
+  if (IF == 1) {
+    const double d11 = (2 * deltaT * (-2 * (-KM1 * (H - VV) - temp3 * beta) * deltaT + 3 * KG * T * beta * deltaTau)) / temp;
+    const double d12 = (-4 * U_ * temp2) / temp;
+    const double d13 = (-4 * V_ * temp2) / temp;
+    const double d14 = (-4 * W_ * temp2) / temp;
+    const double d15 = (4 * temp2) / temp;
+    out[0] = deltaTau * (d11 * Rk[0] + d12 * Rk[1] + d13 * Rk[2] + d14 * Rk[3] + d15 * Rk[4]);
+    const double d21 = (4 * U_ * temp3 * (beta - 1) * deltaT * deltaT) / temp;
+    const double d22 = (2 * deltaT * (2 * KG * (T - U_ * U_ * (beta - 1)) * deltaT + 2 * U_ * U_ * (beta - 1) * deltaT + 3 * KG * T * beta * deltaTau)) / temp;
+    const double d23 = (-4 * U_ * V_ * temp2) / temp;
+    const double d24 = (-4 * U_ * W_ * temp2) / temp;
+    const double d25 = (4 * U_ * temp2) / temp;
+    out[1] = deltaTau * (d21 * Rk[0] + d22 * Rk[1] + d23 * Rk[2] + d24 * Rk[3] + d25 * Rk[4]);
+    const double d31 = (4 * V_ * temp3 * (beta - 1) * deltaT * deltaT) / temp;
+    const double d33 = (2 * deltaT * (2 * KG * (T - V_ * V_ * (beta - 1)) * deltaT + 2 * V_ * V_ * (beta - 1) * deltaT + 3 * KG * T * beta * deltaTau)) / temp;
+    const double d34 = (-4 * V_ * W_ * temp2) / temp;
+    const double d35 = (4 * V_ * temp2) / temp;
+    out[2] = deltaTau * (d31 * Rk[0] + d23 * Rk[1] + d33 * Rk[2] + d34 * Rk[3] + d35 * Rk[4]);
+    const double d41 = (4 * W_ * temp3 * (beta - 1) * deltaT * deltaT) / temp;
+    const double d44 = (2 * deltaT * (2 * KG * (T - W_ * W_ * (beta - 1)) * deltaT + 2 * W_ * W_ * (beta - 1) * deltaT + 3 * KG * T * beta * deltaTau)) / temp;
+    const double d45 = (4 * W_ * temp2) / temp;
+    out[3] = deltaTau * (d41 * Rk[0] + d24 * Rk[1] + d34 * Rk[2] + d44 * Rk[3] + d45 * Rk[4]);
+    const double d51 = (4 * H * temp3 * (beta - 1) * deltaT * deltaT) / temp;
+    const double d52 = (-4 * H * U_ * temp2) / temp;
+    const double d53 = (-4 * H * V_ * temp2) / temp;
+    const double d54 = (-4 * H * W_ * temp2) / temp;
+    const double d55 = (2 * deltaT * (2 * H * KM1 * (beta - 1) * deltaT + KG * T * (2 * deltaT + 3 * beta * deltaTau))) / temp;
+    out[4] = deltaTau * (d51 * Rk[0] + d52 * Rk[1] + d53 * Rk[2] + d54 * Rk[3] + d55 * Rk[4]);
+  } else {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) out[v] = 0;
+  }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) Rku[b + (size_t)v * PLANE] = out[v];
+  if (IF < 1) {
+    Res[b] = -(-(dPxf - dPxb) / dx + vF[1]) * dx * dy * dz;
+    Res[b + PLANE] = -(-(dPzf - dPzb) / dz + vF[3]) * dx * dy * dz;
+  }
+}
+
+// Block sum of NS doubles per lane in a fixed tree order; lane 0 returns the totals.
+template <int NS>
+__device__ __forceinline__ void block_sum(double (&x)[NS], double* sh) {
+#pragma unroll
+  for (int m = 0; m < NS; ++m)
+    for (int off = 32; off > 0; off >>= 1) x[m] += __shfl_down(x[m], off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int m = 0; m < NS; ++m) sh[wave * NS + m] = x[m];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+      double s = sh[m];
+      for (int w = 1; w < nw; ++w) s += sh[w * NS + m];
+      x[m] = s;
+    }
+  }
+}
+
+// RK stage update of fluid cells (Flux.cpp:2417-2599).  At RK3 each block
+// also writes its partial sums of the five change norms (:2530-2566) and of
+// the two force-density components (:2603-2625).
+__global__ __launch_bounds__(WG) void k_update(int RK, double* __restrict__ U,
+                                               double* __restrict__ U1p1,
+                                               double* __restrict__ U1p2,
+                                               const double* __restrict__ Rku,
+                                               const double* __restrict__ Res,
+                                               const int8_t* __restrict__ fws,
+                                               double* __restrict__ partials) {
+  const int c = blockIdx.y + 1;
+  const int i = blockIdx.x + 2;
+  const int j = (threadIdx.x >> 4) + 2;
+  const int k = (threadIdx.x & 15) + 2;
+  const size_t b = sidx(c, 0, i, j, k);
+  const bool fluid = fws[fidx(c, i, j, k)] == 1;
+  if (RK == 1) {
+    if (fluid)
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        const double x = U[b + (size_t)v * PLANE] + Rku[b + (size_t)v * PLANE];
+        U[b + (size_t)v * PLANE] = x;
+        U1p1[b + (size_t)v * PLANE] = x;
+      }
+    return;
+  }
+  if (RK == 2) {
+    if (fluid)
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        const size_t o = b + (size_t)v * PLANE;
+        const double x = 0.75 * U[o] + 0.25 * U1p1[o] + 0.25 * Rku[o];
+        U[o] = x;
+        U1p2[o] = x;
+      }
+    return;
+  }
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (fluid) {
+    double u[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) u[v] = U[b + (size_t)v * PLANE];
+    const double ro = u[0];
+    const double uo = u[1] / ro, vo = u[2] / ro, wo = u[3] / ro;
+    const double VVo = uo * uo + vo * vo + wo * wo;
+    const double Po = (u[4] - 0.5 * ro * VVo) * KM1;
+    const double To = Po / ro;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const size_t o = b + (size_t)v * PLANE;
+      u[v] = 1.0 / 3.0 * u[v] + 2.0 / 3.0 * U1p2[o] + 2.0 / 3.0 * Rku[o];
+      U[o] = u[v];
+    }
+    const double r = u[0];
+    const double un = u[1] / r, vn = u[2] / r, wn = u[3] / r;
+    const double VV = un * un + vn * vn + wn * wn;
+    const double P = (u[4] - 0.5 * r * VV) * KM1;
+    const double T = P / r;
+    acc[0] = (P - Po) * (P - Po);
+    acc[1] = (un - uo) * (un - uo);
+    acc[2] = (vn - vo) * (vn - vo);
+    acc[3] = (wn - wo) * (wn - wo);
+    acc[4] = (T - To) * (T - To);
+  }
+  acc[5] = Res[b];
+  acc[6] = Res[b + PLANE];
+  __shared__ double sh[(WG / 64) * 7];
+  block_sum<7>(acc, sh);
+  if (threadIdx.x == 0) {
+    const size_t blk = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+#pragma unroll
+    for (int m = 0; m < 7; ++m) partials[blk * 8 + m] = acc[m];
+  }
+}
+
+// Final norm reduction (Flux.cpp:2627-2652 at one rank): er[1..7].
+__global__ __launch_bounds__(WG) void k_reduce(const double* __restrict__ partials, int nblocks,
+                                               double* __restrict__ er, int mpi_ncube,
+                                               DevPhys ph) {
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int bi = threadIdx.x; bi < nblocks; bi += WG)
+#pragma unroll
+    for (int m = 0; m < 7; ++m) acc[m] += partials[(size_t)bi * 8 + m];
+  __shared__ double sh[(WG / 64) * 7];
+  block_sum<7>(acc, sh);
+  if (threadIdx.x == 0) {
+    const double DN = 1. / (mpi_ncube * 16 * 16 * 16);
+    er[0] = 0;
+    for (int m = 0; m < 5; ++m) er[m + 1] = sqrt(acc[m]) * DN;
+    er[6] = acc[5] / (0.5 * ph.rho0 * ph.U0 * ph.U0 * 2.602);
+    er[7] = acc[6] / (0.5 * ph.rho0 * ph.U0 * ph.U0 * 2.602);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Domain boundary conditions of one cube per block: X (BCM_X_boundary_condition
+// .cpp:43-118), then Y (BCM_Y_boundary_condition.cpp:36-80), then Z
+// (BCM_Z_boundary_condition.cpp:42-92).  mask bits: 1 x-min, 2 x-max, 4 y-min,
+// 8 y-max, 16 z-min, 32 z-max.
+__global__ __launch_bounds__(WG) void k_bc(double* __restrict__ U, const int* __restrict__ cubes,
+                                           const uint8_t* __restrict__ masks, DevPhys ph) {
+  const int c = cubes[blockIdx.x];
+  const int m = masks[blockIdx.x];
+  const int t = threadIdx.x;
+  {
+    const int j = (t >> 4) + 2, k = (t & 15) + 2;
+    if (m & 1) {
+      const double rho = U[sidx(c, 0, 2, j, k)];
+      const double uu = U[sidx(c, 1, 2, j, k)] / rho;
+      const double vv = U[sidx(c, 2, 2, j, k)] / rho;
+      const double ww = U[sidx(c, 3, 2, j, k)] / rho;
+      double VV = uu * uu + vv * vv + ww * ww;
+      const double P = (U[sidx(c, 4, 2, j, k)] - 0.5 * rho * VV) * KM1;
+      VV = ph.Uin * ph.Uin + ph.V0 * ph.V0 + ph.W0 * ph.W0;
+      const double E = P / KM1 + 0.5 * rho * VV;
+      for (int i = 0; i < 2; ++i) {
+        U[sidx(c, 0, i, j, k)] = rho;
+        U[sidx(c, 1, i, j, k)] = rho * ph.Uin;
+        U[sidx(c, 2, i, j, k)] = rho * ph.V0;
+        U[sidx(c, 3, i, j, k)] = rho * ph.W0;
+        U[sidx(c, 4, i, j, k)] = E;
+      }
+    }
+    if (m & 2)
+      for (int v = 0; v < 5; ++v) {
+        const double x = U[sidx(c, v, 17, j, k)];
+        U[sidx(c, v, 18, j, k)] = x;
+        U[sidx(c, v, 19, j, k)] = x;
+      }
+  }
+  __syncthreads();
+  for (int w = t; w < 20 * 16; w += WG) {
+    const int i = w >> 4, k = (w & 15) + 2;
+    if (m & 4)
+      for (int v = 0; v < 5; ++v) {
+        const double x = U[sidx(c, v, i, 2, k)];
+        U[sidx(c, v, i, 1, k)] = x;
+        U[sidx(c, v, i, 0, k)] = x;
+      }
+    if (m & 8)
+      for (int v = 0; v < 5; ++v) {
+        const double x = U[sidx(c, v, i, 17, k)];
+        U[sidx(c, v, i, 18, k)] = x;
+        U[sidx(c, v, i, 19, k)] = x;
+      }
+  }
+  __syncthreads();
+  for (int w = t; w < 400; w += WG) {
+    const int i = w / 20, j = w % 20;
+    if (m & 16)
+      for (int v = 0; v < 5; ++v) {
+        const double x = U[sidx(c, v, i, j, 2)];
+        const double y = v == 3 ? -x : x;
+        U[sidx(c, v, i, j, 1)] = y;
+        U[sidx(c, v, i, j, 0)] = y;
+      }
+    if (m & 32)
+      for (int v = 0; v < 5; ++v) {
+        const double x = U[sidx(c, v, i, j, 17)];
+        U[sidx(c, v, i, j, 18)] = x;
+        U[sidx(c, v, i, j, 19)] = x;
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Immersed-boundary ghost fill (BCM_Ghostcell_minus.cpp:72-489).  The
+// reference walks the minus list then the plus list in order, writing in
+// place; entries here come grouped per cube into dependency levels built on
+// the host (an entry's level is above every earlier entry it reads from or
+// whose corners it overwrites), so all entries of one level run in parallel
+// and levels are separated by workgroup barriers.  Interpolation corners
+// always lie in the ghost cell's own padded cube.
+__global__ __launch_bounds__(WG) void k_ghost(double* __restrict__ U,
+                                              const GhostEntry* __restrict__ ent,
+                                              const int* __restrict__ cube_level_off,
+                                              const int* __restrict__ level_off, DevPhys ph) {
+  const int l0 = cube_level_off[blockIdx.x], l1 = cube_level_off[blockIdx.x + 1];
+  const double rho0 = ph.rho0, P0 = ph.P0;
+  for (int lv = l0; lv < l1; ++lv) {
+    const int e0 = level_off[lv], e1 = level_off[lv + 1];
+    for (int n = e0 + threadIdx.x; n < e1; n += WG) {
+      const GhostEntry& E = ent[n];
+      const int c = E.cube;
+      double rc[8], uc[8], vc[8], wc[8], pc[8];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const int a = (m == 1 || m == 4 || m == 5 || m == 7);
+        const int bq = (m == 2 || m == 4 || m == 6 || m == 7);
+        const int cq = (m == 3 || m == 5 || m == 6 || m == 7);
+        const size_t o = sidx(c, 0, E.pi + a, E.pj + bq, E.pk + cq);
+        rc[m] = U[o];
+        uc[m] = U[o + PLANE] / rc[m];
+        vc[m] = U[o + 2 * PLANE] / rc[m];
+        wc[m] = U[o + 3 * PLANE] / rc[m];
+        const double VV = uc[m] * uc[m] + vc[m] * vc[m] + wc[m] * wc[m];
+        pc[m] = (U[o + 4 * PLANE] - 0.5 * rc[m] * VV) * KM1;
+      }
+      double rho = E.w[0] * (rc[0] - rho0), P = E.w[0] * (pc[0] - P0);
+      double uu = E.w[0] * uc[0], vv = E.w[0] * vc[0], ww = E.w[0] * wc[0];
+#pragma unroll
+      for (int m = 1; m < 8; ++m) {
+        rho = rho + E.w[m] * (rc[m] - rho0);
+        P = P + E.w[m] * (pc[m] - P0);
+        uu = uu + E.w[m] * uc[m];
+        vv = vv + E.w[m] * vc[m];
+        ww = ww + E.w[m] * wc[m];
+      }
+      rho = rho + rho0;
+      P = P + P0;
+      double VV = uu * uu + vv * vv + ww * ww;
+      double o0 = rho, o1 = 0.5 * rho * uu, o2 = 0.5 * rho * vv, o3 = 0.5 * rho * ww;
+      double o4 = P / KM1 + 0.5 * rho * 0.25 * VV;
+      // The ghost cell may itself be an interpolation corner (:192-449).
+      const int di = E.gi - E.pi, dj = E.gj - E.pj, dk = E.gk - E.pk;
+      if (di >= 0 && di <= 1 && dj >= 0 && dj <= 1 && dk >= 0 && dk <= 1) {
+        const int s = di && dj && dk ? 7
+                      : (dj && dk ? 6 : (di && dk ? 5 : (di && dj ? 4 : (dk ? 3 : (dj ? 2 : (di ? 1 : 0))))));
+        const double ws = E.w[s];
+        if (ws > 0.9999) {
+          rho = rc[s];
+          P = pc[s];
+        } else {
+          double sr = 0, sp = 0;
+          bool first = true;
+#pragma unroll
+          for (int m = 0; m < 8; ++m) {
+            if (m == s) continue;
+            if (first) {
+              sr = E.w[m] * (rc[m] - rho0);
+              sp = E.w[m] * (pc[m] - P0);
+              first = false;
+            } else {
+              sr = sr + E.w[m] * (rc[m] - rho0);
+              sp = sp + E.w[m] * (pc[m] - P0);
+            }
+          }
+          rho = sr / (1 - ws) + rho0;
+          P = sp / (1 - ws) + P0;
+        }
+        double su = 0, sv = 0, sw = 0;
+        bool first = true;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+          if (m == s) continue;
+          if (first) {
+            su = E.w[m] * uc[m];
+            sv = E.w[m] * vc[m];
+            sw = E.w[m] * wc[m];
+            first = false;
+          } else {
+            su = su + E.w[m] * uc[m];
+            sv = sv + E.w[m] * vc[m];
+            sw = sw + E.w[m] * wc[m];
+          }
+        }
+        uu = su / (2 - ws);
+        vv = sv / (2 - ws);
+        ww = sw / (2 - ws);
+        VV = uu * uu + vv * vv + ww * ww;
+        o0 = rho;
+        o1 = rho * uu;
+        o2 = rho * vv;
+        o3 = rho * ww;
+        o4 = P / KM1 + 0.5 * rho * VV;
+      }
+      const size_t g = sidx(c, 0, E.gi, E.gj, E.gk);
+      U[g] = o0;
+      U[g + PLANE] = o1;
+      U[g + 2 * PLANE] = o2;
+      U[g + 3 * PLANE] = o3;
+      U[g + 4 * PLANE] = o4;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Cube-to-cube halo of equal-size neighbours as a gather into each cube's own
+// face layers (Interface.cpp:1745-1782, 2604-2641, 3474-3511), then the
+// cube's edge-ghost averaging in the reference's three sequential passes
+// (:5640-5732).  Sources are interior layers, which nothing in this kernel
+// writes, so cubes proceed independently.
+__global__ __launch_bounds__(WG) void k_halo_edge(double* __restrict__ U,
+                                                  const HaloSrc* __restrict__ srcs) {
+  const int c = blockIdx.x + 1;
+  const HaloSrc hs = srcs[c];
+  const int t = threadIdx.x;
+  const int p = (t >> 4) + 2, r = (t & 15) + 2;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    const int sc = hs.src[f];
+    if (sc <= 0) continue;
+    const int ax = f >> 1;
+    const bool plus = f & 1;
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+      const int dl = plus ? 18 + l : l;      // destination layer
+      const int sl = plus ? 2 + l : 16 + l;  // source layer
+      const int di = ax == 0 ? dl : p, dj = ax == 1 ? dl : (ax == 0 ? p : r),
+                dk = ax == 2 ? dl : r;
+      const int si = ax == 0 ? sl : p, sj = ax == 1 ? sl : (ax == 0 ? p : r),
+                sk = ax == 2 ? sl : r;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) U[sidx(c, v, di, dj, dk)] = U[sidx(sc, v, si, sj, sk)];
+    }
+  }
+  __syncthreads();
+  // i-edges at (j,k) in {1,18}^2, i = 2..17.
+  for (int w = t; w < 16 * 4 * 5; w += WG) {
+    const int i = (w >> 2) / 5 + 2, e = w & 3, v = (w >> 2) % 5;
+    double x;
+    if (e == 0) x = 0.5 * (U[sidx(c, v, i, 1, 2)] + U[sidx(c, v, i, 2, 1)]);
+    else if (e == 1) x = 0.5 * (U[sidx(c, v, i, 18, 2)] + U[sidx(c, v, i, 17, 1)]);
+    else if (e == 2) x = 0.5 * (U[sidx(c, v, i, 1, 17)] + U[sidx(c, v, i, 2, 18)]);
+    else x = 0.5 * (U[sidx(c, v, i, 18, 17)] + U[sidx(c, v, i, 17, 18)]);
+    const int jj = (e == 1 || e == 3) ? 18 : 1, kk = (e >= 2) ? 18 : 1;
+    U[sidx(c, v, i, jj, kk)] = x;
+  }
+  __syncthreads();
+  // j-edges at (i,k) in {1,18}^2, j = 1..18.
+  for (int w = t; w < 18 * 4 * 5; w += WG) {
+    const int j = (w >> 2) / 5 + 1, e = w & 3, v = (w >> 2) % 5;
+    double x;
+    if (e == 0) x = 0.5 * (U[sidx(c, v, 1, j, 2)] + U[sidx(c, v, 2, j, 1)]);
+    else if (e == 1) x = 0.5 * (U[sidx(c, v, 2, j, 18)] + U[sidx(c, v, 1, j, 17)]);
+    else if (e == 2) x = 0.5 * (U[sidx(c, v, 17, j, 1)] + U[sidx(c, v, 18, j, 2)]);
+    else x = 0.5 * (U[sidx(c, v, 17, j, 18)] + U[sidx(c, v, 18, j, 17)]);
+    const int ii = (e >= 2) ? 18 : 1, kk = (e == 1 || e == 3) ? 18 : 1;
+    U[sidx(c, v, ii, j, kk)] = x;
+  }
+  __syncthreads();
+  // k-edges at (i,j) in {1,18}^2, k = 1..18.
+  for (int w = t; w < 18 * 4 * 5; w += WG) {
+    const int k = (w >> 2) / 5 + 1, e = w & 3, v = (w >> 2) % 5;
+    double x;
+    if (e == 0) x = 0.5 * (U[sidx(c, v, 1, 2, k)] + U[sidx(c, v, 2, 1, k)]);
+    else if (e == 1) x = 0.5 * (U[sidx(c, v, 2, 18, k)] + U[sidx(c, v, 1, 17, k)]);
+    else if (e == 2) x = 0.5 * (U[sidx(c, v, 17, 1, k)] + U[sidx(c, v, 18, 2, k)]);
+    else x = 0.5 * (U[sidx(c, v, 17, 18, k)] + U[sidx(c, v, 18, 17, k)]);
+    const int ii = (e >= 2) ? 18 : 1, jj = (e == 1 || e == 3) ? 18 : 1;
+    U[sidx(c, v, ii, jj, k)] = x;
+  }
+}
+
+// Physical-time shift of interior cells (main.cpp:1292-1315).
+__global__ __launch_bounds__(WG) void k_shift(const double* __restrict__ U,
+                                              double* __restrict__ U1,
+                                              double* __restrict__ U1q) {
+  const int c = blockIdx.y + 1;
+  const int i = blockIdx.x + 2;
+  const int j = (threadIdx.x >> 4) + 2;
+  const int k = (threadIdx.x & 15) + 2;
+  const size_t b = sidx(c, 0, i, j, k);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const size_t o = b + (size_t)v * PLANE;
+    U1q[o] = U1[o];
+    U1[o] = U[o];
+  }
+}
+
+__global__ __launch_bounds__(WG) void k_aos_to_soa(const double* __restrict__ aos,
+                                                   double* __restrict__ soa, int ncells) {
+  const int g = blockIdx.x * WG + threadIdx.x;
+  if (g >= ncells) return;
+  const int c = g / PLANE, cell = g % PLANE;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) soa[(size_t)c * CUBE_STRIDE + (size_t)v * PLANE + cell] = aos[(size_t)g * 5 + v];
+}
+
+__global__ __launch_bounds__(WG) void k_soa_to_aos(const double* __restrict__ soa,
+                                                   double* __restrict__ aos, int ncells) {
+  const int g = blockIdx.x * WG + threadIdx.x;
+  if (g >= ncells) return;
+  const int c = g / PLANE, cell = g % PLANE;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) aos[(size_t)g * 5 + v] = soa[(size_t)c * CUBE_STRIDE + (size_t)v * PLANE + cell];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+int update_blocks(int ncube) { return 16 * (ncube - 1); }
+
+void launch_bc(hipStream_t s, double* U, const int* cubes, const uint8_t* masks, int n,
+               DevPhys ph) {
+  if (n > 0) hipLaunchKernelGGL(k_bc, dim3(n), dim3(WG), 0, s, U, cubes, masks, ph);
+}
+
+void launch_ghost(hipStream_t s, double* U, const GhostEntry* e, const int* cube_level_off,
+                  const int* level_off, const int* cubes, int ncubes_with_ghosts, DevPhys ph) {
+  (void)cubes;
+  if (ncubes_with_ghosts > 0)
+    hipLaunchKernelGGL(k_ghost, dim3(ncubes_with_ghosts), dim3(WG), 0, s, U, e, cube_level_off,
+                       level_off, ph);
+}
+
+void launch_halo_edge(hipStream_t s, double* U, const HaloSrc* src, int ncube) {
+  if (ncube > 1) hipLaunchKernelGGL(k_halo_edge, dim3(ncube - 1), dim3(WG), 0, s, U, src);
+}
+
+void launch_flux(hipStream_t s, const double* U, const double* U1, const double* U1q,
+                 const int8_t* fws, const double* cube_size, double* Rku, double* Res,
+                 int ncube, DevPhys ph, double deltaT, double deltaTau, double e) {
+  if (ncube > 1)
+    hipLaunchKernelGGL(k_flux, dim3(16, ncube - 1), dim3(WG), 0, s, U, U1, U1q, fws, cube_size,
+                       Rku, Res, ph, deltaT, deltaTau, e);
+}
+
+void launch_update(hipStream_t s, int RK, double* U, double* U1p1, double* U1p2,
+                   const double* Rku, const double* Res, const int8_t* fws, int ncube,
+                   double* partials) {
+  if (ncube > 1)
+    hipLaunchKernelGGL(k_update, dim3(16, ncube - 1), dim3(WG), 0, s, RK, U, U1p1, U1p2, Rku,
+                       Res, fws, partials);
+}
+
+void launch_reduce(hipStream_t s, const double* partials, int nblocks, double* er,
+                   int mpi_ncube, DevPhys ph) {
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(WG), 0, s, partials, nblocks, er, mpi_ncube, ph);
+}
+
+void launch_shift(hipStream_t s, const double* U, double* U1, double* U1q, int ncube) {
+  if (ncube > 1) hipLaunchKernelGGL(k_shift, dim3(16, ncube - 1), dim3(WG), 0, s, U, U1, U1q);
+}
+
+void launch_aos_to_soa(hipStream_t s, const double* aos, double* soa, int ncube) {
+  const int n = ncube * PLANE;
+  hipLaunchKernelGGL(k_aos_to_soa, dim3((n + WG - 1) / WG), dim3(WG), 0, s, aos, soa, n);
+}
+
+void launch_soa_to_aos(hipStream_t s, const double* soa, double* aos, int ncube) {
+  const int n = ncube * PLANE;
+  hipLaunchKernelGGL(k_soa_to_aos, dim3((n + WG - 1) / WG), dim3(WG), 0, s, soa, aos, n);
+}
+
+}  // namespace bcm

@@ -11,6 +11,8 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
+#include <thread>
 #include <map>
 
 namespace bcm {
@@ -441,14 +443,14 @@ void immersed_boundary(Case& cs) {
   cs.n_ghost_candidates = (int)gcs.size();
   const int Ngc = (int)gcs.size();
   std::vector<GhostHit> hits(Ngc);
-  // Surface search per ghost cell; independent across ghost cells.
-#pragma omp parallel for schedule(dynamic, 16)
-  for (int g = 0; g < Ngc; ++g) {
+  // Surface search per ghost cell; independent across ghost cells, so the
+  // cells are split over host threads.
+  auto search = [&](int g) {
     GhostHit& h = hits[g];
     h.hit = false;
     h.tmin = BIGNUM;
     h.dotp = 0.0;
-    if (Ntri == 0) continue;
+    if (Ntri == 0) return;
     const double orig[3] = {gcs[g].x, gcs[g].y, gcs[g].z};
     double ndis_min = BIGNUM;
     for (int it = 0; it < Ntri; ++it) {
@@ -489,6 +491,19 @@ void immersed_boundary(Case& cs) {
         }
       }
     }
+  };
+  {
+    int nth = (int)std::thread::hardware_concurrency();
+    if (nth < 1) nth = 1;
+    if (nth > 32) nth = 32;
+    std::atomic<int> next(0);
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nth; ++t)
+      pool.emplace_back([&]() {
+        for (int g; (g = next.fetch_add(16)) < Ngc;)
+          for (int q = g; q < std::min(g + 16, Ngc); ++q) search(q);
+      });
+    for (auto& th : pool) th.join();
   }
   // Sequential pass: a ghost cell without a ray hit reuses the orientation
   // (dotp) of the last ghost cell that had one, as the reference's dotp

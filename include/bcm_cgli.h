@@ -1,0 +1,178 @@
+/*
+ * bcm_cgli.h -- C ABI of the MI355X BCM solver library (libbcm_cgli.so).
+ *
+ * Three groups of entry points:
+ *   1. Drop-ins with the names and parameter order of the reference's BCM_*
+ *      routines (main.h prototypes).  They run on the device mirror of host
+ *      arrays registered with bcm_bind(); tables passed per call are copied to
+ *      the device on first use and reused while their host pointer and length
+ *      stay the same.  Like the reference (MPI_ERRORS_ARE_FATAL) they return
+ *      nothing and abort the process with a message on any error.
+ *   2. Host case construction (lattice, flags, IB tables, initial state).
+ *   3. A device engine that keeps one case resident and runs the reference
+ *      stage order, optionally as a captured HIP graph.
+ * Every function of groups 2 and 3 returns 0 on success and non-zero on
+ * failure, with the message in bcm_last_error().  Array arguments use the
+ * reference layouts of Array.h:11-36: state [ncube][20][20][20][5] doubles
+ * (k fastest, variable innermost), flags [ncube][20][20][20] ints, cubes
+ * numbered from 1 (slot 0 unused).  One host thread per engine.
+ */
+#ifndef BCM_CGLI_H
+#define BCM_CGLI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message of the most recent failure on the calling thread. */
+const char* bcm_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* 1. Drop-in entry points.                                                   */
+
+typedef struct bcm_engine bcm_engine;
+
+/* Registers the caller's state arrays (Array.h:11-23), flags, cube sizes and
+ * flow constants with engine e, uploads them, and makes e the engine the
+ * drop-ins below run on.  The constants replace prm.h:25-37 (rho0, U0, Uin,
+ * V0, W0, P0, mu_L; T0 = P0/rho0/R) and Resolution.h:7 (n_cubes = MPI_Ncube).
+ * No counterpart in the reference, whose arrays live in host memory only. */
+int bcm_bind(bcm_engine* e, int ncube, int n_cubes_norm, double* U1_, double* U1, double* U1q,
+             double* U1p1, double* U1p2, double* Rku1, double* Residual1, int* FWS,
+             double* cube_size, const double* phys7);
+/* Copies the device mirrors of the bound arrays back to / from the host. */
+int bcm_sync_to_host(bcm_engine* e);
+int bcm_sync_to_device(bcm_engine* e);
+
+/* Replaces BCM_X_boundary_condition.cpp:19-122 (main.h:418-436). */
+void BCM_X_boundary_condition(int myid, int nXbc_l, int nXbc_u, int* Xbc_l, int* Xbc_u,
+                              double* U1_);
+/* Replaces BCM_Y_boundary_condition.cpp:17-88 (main.h:438-453). */
+void BCM_Y_boundary_condition(int nYbc_l, int nYbc_u, int* Ybc_l, int* Ybc_u, double* U1_);
+/* Replaces BCM_Z_boundary_condition.cpp:20-96 (main.h:455-470). */
+void BCM_Z_boundary_condition(int nZbc_l, int nZbc_u, int* Zbc_l, int* Zbc_u, double* U1_);
+/* Replace BCM_Ghostcell_minus.cpp:17-495 and BCM_Ghostcell_plus.cpp:16-488
+ * (main.h:378-413).  GCindex/IPsur are 1-based [4*NNBC+1], weight [8*NNBC+1]. */
+void BCM_Ghostcell_minus(int myid, int* NNBC, double* weight, int* GCindex, int* IPsur,
+                         int* FWS, double* U1_);
+void BCM_Ghostcell_plus(int myid, int* NNBC, double* weight, int* GCindex, int* IPsur,
+                        int* FWS, double* U1_);
+/* Replaces BCM_Interface_table.cpp:20-308 (main.h:170-219); host-side table
+ * construction.  adj_number is [ncube][5][7]; lists are 1-based [ncube]. */
+void BCM_Interface_table(int myid, int ncube, int n_wallcube, int* nadjX_eq, int* nadjY_eq,
+                         int* nadjZ_eq, int* nadjX_bs_plus, int* nadjX_sb_plus,
+                         int* nadjX_bs_minus, int* nadjX_sb_minus, int* nadjY_bs_plus,
+                         int* nadjY_sb_plus, int* nadjY_bs_minus, int* nadjY_sb_minus,
+                         int* nadjZ_bs_plus, int* nadjZ_sb_plus, int* nadjZ_bs_minus,
+                         int* nadjZ_sb_minus, int* csl, int* adj_number, int* adjX_eq,
+                         int* adjY_eq, int* adjZ_eq, int* adjX_bs_plus, int* adjX_sb_plus,
+                         int* adjX_bs_minus, int* adjX_sb_minus, int* adjY_bs_plus,
+                         int* adjY_sb_plus, int* adjY_bs_minus, int* adjY_sb_minus,
+                         int* adjZ_bs_plus, int* adjZ_sb_plus, int* adjZ_bs_minus,
+                         int* adjZ_sb_minus);
+/* Replaces BCM_Interface.cpp:27-5735 (main.h:224-354) on one device: the
+ * equal-size halo copies and the edge-ghost averaging.  The MPI arguments
+ * must describe a single rank (all neighbour-rank counts zero); cube pairs of
+ * different size (any bs/sb count non-zero) are rejected. */
+void BCM_Interface(int myid, int ncube, int mPI_Nadj, int ncpu_bs, int ncpu_eq, int ncpu_sb,
+                   int max_nei_bs, int max_nei_eq, int max_nei_sb, int nadjX_eq, int nadjY_eq,
+                   int nadjZ_eq, int nadjX_bs_plus, int nadjX_sb_plus, int nadjX_bs_minus,
+                   int nadjX_sb_minus, int nadjY_bs_plus, int nadjY_sb_plus, int nadjY_bs_minus,
+                   int nadjY_sb_minus, int nadjZ_bs_plus, int nadjZ_sb_plus, int nadjZ_bs_minus,
+                   int nadjZ_sb_minus, int* rank_map, int* MPI_cpu, int* MPI_cube,
+                   int* MPI_cpu_adj, int* MPI_cube_adj, int* MPI_direction, int* MPI_interface,
+                   int* neighbor_cpu_eq, int* Ncube_Ncpu_eq, int* neighbor_cpu_sb,
+                   int* Ncube_Ncpu_sb, int* neighbor_cpu_bs, int* Ncube_Ncpu_bs,
+                   int* Scube_Ncpu_eq, int* Rcube_Ncpu_eq, double* send_data_curr_eq,
+                   double* recv_data_curr_eq, double* send_data_neig_eq,
+                   double* recv_data_neig_eq, int* Sdir_eq, int* Rdir_eq, int* Scube_Ncpu_sb,
+                   int* Rcube_Ncpu_sb, double* send_data_curr_sb, double* recv_data_curr_sb,
+                   double* send_data_neig_sb, double* recv_data_neig_sb, int* Sdir_sb,
+                   int* Rdir_sb, int* Scube_Ncpu_bs, int* Rcube_Ncpu_bs,
+                   double* send_data_curr_bs, double* recv_data_curr_bs,
+                   double* send_data_neig_bs, double* recv_data_neig_bs, int* Sdir_bs,
+                   int* Rdir_bs, int* ist_eq, int* ist_sb, int* ist_bs, int* adjN_sb,
+                   int* RadjN_bs, int* SadjN_bs, int* csl, int* adj_number, int* adjX_eq,
+                   int* adjY_eq, int* adjZ_eq, int* adjX_bs_plus, int* adjX_sb_plus,
+                   int* adjX_bs_minus, int* adjX_sb_minus, int* adjY_bs_plus,
+                   int* adjY_sb_plus, int* adjY_bs_minus, int* adjY_sb_minus,
+                   int* adjZ_bs_plus, int* adjZ_sb_plus, int* adjZ_bs_minus,
+                   int* adjZ_sb_minus, double* U1_);
+/* Replaces BCM_Flux_XYZ_Viscous_Runge_kutta.cpp:18-2658 (main.h:473-513):
+ * flux, residual and preconditioned stage update of every cube; at RK == 3
+ * also the change norms and force sums, returned in er[1..7]. */
+void BCM_Flux_XYZ_Viscous_Runge_kutta(int myid, int ncube, int RK, double deltaT,
+                                      double deltaTau, double e, int* FWS, int* csl,
+                                      double* cube_size, double* U1_, double* U1, double* U1q,
+                                      double* U1p1, double* U1p2, double* Rku1,
+                                      double* Residual1, double* er);
+/* Physical-time shift U1q <- U1, U1 <- U1_ on interior cells, which the
+ * reference does inline in its main loop (main.cpp:1292-1315). */
+void BCM_Physical_time_shift(double* U1_, double* U1, double* U1q);
+
+/* ------------------------------------------------------------------------ */
+/* 2. Host case construction.                                                 */
+
+typedef struct bcm_case_params {
+  int lattice[3];             /* cubes along x, y, z */
+  double cube_size;           /* cube edge length */
+  double origin[3];           /* lower box corner */
+  double sphere_center[3];
+  double sphere_diameter;
+  int ico_level;              /* icosphere level: 20*4^L triangles */
+  double mach, reynolds;
+  unsigned long long seed;    /* density-noise seed */
+  double noise_amp;           /* relative density noise amplitude */
+} bcm_case_params;
+
+typedef struct bcm_case bcm_case;
+
+int bcm_case_build(const bcm_case_params* p, bcm_case** out);
+void bcm_case_free(bcm_case* c);
+/* info[16]: ncube, n_wallcube, nbc_minus, nbc_plus, nXbc_l, nXbc_u, nYbc_l,
+ * nYbc_u, nZbc_l, nZbc_u, nadjX_eq, nadjY_eq, nadjZ_eq, n_triangles,
+ * n_ghost_candidates, 0.  phys[8]: rho0, U0, Uin, V0, W0, P0, T0, mu_L. */
+int bcm_case_info(const bcm_case* c, int* info, double* phys);
+/* Pointer to a case array and its element count.  Names: "U1_", "U1", "U1q",
+ * "cube_size", "Xcube", "Ycube", "Zcube" (double); "FWS", "csl",
+ * "adj_number", "ib_minus_gc", "ib_minus_ip", "ib_plus_gc", "ib_plus_ip",
+ * "Xbc_l", "Xbc_u", "Ybc_l", "Ybc_u", "Zbc_l", "Zbc_u", "adjX_eq", "adjY_eq",
+ * "adjZ_eq" (int); "ib_minus_w", "ib_plus_w" (double). */
+void* bcm_case_array(bcm_case* c, const char* name, long long* count);
+/* Writes BCMgrid00000.dat and BCM_STL_fine.stl, the reference's input files. */
+int bcm_case_write_inputs(const bcm_case* c, const char* dir);
+
+/* ------------------------------------------------------------------------ */
+/* 3. Device engine.                                                          */
+
+int bcm_engine_create(int device, bcm_engine** out);
+void bcm_engine_destroy(bcm_engine* e);
+/* Uploads a host case: state, flags and every table. */
+int bcm_engine_load_case(bcm_engine* e, const bcm_case* c);
+/* Physical and pseudo time steps and the preconditioning floor e
+ * (main.cpp:57-59 defaults 1e-4, 1e-4/200, 0.1). */
+int bcm_engine_set_time(bcm_engine* e, double deltaT, double deltaTau, double e_floor);
+/* n_steps physical steps of n_inner inner iterations (3 RK stages each),
+ * each step closed by the physical-time shift.  er_hist, if not null,
+ * receives er[1..7] after every inner iteration (7 doubles each). */
+int bcm_engine_run(bcm_engine* e, int n_steps, int n_inner, double* er_hist, int use_graph);
+/* One piece of a stage: 0 domain BCs, 1 IB ghost fill, 2 halo + edges,
+ * 3 flux, 4 RK update (with norms at RK 3), 5 time shift, 6 whole stage. */
+int bcm_engine_part(bcm_engine* e, int part, int RK);
+/* which: 0 U1_, 1 U1, 2 U1q, 3 U1p1, 4 U1p2, 5 Rku1, 6 Residual1. */
+int bcm_engine_get_state(bcm_engine* e, int which, double* aos);
+int bcm_engine_set_state(bcm_engine* e, int which, const double* aos);
+/* er[0..7] as last reduced (er[1..7] meaningful). */
+int bcm_engine_get_er(bcm_engine* e, double* er8);
+int bcm_engine_sync(bcm_engine* e);
+/* Average device time in ms over reps launches of one part (events on the
+ * engine stream). */
+int bcm_engine_time_part(bcm_engine* e, int part, int RK, int reps, double* ms);
+/* Ghost schedule statistics: entries, cubes with entries, maximum levels. */
+int bcm_engine_ghost_stats(bcm_engine* e, int* stats3);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BCM_CGLI_H */
