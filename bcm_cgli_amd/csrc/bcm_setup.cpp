@@ -494,8 +494,9 @@ void immersed_boundary(Case& cs) {
   };
   {
     int nth = (int)std::thread::hardware_concurrency();
+    if (const char* env = getenv("OMP_NUM_THREADS")) nth = std::min(nth, atoi(env));
     if (nth < 1) nth = 1;
-    if (nth > 32) nth = 32;
+    if (nth > 16) nth = 16;
     std::atomic<int> next(0);
     std::vector<std::thread> pool;
     for (int t = 0; t < nth; ++t)

@@ -1,0 +1,104 @@
+"""Writes the golden fixtures from the reference solver built from its own sources.
+
+Runs in the build container only (it needs the reference tree and MPICH):
+
+    python tests/golden/make_golden.py [tiny] [cfg1]
+
+For each case it writes the grid and surface files with the case generator,
+builds the reference with oracle/build_ref.sh (MPI_Ncube, U0, Uin and mu_L of
+the case substituted into its configuration headers), runs oracle/ref_driver
+with a fixed inner-iteration count, and stores:
+
+* ``<case>_golden.json`` -- SHA-256 of every dumped array (setup flags and
+  state, state after each of the first three RK stages, final state), the
+  per-inner-iteration norm history er[1..7] as exact hex floats, and the
+  run parameters;
+* ``<case>_ib.npz`` -- the IB tables the reference wrote (IBM_minus/plus),
+  parsed into integer and float arrays.
+"""
+import hashlib
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import bcm_cgli_amd as B  # noqa: E402
+from bcm_cgli_amd.configs import STEPPING  # noqa: E402
+
+REF_ROOT = os.path.join(ROOT, "oracle", "_ref")
+
+
+def sha(path):
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def parse_ibm(path):
+    with open(path) as f:
+        lines = f.read().split("\n")
+    n = int(lines[lines.index("Number of Boundary Cells") + 1])
+    gc, ip, w = [], [], []
+    for e in range(n):
+        gc.append([int(x) for x in lines[3 * e].split()])
+        ip.append([int(x) for x in lines[3 * e + 1].split()])
+        w.append([float(x) for x in lines[3 * e + 2].split()])
+    return (np.array(gc, dtype=np.int32).reshape(n, 4), np.array(ip, dtype=np.int32).reshape(n, 4),
+            np.array(w, dtype=np.float64).reshape(n, 8))
+
+
+def make(name):
+    params = B.case_params(name)
+    n_steps, n_inner = STEPPING[name]
+    case = B.Case(params)
+    work = os.path.join(REF_ROOT, name)
+    run_dir = os.path.join(work, "run")
+    dump = os.path.join(work, "dump")
+    os.makedirs(run_dir, exist_ok=True)
+    os.makedirs(dump, exist_ok=True)
+    case.write_inputs(run_dir)
+    subprocess.run(["bash", os.path.join(ROOT, "oracle", "build_ref.sh"), os.path.join(work, "build"),
+                    str(case.ncube - 1), repr(case.phys["U0"]), repr(case.phys["mu_L"])],
+                   check=True, stdout=subprocess.DEVNULL)
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    subprocess.run([os.path.join(work, "build", "ref_driver"), str(n_steps), str(n_inner),
+                    str(params["seed"]), repr(params["noise_amp"]), dump, "3"],
+                   cwd=run_dir, env=env, check=True, stdout=subprocess.DEVNULL)
+    hashes = {f: sha(os.path.join(dump, f)) for f in sorted(os.listdir(dump)) if f.endswith(".bin")}
+    er = []
+    with open(os.path.join(dump, "er.txt")) as f:
+        for line in f:
+            v = line.split()
+            er.append([float(x).hex() for x in v[2:9]])
+    out = {
+        "case": name,
+        "params": {k: list(v) if isinstance(v, tuple) else v for k, v in params.items()},
+        "n_steps": n_steps,
+        "n_inner": n_inner,
+        "deltaT": 1e-4,
+        "deltaTau": 1e-4 / 200.0,
+        "e": 0.1,
+        "ncube": case.ncube,
+        "nbc_minus": case.info["nbc_minus"],
+        "nbc_plus": case.info["nbc_plus"],
+        "sha256": hashes,
+        "er_hex": er,
+        "source": "reference sources compiled by oracle/build_ref.sh, driven by oracle/ref_driver.cpp",
+    }
+    with open(os.path.join(HERE, f"{name}_golden.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    gm, im, wm = parse_ibm(os.path.join(run_dir, "IBM_minus00000.dat"))
+    gp, ipp, wp = parse_ibm(os.path.join(run_dir, "IBM_plus00000.dat"))
+    np.savez_compressed(os.path.join(HERE, f"{name}_ib.npz"), minus_gc=gm, minus_ip=im, minus_w=wm,
+                        plus_gc=gp, plus_ip=ipp, plus_w=wp)
+    print(f"{name}: {len(hashes)} arrays, {len(er)} norm rows, IB {len(gm)} minus / {len(gp)} plus")
+
+
+if __name__ == "__main__":
+    for n in (sys.argv[1:] or ["tiny", "cfg1"]):
+        make(n)
