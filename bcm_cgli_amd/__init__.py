@@ -16,7 +16,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbcm_cgli.so")
 
 STATE_NAMES = ("U1_", "U1", "U1q", "U1p1", "U1p2", "Rku1", "Residual1")
-PARTS = {"bc": 0, "ghost": 1, "interface": 2, "flux": 3, "update": 4, "shift": 5, "stage": 6}
+PARTS = {"bc": 0, "ghost": 1, "interface": 2, "flux": 3, "update": 4, "shift": 5, "stage": 6,
+         "flux_update": 7}
 
 
 class BCMError(RuntimeError):
@@ -72,6 +73,8 @@ def load_library(path=LIB_PATH):
     lib.bcm_engine_sync.argtypes = [c_void_p]
     lib.bcm_engine_time_part.argtypes = [c_void_p, c_int, c_int, c_int, P(c_double)]
     lib.bcm_engine_ghost_stats.argtypes = [c_void_p, c_void_p]
+    lib.bcm_engine_set_flux.argtypes = [c_void_p, c_int, c_int, c_int]
+    lib.bcm_engine_autotune.argtypes = [c_void_p, c_void_p]
     _lib = lib
     return lib
 
@@ -197,6 +200,14 @@ class Engine:
         ms = ctypes.c_double()
         _check(_lib.bcm_engine_time_part(self._h, PARTS[name], int(RK), int(reps), ctypes.byref(ms)))
         return ms.value
+
+    def set_flux(self, variant, nseg=1, fuse=True):
+        _check(_lib.bcm_engine_set_flux(self._h, int(variant), int(nseg), int(bool(fuse))))
+
+    def autotune(self):
+        out = (ctypes.c_int * 3)()
+        _check(_lib.bcm_engine_autotune(self._h, out))
+        return {"variant": out[0], "nseg": out[1], "fuse": bool(out[2])}
 
     def ghost_stats(self):
         s = (ctypes.c_int * 3)()

@@ -338,6 +338,7 @@ int bcm_engine_part(bcm_engine* h, int part, int RK) {
     case 4: engine_update(e, RK, 0); break;
     case 5: engine_shift(e); break;
     case 6: engine_stage(e, RK, 0); break;
+    case 7: engine_flux_update(e, RK, 0); break;
     default: return fail("unknown part");
   }
   std::string err;
@@ -389,6 +390,7 @@ int bcm_engine_time_part(bcm_engine* h, int part, int RK, int reps, double* ms) 
       case 4: engine_update(e, RK, 0); break;
       case 5: engine_shift(e); break;
       case 6: engine_stage(e, RK, 0); break;
+      case 7: engine_flux_update(e, RK, 0); break;
       default: return fail("unknown part");
     }
   }
@@ -399,6 +401,32 @@ int bcm_engine_time_part(bcm_engine* h, int part, int RK, int reps, double* ms) 
   *ms = (double)t / reps;
   std::string err;
   if (engine_sync(e, err)) return fail(err);
+  return 0;
+}
+
+int bcm_engine_set_flux(bcm_engine* h, int variant, int nseg, int fuse) {
+  if (!h) return fail("null engine");
+  h->e->flux_fuse = fuse != 0;
+  if (variant != 0 && variant != 1) return fail("flux variant must be 0 or 1");
+  if (nseg != 1 && nseg != 2 && nseg != 4 && nseg != 8 && nseg != 16)
+    return fail("nseg must divide 16");
+  h->e->flux_variant = variant;
+  h->e->flux_nseg = nseg;
+  h->e->flux_tuned = true;
+  h->e->graph_valid = false;
+  return 0;
+}
+
+int bcm_engine_autotune(bcm_engine* h, int* out3) {
+  if (!h) return fail("null engine");
+  std::string err;
+  int v = 0, n = 1;
+  if (engine_autotune(h->e, &v, &n, err)) return fail(err);
+  if (out3) {
+    out3[0] = v;
+    out3[1] = n;
+    out3[2] = h->e->flux_fuse ? 1 : 0;
+  }
   return 0;
 }
 
@@ -605,8 +633,7 @@ void BCM_Flux_XYZ_Viscous_Runge_kutta(int myid, int ncube, int RK, double deltaT
   en->deltaT = deltaT;
   en->deltaTau = deltaTau;
   en->e = e;
-  engine_flux(en, RK);
-  engine_update(en, RK, 0);
+  engine_flux_update(en, RK, 0);
   check(where, hipGetLastError());
   if (RK == 3) {
     double tmp[8];

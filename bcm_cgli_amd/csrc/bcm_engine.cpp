@@ -188,6 +188,7 @@ int engine_alloc(Engine* e, int ncube, std::string& err) {
 
 int engine_load_case(Engine* e, const Case& cs, std::string& err) {
   if (engine_alloc(e, cs.ncube, err)) return 1;
+  e->flux_tuned = false;
   e->ph = DevPhys{cs.phys.rho0, cs.phys.U0, cs.phys.Uin, cs.phys.V0,
                   cs.phys.W0,   cs.phys.P0, cs.phys.T0,  cs.phys.mu_L};
   if (engine_upload_flags(e, cs.fws.data())) {
@@ -308,9 +309,84 @@ void engine_ghost(Engine* e) {
 void engine_interface(Engine* e) { launch_halo_edge(e->stream, e->U, e->halo, e->ncube); }
 
 void engine_flux(Engine* e, int RK) {
-  (void)RK;
-  launch_flux(e->stream, e->U, e->U1, e->U1q, e->fws, e->cube_size, e->Rku, e->Res, e->ncube,
-              e->ph, e->deltaT, e->deltaTau, e->e);
+  if (e->flux_variant == 0)
+    launch_flux(e->stream, e->U, e->U1, e->U1q, e->fws, e->cube_size, e->Rku, e->Res, e->ncube,
+                e->ph, e->deltaT, e->deltaTau, e->e);
+  else
+    launch_flux_march(e->stream, e->U, e->U1, e->U1q, e->fws, e->cube_size, e->Rku, e->Res,
+                      e->ncube, e->ph, e->deltaT, e->deltaTau, e->e, e->flux_nseg, RK, 0,
+                      e->U1p1, e->U1p2, e->partials);
+}
+
+bool engine_fused(const Engine* e) {
+  return e->flux_variant == 1 && e->flux_nseg == 1 && e->flux_fuse;
+}
+
+void engine_flux_update(Engine* e, int RK, int er_slot) {
+  if (!engine_fused(e)) {
+    engine_flux(e, RK);
+    engine_update(e, RK, er_slot);
+    return;
+  }
+  launch_flux_march(e->stream, e->U, e->U1, e->U1q, e->fws, e->cube_size, e->Rku, e->Res,
+                    e->ncube, e->ph, e->deltaT, e->deltaTau, e->e, 1, RK, 1, e->U1p1, e->U1p2,
+                    e->partials);
+  if (RK == 3)
+    launch_reduce(e->stream, e->partials, e->ncube - 1, e->er_dev + 8 * er_slot, e->mpi_ncube,
+                  e->ph);
+}
+
+int engine_autotune(Engine* e, int* variant, int* nseg, std::string& err) {
+  HIP_TRY(hipSetDevice(e->device));
+  if (e->ncube <= 1) {
+    err = "no case loaded";
+    return 1;
+  }
+  // Candidates (variant, nseg, fuse): the plane kernel, the marching kernel
+  // with 1..8 segments, and the marching kernel with the RK update fused.
+  // Each is timed as flux + update of one RK stage on the resident case; the
+  // arrays the update writes are saved first and restored afterwards.
+  const int cand[6][3] = {{0, 1, 0}, {1, 1, 1}, {1, 1, 0}, {1, 2, 0}, {1, 4, 0}, {1, 8, 0}};
+  double* saved[3] = {nullptr, nullptr, nullptr};
+  double* live[3] = {e->U, e->U1p1, e->U1p2};
+  for (int a = 0; a < 3; ++a) {
+    HIP_TRY(hipMalloc((void**)&saved[a], e->state_bytes));
+    HIP_TRY(hipMemcpyAsync(saved[a], live[a], e->state_bytes, hipMemcpyDeviceToDevice, e->stream));
+  }
+  float best = 1e30f;
+  int bv = e->flux_variant, bs = e->flux_nseg;
+  bool bf = e->flux_fuse;
+  for (auto& cv : cand) {
+    e->flux_variant = cv[0];
+    e->flux_nseg = cv[1];
+    e->flux_fuse = cv[2] != 0;
+    engine_flux_update(e, 2, 0);  // warm up
+    HIP_TRY(hipEventRecord(e->ev0, e->stream));
+    for (int r = 0; r < 3; ++r) engine_flux_update(e, 2, 0);
+    HIP_TRY(hipEventRecord(e->ev1, e->stream));
+    HIP_TRY(hipEventSynchronize(e->ev1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    if (ms < best) {
+      best = ms;
+      bv = cv[0];
+      bs = cv[1];
+      bf = cv[2] != 0;
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    HIP_TRY(hipMemcpyAsync(live[a], saved[a], e->state_bytes, hipMemcpyDeviceToDevice, e->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  for (int a = 0; a < 3; ++a) hipFree(saved[a]);
+  e->flux_variant = bv;
+  e->flux_nseg = bs;
+  e->flux_fuse = bf;
+  e->flux_tuned = true;
+  e->graph_valid = false;
+  if (variant) *variant = bv;
+  if (nseg) *nseg = bs;
+  return 0;
 }
 
 void engine_update(Engine* e, int RK, int er_slot) {
@@ -327,8 +403,7 @@ void engine_stage(Engine* e, int RK, int er_slot) {
   engine_bc(e);
   engine_ghost(e);
   engine_interface(e);
-  engine_flux(e, RK);
-  engine_update(e, RK, er_slot);
+  engine_flux_update(e, RK, er_slot);
   engine_interface(e);
 }
 
@@ -363,6 +438,10 @@ int engine_run(Engine* e, int n_steps, int n_inner, double* er_hist, bool use_gr
     return 1;
   }
   if (ensure_er(e, n_inner, err)) return 1;
+  if (!e->flux_tuned) {
+    int v, n;
+    if (engine_autotune(e, &v, &n, err)) return 1;
+  }
   if (use_graph && (!e->graph_valid || e->graph_inner != n_inner)) {
     free_graph(e);
     HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));

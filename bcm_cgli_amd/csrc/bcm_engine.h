@@ -54,6 +54,13 @@ struct Engine {
   double* er_host = nullptr;    // pinned mirror of er_dev
   size_t state_bytes = 0;
 
+  // Flux kernel choice: variant 0 = one workgroup per cell plane (k_flux),
+  // 1 = marching workgroups over nseg segments of each cube (k_flux_march).
+  int flux_variant = 1;
+  int flux_nseg = 1;
+  bool flux_fuse = true;     // marching kernel with nseg 1 also applies the RK update
+  bool flux_tuned = false;
+
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool graph_valid = false;
   hipGraphExec_t graph_exec = nullptr;
@@ -79,6 +86,10 @@ void engine_ghost(Engine* e);
 void engine_interface(Engine* e);
 void engine_flux(Engine* e, int RK);
 void engine_update(Engine* e, int RK, int er_slot);
+// Flux and RK update of one stage, fused into one kernel when the flux
+// choice allows it (marching kernel, one workgroup per cube).
+void engine_flux_update(Engine* e, int RK, int er_slot);
+bool engine_fused(const Engine* e);
 void engine_shift(Engine* e);
 void engine_stage(Engine* e, int RK, int er_slot);
 // n_steps physical steps of n_inner inner iterations; er_hist (may be null)
@@ -86,5 +97,8 @@ void engine_stage(Engine* e, int RK, int er_slot);
 int engine_run(Engine* e, int n_steps, int n_inner, double* er_hist, bool use_graph,
                std::string& err);
 int engine_sync(Engine* e, std::string& err);
+// Times every flux variant on the resident case and keeps the fastest.
+// Returns the chosen (variant, nseg) through the pointers.
+int engine_autotune(Engine* e, int* variant, int* nseg, std::string& err);
 
 }  // namespace bcm

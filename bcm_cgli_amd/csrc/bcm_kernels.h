@@ -38,6 +38,14 @@ void launch_halo_edge(hipStream_t s, double* U, const HaloSrc* src, int ncube);
 void launch_flux(hipStream_t s, const double* U, const double* U1, const double* U1q,
                  const int8_t* fws, const double* cube_size, double* Rku, double* Res,
                  int ncube, DevPhys ph, double deltaT, double deltaTau, double e);
+// Marching variant of the flux kernel: one workgroup per cube segment
+// (nseg in {1, 2, 4, 8, 16} segments of the cube's 16 i-planes).  With
+// fuse != 0 and nseg == 1 it also applies the RK update in place and, at
+// RK 3, writes one row of 8 norm partials per cube (as k_update does per block).
+void launch_flux_march(hipStream_t s, double* U, const double* U1, const double* U1q,
+                       const int8_t* fws, const double* cube_size, double* Rku, double* Res,
+                       int ncube, DevPhys ph, double deltaT, double deltaTau, double e, int nseg,
+                       int RK, int fuse, double* U1p1, double* U1p2, double* partials);
 void launch_update(hipStream_t s, int RK, double* U, double* U1p1, double* U1p2,
                    const double* Rku, const double* Res, const int8_t* fws, int ncube,
                    double* partials);

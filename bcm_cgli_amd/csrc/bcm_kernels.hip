@@ -382,6 +382,419 @@ __device__ __forceinline__ void block_sum(double (&x)[NS], double* sh) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Face-based formulation of the same flux (used by k_flux_march).
+//
+// The states at the face between cells p and p+1 depend only on the
+// reconstruction order and on the six cells p-2..p+3: the reference's
+// backward-face formulas of cell p+1 and forward-face formulas of cell p are
+// the same expressions (Flux.cpp:499-562).  Only the order choice differs
+// between the two views, so one evaluation serves both cells whenever their
+// orders agree, with bitwise the same result.
+
+// Order of face p+1/2 as cell p sees it (forward face, Flux.cpp:497,533).
+__device__ __forceinline__ int order_fwd(int p, int fm1, int fp1, int fp2) {
+  if (fp1 == 1 && fm1 == 1) return (p == 17 || fp2 != 1) ? 3 : 5;
+  return 1;
+}
+// Order of face q-1/2 as cell q sees it (backward face, Flux.cpp:497,499).
+__device__ __forceinline__ int order_bwd(int q, int fm2, int fm1, int fp1) {
+  if (fp1 == 1 && fm1 == 1) return (q == 2 || fm2 != 1) ? 3 : 5;
+  return 1;
+}
+
+// Flux through the face between p and p+1 along axis D.  b indexes cell p
+// (variable 0); only the cells the order needs are loaded.
+template <int D>
+__device__ __forceinline__ void face_flux(const double* __restrict__ U, size_t b, int order,
+                                          double e, double (&F)[5], double& dP) {
+  constexpr int st = D == 0 ? 400 : (D == 1 ? 20 : 1);
+  const double c6 = 1. / 6, c60 = 1. / 60;
+  double L[5], R[5];
+  if (order == 5) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const double* q = U + b + (size_t)v * PLANE;
+      const double s0 = q[-2 * st], s1 = q[-st], s2 = q[0], s3 = q[st], s4 = q[2 * st], s5 = q[3 * st];
+      L[v] = c60 * (2 * s0 - 13 * s1 + 47 * s2 + 27 * s3 - 3 * s4);
+      R[v] = c60 * (-3 * s1 + 27 * s2 + 47 * s3 - 13 * s4 + 2 * s5);
+    }
+  } else if (order == 3) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const double* q = U + b + (size_t)v * PLANE;
+      const double s1 = q[-st], s2 = q[0], s3 = q[st], s4 = q[2 * st];
+      L[v] = c6 * (-s1 + 5 * s2 + 2 * s3);
+      R[v] = c6 * (2 * s2 + 5 * s3 - s4);
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const double* q = U + b + (size_t)v * PLANE;
+      L[v] = q[0];
+      R[v] = q[st];
+    }
+  }
+  roe_flux<D>(L, R, e, F, dP);
+}
+
+// LDS layout of k_flux_march.
+constexpr int PR_PLANE = 18 * 18;   // primitives of j,k in [1,18]
+constexpr int NFACE = 17 * 16;      // faces of one axis in one i-plane
+
+// Flux, residual and stage increment with the work of one cube (or one
+// segment of its i-range) per workgroup, marching along i.  Per plane:
+// primitives of the next plane go into a 3-plane LDS ring (each computed once
+// instead of once per neighbour); the 272 y-faces and 272 z-faces of the plane
+// are evaluated once each into LDS; the x-face i+1/2 is carried in registers to
+// the next plane.  A cell whose backward-face order differs from its
+// neighbour's forward-face order evaluates its own backward face.  All
+// arithmetic is the reference's, in its order: results are bitwise those of
+// k_flux.
+__global__ __launch_bounds__(WG, 2) void k_flux_march(
+    double* __restrict__ U, const double* __restrict__ U1, const double* __restrict__ U1q,
+    const int8_t* __restrict__ fws, const double* __restrict__ cube_size,
+    double* __restrict__ Rku, double* __restrict__ Res, DevPhys ph, double deltaT,
+    double deltaTau, double e, int nseg, int RK, int fuse, double* __restrict__ U1p1,
+    double* __restrict__ U1p2, double* __restrict__ partials) {
+  __shared__ double pr[4][3 * PR_PLANE];   // u, v, w, T = P/(rho R) of 3 planes
+  __shared__ double fy[5][NFACE];
+  __shared__ double fz[6][NFACE];          // flux + face pressure average
+  const int seg = blockIdx.x % nseg;
+  const int c = blockIdx.x / nseg + 1;
+  const int len = 16 / nseg;
+  const int i0 = 2 + seg * len, i1 = i0 + len - 1;
+  const int t = threadIdx.x;
+  const int j = (t >> 4) + 2, k = (t & 15) + 2;
+  const double dx = cube_size[c] / 16;
+  const double dy = dx, dz = dx;
+  const double inv = 1. / dx;
+  // x / 2^n equals x * 2^-n exactly: use the multiply when dx is a power of two.
+  int ex;
+  const bool pow2 = frexp(dx, &ex) == 0.5;
+
+  auto fill_prims = [&](int ip) {
+    const int slot = ip % 3;
+    for (int n = t; n < PR_PLANE; n += WG) {
+      const int jj = n / 18 + 1, kk = n % 18 + 1;
+      const size_t o = sidx(c, 0, ip, jj, kk);
+      const double r = U[o];
+      const double u = U[o + PLANE] / r, v = U[o + 2 * PLANE] / r, w = U[o + 3 * PLANE] / r;
+      const double P = (U[o + 4 * PLANE] - 0.5 * r * (u * u + v * v + w * w)) * KM1;
+      pr[0][slot * PR_PLANE + n] = u;
+      pr[1][slot * PR_PLANE + n] = v;
+      pr[2][slot * PR_PLANE + n] = w;
+      pr[3][slot * PR_PLANE + n] = P / (r * RG);
+    }
+  };
+  fill_prims(i0 - 1);
+  fill_prims(i0);
+
+  double cF[5], cdP = 0;  // x-face flux carried from the previous plane
+  int corder = -1;
+  // With one workgroup per cube (fuse), the RK update of a plane runs three
+  // planes behind the flux: no later plane's stencil reaches back that far,
+  // and no other workgroup reads this cube.  RK3 norms accumulate per lane.
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  auto update_plane = [&](int ip) {
+    const size_t bu = sidx(c, 0, ip, j, k);
+    if (fws[fidx(c, ip, j, k)] != 1) return;
+    if (RK == 1) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        const size_t o = bu + (size_t)v * PLANE;
+        const double x = U[o] + Rku[o];
+        U[o] = x;
+        U1p1[o] = x;
+      }
+    } else if (RK == 2) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        const size_t o = bu + (size_t)v * PLANE;
+        const double x = 0.75 * U[o] + 0.25 * U1p1[o] + 0.25 * Rku[o];
+        U[o] = x;
+        U1p2[o] = x;
+      }
+    } else {
+      double uo[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) uo[v] = U[bu + (size_t)v * PLANE];
+      const double ro = uo[0];
+      const double uo1 = uo[1] / ro, vo1 = uo[2] / ro, wo1 = uo[3] / ro;
+      const double VVo = uo1 * uo1 + vo1 * vo1 + wo1 * wo1;
+      const double Po = (uo[4] - 0.5 * ro * VVo) * KM1;
+      const double To = Po / ro;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        const size_t o = bu + (size_t)v * PLANE;
+        uo[v] = 1.0 / 3.0 * uo[v] + 2.0 / 3.0 * U1p2[o] + 2.0 / 3.0 * Rku[o];
+        U[o] = uo[v];
+      }
+      const double r = uo[0];
+      const double un = uo[1] / r, vn = uo[2] / r, wn = uo[3] / r;
+      const double VV = un * un + vn * vn + wn * wn;
+      const double P = (uo[4] - 0.5 * r * VV) * KM1;
+      const double T = P / r;
+      acc[0] += (P - Po) * (P - Po);
+      acc[1] += (un - uo1) * (un - uo1);
+      acc[2] += (vn - vo1) * (vn - vo1);
+      acc[3] += (wn - wo1) * (wn - wo1);
+      acc[4] += (T - To) * (T - To);
+    }
+  };
+#pragma unroll 1
+  for (int i = i0; i <= i1; ++i) {
+    // Plane i-4 was last read during iteration i-1 (backward-face stencil of
+    // plane i-1 reaches i-4); the barrier that ended that iteration makes its
+    // update safe now.
+    if (fuse && i - 4 >= i0) update_plane(i - 4);
+    fill_prims(i + 1);
+    // y-faces of plane i: face f joins cells j = p and p+1 at k.
+    for (int f = t; f < NFACE; f += WG) {
+      const int p = f / 16 + 1, kk = f % 16 + 2;
+      const int8_t* fb = fws + fidx(c, i, p, kk);
+      const int o = p >= 2 ? order_fwd(p, fb[-20], fb[20], fb[40]) : order_bwd(2, fb[-20], fb[0], fb[40]);
+      double F[5], dP;
+      face_flux<1>(U, sidx(c, 0, i, p, kk), o, e, F, dP);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) fy[v][f] = F[v];
+    }
+    // z-faces of plane i: face f joins cells k = p and p+1 at j.
+    for (int f = t; f < NFACE; f += WG) {
+      const int jj = f / 17 + 2, p = f % 17 + 1;
+      const int8_t* fb = fws + fidx(c, i, jj, p);
+      const int o = p >= 2 ? order_fwd(p, fb[-1], fb[1], fb[2]) : order_bwd(2, fb[-1], fb[0], fb[2]);
+      double F[5], dP;
+      face_flux<2>(U, sidx(c, 0, i, jj, p), o, e, F, dP);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) fz[v][f] = F[v];
+      fz[5][f] = dP;
+    }
+    __syncthreads();
+
+    const size_t b = sidx(c, 0, i, j, k);
+    const int8_t* fc = fws + fidx(c, i, j, k);
+    const int IF = fc[0];
+    // ---- inviscid, x: forward face computed, backward face carried ----
+    double Rx[5], Ry[5], Rz[5], dPxb, dPxf, dPzb, dPzf;
+    {
+      const int fm2 = fc[-800], fm1 = fc[-400], fp1 = fc[400], fp2 = fc[800];
+      const int of = order_fwd(i, fm1, fp1, fp2);
+      double Ff[5];
+      face_flux<0>(U, b, of, e, Ff, dPxf);
+      const int ob = order_bwd(i, fm2, fm1, fp1);
+      if (i != i0 && ob == corder) {
+        dPxb = cdP;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Rx[v] = Ff[v] - cF[v];
+      } else {
+        double Fb[5];
+        face_flux<0>(U, b - 400, ob, e, Fb, dPxb);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Rx[v] = Ff[v] - Fb[v];
+      }
+#pragma unroll
+      for (int v = 0; v < 5; ++v) cF[v] = Ff[v];
+      cdP = dPxf;
+      corder = of;
+    }
+    // ---- inviscid, y ----
+    {
+      const int fm2 = fc[-40], fm1 = fc[-20], fp1 = fc[20];
+      const int ff = (j - 1) * 16 + (k - 2), fbk = ff - 16;
+      const int ob = order_bwd(j, fm2, fm1, fp1);
+      const bool share = j == 2 || ob == order_fwd(j - 1, fm2, fc[0], fp1);
+      if (share) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Ry[v] = fy[v][ff] - fy[v][fbk];
+      } else {
+        double Fb[5], dPu;
+        face_flux<1>(U, b - 20, ob, e, Fb, dPu);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Ry[v] = fy[v][ff] - Fb[v];
+      }
+    }
+    // ---- inviscid, z ----
+    {
+      const int fm2 = fc[-2], fm1 = fc[-1], fp1 = fc[1];
+      const int ff = (j - 2) * 17 + (k - 1), fbk = ff - 1;
+      const int ob = order_bwd(k, fm2, fm1, fp1);
+      const bool share = k == 2 || ob == order_fwd(k - 1, fm2, fc[0], fp1);
+      dPzf = fz[5][ff];
+      if (share) {
+        dPzb = fz[5][fbk];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Rz[v] = fz[v][ff] - fz[v][fbk];
+      } else {
+        double Fb[5];
+        face_flux<2>(U, b - 1, ob, e, Fb, dPzb);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Rz[v] = fz[v][ff] - Fb[v];
+      }
+    }
+    double Rf[5];
+    if (pow2) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) Rf[v] = -(Rx[v] * inv + Ry[v] * inv + Rz[v] * inv);
+    } else {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) Rf[v] = -(Rx[v] / dx + Ry[v] / dy + Rz[v] / dz);
+    }
+
+    // ---- viscous (Flux.cpp:1722-2279) from the primitive ring ----
+    double u[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) u[v] = U[b + (size_t)v * PLANE];
+    const double rho = u[0];
+    auto Q = [&](int a, int bb, int cc, int m) -> double {
+      return pr[m][((i + a) % 3) * PR_PLANE + (j + bb - 1) * 18 + (k + cc - 1)];
+    };
+    Prim me;
+    me.u = Q(0, 0, 0, 0);
+    me.v = Q(0, 0, 0, 1);
+    me.w = Q(0, 0, 0, 2);
+    {
+      const double VV = me.u * me.u + me.v * me.v + me.w * me.w;
+      const double P = (u[4] - 0.5 * rho * VV) * KM1;
+      me.T = P / rho / RG;
+    }
+    double vF[5] = {0, 0, 0, 0, 0};
+    double Lb[5], Lf[5];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+      for (int side = -1; side <= 1; side += 2) {
+        const int o0 = d == 0 ? side : 0, o1 = d == 1 ? side : 0, o2 = d == 2 ? side : 0;
+        Prim f;
+        f.u = Q(o0, o1, o2, 0);
+        f.v = Q(o0, o1, o2, 1);
+        f.w = Q(o0, o1, o2, 2);
+        f.T = Q(o0, o1, o2, 3);
+        const double Tf = 0.5 * (me.T + f.T);
+        const double temp = Tf / ph.T0;
+        const double mu = ph.mu_L * temp * sqrt(temp) * (ph.T0 + 110.) / (Tf + 110.);
+        double g[3][4];
+#pragma unroll
+        for (int tt = 0; tt < 3; ++tt) {
+          if (tt == d) {
+            if (side < 0) {
+              g[tt][0] = (me.u - f.u) * inv;
+              g[tt][1] = (me.v - f.v) * inv;
+              g[tt][2] = (me.w - f.w) * inv;
+              g[tt][3] = (me.T - f.T) * inv;
+            } else {
+              g[tt][0] = (f.u - me.u) * inv;
+              g[tt][1] = (f.v - me.v) * inv;
+              g[tt][2] = (f.w - me.w) * inv;
+              g[tt][3] = (f.T - me.T) * inv;
+            }
+          } else {
+            const int p0 = tt == 0, p1 = tt == 1, p2 = tt == 2;
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+              g[tt][m] = (Q(p0, p1, p2, m) + Q(p0 + o0, p1 + o1, p2 + o2, m) - Q(-p0, -p1, -p2, m) -
+                          Q(-p0 + o0, -p1 + o1, -p2 + o2, m)) / 4 * inv;
+          }
+        }
+        const double uf = 0.5 * (me.u + f.u), vf = 0.5 * (me.v + f.v), wf = 0.5 * (me.w + f.w);
+        const double div = g[0][0] + g[1][1] + g[2][2];
+        double sg[3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+          sg[m] = (m == d) ? mu * (2 * g[d][d] - 2.0 / 3.0 * div) : mu * (g[m][d] + g[d][m]);
+        double* L = side < 0 ? Lb : Lf;
+        L[1] = sg[0];
+        L[2] = sg[1];
+        L[3] = sg[2];
+        L[4] = uf * sg[0] + vf * sg[1] + wf * sg[2] + mu * CVG * KG * g[d][3] / (PRG);
+      }
+#pragma unroll
+      for (int v = 1; v < 5; ++v) {
+        const double term = (Lf[v] - Lb[v]) * inv;
+        vF[v] = d == 0 ? term : vF[v] + term;
+      }
+    }
+
+    // ---- residual with the dual-time source, preconditioner, increment ----
+    double Rk[5];
+    {
+      const double uu0 = U1[b], uq0 = U1q[b];
+      Rk[0] = -(3 * u[0] - 4 * uu0 + uq0) / (2 * deltaT) + Rf[0];
+    }
+#pragma unroll
+    for (int v = 1; v < 5; ++v) {
+      const double uu = U1[b + (size_t)v * PLANE], uq = U1q[b + (size_t)v * PLANE];
+      const double Rp = -(3 * u[v] - 4 * uu + uq) / (2 * deltaT);
+      Rk[v] = Rp + Rf[v] + vF[v];
+    }
+    double out[5] = {0, 0, 0, 0, 0};
+    if (IF == 1) {
+      const double U_ = me.u, V_ = me.v, W_ = me.w;
+      const double VV = U_ * U_ + V_ * V_ + W_ * W_;
+      const double P = (u[4] - 0.5 * rho * VV) * KM1;
+      const double C = KG * P / rho;
+      const double T = P / rho;
+      const double H = 0.5 * VV + C / KM1;
+      const double beta = rmax(VV / C, e);
+      const double temp = KG * T * (2 * deltaT + 3 * deltaTau) * (2 * deltaT + 3 * beta * deltaTau);
+      const double temp2 = KM1 * (beta - 1) * deltaT * deltaT;
+      const double temp3 = H - H * KG - VV + KG * (T + VV);
+      const double d11 = (2 * deltaT * (-2 * (-KM1 * (H - VV) - temp3 * beta) * deltaT + 3 * KG * T * beta * deltaTau)) / temp;
+      const double d12 = (-4 * U_ * temp2) / temp;
+      const double d13 = (-4 * V_ * temp2) / temp;
+      const double d14 = (-4 * W_ * temp2) / temp;
+      const double d15 = (4 * temp2) / temp;
+      out[0] = deltaTau * (d11 * Rk[0] + d12 * Rk[1] + d13 * Rk[2] + d14 * Rk[3] + d15 * Rk[4]);
+      const double d21 = (4 * U_ * temp3 * (beta - 1) * deltaT * deltaT) / temp;
+      const double d22 = (2 * deltaT * (2 * KG * (T - U_ * U_ * (beta - 1)) * deltaT + 2 * U_ * U_ * (beta - 1) * deltaT + 3 * KG * T * beta * deltaTau)) / temp;
+      const double d23 = (-4 * U_ * V_ * temp2) / temp;
+      const double d24 = (-4 * U_ * W_ * temp2) / temp;
+      const double d25 = (4 * U_ * temp2) / temp;
+      out[1] = deltaTau * (d21 * Rk[0] + d22 * Rk[1] + d23 * Rk[2] + d24 * Rk[3] + d25 * Rk[4]);
+      const double d31 = (4 * V_ * temp3 * (beta - 1) * deltaT * deltaT) / temp;
+      const double d33 = (2 * deltaT * (2 * KG * (T - V_ * V_ * (beta - 1)) * deltaT + 2 * V_ * V_ * (beta - 1) * deltaT + 3 * KG * T * beta * deltaTau)) / temp;
+      const double d34 = (-4 * V_ * W_ * temp2) / temp;
+      const double d35 = (4 * V_ * temp2) / temp;
+      out[2] = deltaTau * (d31 * Rk[0] + d23 * Rk[1] + d33 * Rk[2] + d34 * Rk[3] + d35 * Rk[4]);
+      const double d41 = (4 * W_ * temp3 * (beta - 1) * deltaT * deltaT) / temp;
+      const double d44 = (2 * deltaT * (2 * KG * (T - W_ * W_ * (beta - 1)) * deltaT + 2 * W_ * W_ * (beta - 1) * deltaT + 3 * KG * T * beta * deltaTau)) / temp;
+      const double d45 = (4 * W_ * temp2) / temp;
+      out[3] = deltaTau * (d41 * Rk[0] + d24 * Rk[1] + d34 * Rk[2] + d44 * Rk[3] + d45 * Rk[4]);
+      const double d51 = (4 * H * temp3 * (beta - 1) * deltaT * deltaT) / temp;
+      const double d52 = (-4 * H * U_ * temp2) / temp;
+      const double d53 = (-4 * H * V_ * temp2) / temp;
+      const double d54 = (-4 * H * W_ * temp2) / temp;
+      const double d55 = (2 * deltaT * (2 * H * KM1 * (beta - 1) * deltaT + KG * T * (2 * deltaT + 3 * beta * deltaTau))) / temp;
+      out[4] = deltaTau * (d51 * Rk[0] + d52 * Rk[1] + d53 * Rk[2] + d54 * Rk[3] + d55 * Rk[4]);
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Rku[b + (size_t)v * PLANE] = out[v];
+    if (IF < 1) {
+      const double px = pow2 ? -(dPxf - dPxb) * inv : -(dPxf - dPxb) / dx;
+      const double pz = pow2 ? -(dPzf - dPzb) * inv : -(dPzf - dPzb) / dz;
+      const double r0 = -(px + vF[1]) * dx * dy * dz;
+      const double r1 = -(pz + vF[3]) * dx * dy * dz;
+      Res[b] = r0;
+      Res[b + PLANE] = r1;
+      if (fuse && RK == 3) {
+        acc[5] += r0;
+        acc[6] += r1;
+      }
+    }
+    __syncthreads();
+  }
+  if (fuse) {
+    for (int ip = i1 - 3 < i0 ? i0 : i1 - 3; ip <= i1; ++ip) update_plane(ip);
+    if (RK == 3) {
+      __shared__ double sh[(WG / 64) * 7];
+      block_sum<7>(acc, sh);
+      if (threadIdx.x == 0)
+#pragma unroll
+        for (int m = 0; m < 7; ++m) partials[(size_t)blockIdx.x * 8 + m] = acc[m];
+    }
+  }
+}
+
 // RK stage update of fluid cells (Flux.cpp:2417-2599).  At RK3 each block
 // also writes its partial sums of the five change norms (:2530-2566) and of
 // the two force-density components (:2603-2625).
@@ -790,6 +1203,16 @@ void launch_flux(hipStream_t s, const double* U, const double* U1, const double*
   if (ncube > 1)
     hipLaunchKernelGGL(k_flux, dim3(16, ncube - 1), dim3(WG), 0, s, U, U1, U1q, fws, cube_size,
                        Rku, Res, ph, deltaT, deltaTau, e);
+}
+
+void launch_flux_march(hipStream_t s, double* U, const double* U1, const double* U1q,
+                       const int8_t* fws, const double* cube_size, double* Rku, double* Res,
+                       int ncube, DevPhys ph, double deltaT, double deltaTau, double e, int nseg,
+                       int RK, int fuse, double* U1p1, double* U1p2, double* partials) {
+  if (ncube > 1)
+    hipLaunchKernelGGL(k_flux_march, dim3((ncube - 1) * nseg), dim3(WG), 0, s, U, U1, U1q, fws,
+                       cube_size, Rku, Res, ph, deltaT, deltaTau, e, nseg, RK, fuse && nseg == 1,
+                       U1p1, U1p2, partials);
 }
 
 void launch_update(hipStream_t s, int RK, double* U, double* U1p1, double* U1p2,

@@ -157,7 +157,8 @@ int bcm_engine_set_time(bcm_engine* e, double deltaT, double deltaTau, double e_
  * receives er[1..7] after every inner iteration (7 doubles each). */
 int bcm_engine_run(bcm_engine* e, int n_steps, int n_inner, double* er_hist, int use_graph);
 /* One piece of a stage: 0 domain BCs, 1 IB ghost fill, 2 halo + edges,
- * 3 flux, 4 RK update (with norms at RK 3), 5 time shift, 6 whole stage. */
+ * 3 flux, 4 RK update (with norms at RK 3), 5 time shift, 6 whole stage,
+ * 7 flux and RK update as a stage runs them (fused when the choice allows). */
 int bcm_engine_part(bcm_engine* e, int part, int RK);
 /* which: 0 U1_, 1 U1, 2 U1q, 3 U1p1, 4 U1p2, 5 Rku1, 6 Residual1. */
 int bcm_engine_get_state(bcm_engine* e, int which, double* aos);
@@ -168,6 +169,17 @@ int bcm_engine_sync(bcm_engine* e);
 /* Average device time in ms over reps launches of one part (events on the
  * engine stream). */
 int bcm_engine_time_part(bcm_engine* e, int part, int RK, int reps, double* ms);
+/* Flux kernel choice: variant 0 (one workgroup per cell plane) or 1 (one
+ * workgroup per cube segment marching along i, nseg in {1,2,4,8,16}); fuse
+ * non-zero lets variant 1 with nseg 1 apply the RK update in the same kernel.
+ * All choices give bitwise the same results.  Setting a choice disables
+ * tuning. */
+int bcm_engine_set_flux(bcm_engine* e, int variant, int nseg, int fuse);
+/* Times the flux choices (flux + RK update of one stage) on the loaded case,
+ * restoring the state afterwards, and keeps the fastest; runs automatically
+ * before the first bcm_engine_run otherwise.  out3 (may be null) receives
+ * {variant, nseg, fuse}. */
+int bcm_engine_autotune(bcm_engine* e, int* out3);
 /* Ghost schedule statistics: entries, cubes with entries, maximum levels. */
 int bcm_engine_ghost_stats(bcm_engine* e, int* stats3);
 

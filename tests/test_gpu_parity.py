@@ -57,7 +57,7 @@ def test_each_kernel_bitwise(B, O, engine, name):
         engine.part("bc", RK); orc.bc(); compare(engine, orc, ("U1_",))
         engine.part("ghost", RK); orc.ghost(); compare(engine, orc, ("U1_",))
         engine.part("interface", RK); orc.interface(); compare(engine, orc, ("U1_",))
-        engine.part("flux", RK); engine.part("update", RK); orc.flux_update(RK)
+        engine.part("flux_update", RK); orc.flux_update(RK)
         compare(engine, orc, ("U1_", "Rku1", "Residual1", "U1p1", "U1p2"))
         engine.part("interface", RK); orc.interface(); compare(engine, orc, ("U1_",))
     assert er_close(engine.er()[1:], orc.er[1:])
@@ -74,6 +74,38 @@ def test_run_matches_reference_fixture(B, engine, name):
     for n in STATE:
         assert sha(engine.get_state(n)) == g["sha256"][f"final_{n}.bin"], n
     assert er_close(hist, g["er"])
+
+
+FLUX_CHOICES = [(0, 1, False), (1, 1, True), (1, 1, False), (1, 2, False), (1, 4, False),
+                (1, 16, False)]
+
+
+@pytest.mark.parametrize("variant,nseg,fuse", FLUX_CHOICES)
+def test_flux_variants_bitwise(B, O, engine, variant, nseg, fuse):
+    # Every flux kernel choice the autotuner may pick gives the oracle's bits,
+    # including cubes where neighbouring cells choose different face orders,
+    # and the RK update fused behind the marching flux.
+    case = B.Case(B.case_params("tiny"))
+    engine.load_case(case)
+    engine.set_flux(variant, nseg, fuse)
+    orc = O.OracleRun(case, nthreads=8)
+    h = engine.run(1, 2)
+    ho = orc.run(1, 2)
+    compare(engine, orc, STATE)
+    assert er_close(h, ho)
+
+
+def test_autotune_choice_is_bitwise(B, O, engine):
+    case = B.Case(B.case_params("cfg1"))
+    engine.load_case(case)
+    choice = engine.autotune()
+    assert choice["variant"] in (0, 1)
+    # tuning restores the state it timed on
+    assert sha(engine.get_state("U1_")) == sha(case.array("U1_"))
+    orc = O.OracleRun(case, nthreads=8)
+    engine.run(1, 1)
+    orc.run(1, 1)
+    compare(engine, orc, STATE)
 
 
 def test_graph_replays_match_eager(B, engine):
