@@ -337,8 +337,9 @@ int bcm_engine_part(bcm_engine* h, int part, int RK) {
     case 3: engine_flux(e, RK); break;
     case 4: engine_update(e, RK, 0); break;
     case 5: engine_shift(e); break;
-    case 6: engine_stage(e, RK, 0); break;
+    case 6: engine_stage(e, RK, 0, e->halo_full_next); e->halo_full_next = false; break;
     case 7: engine_flux_update(e, RK, 0); break;
+    case 8: launch_halo_sparse(e->stream, e->U, e->halo_pairs, e->n_halo_pairs, e->ncube); break;
     default: return fail("unknown part");
   }
   std::string err;
@@ -391,6 +392,7 @@ int bcm_engine_time_part(bcm_engine* h, int part, int RK, int reps, double* ms) 
       case 5: engine_shift(e); break;
       case 6: engine_stage(e, RK, 0); break;
       case 7: engine_flux_update(e, RK, 0); break;
+      case 8: launch_halo_sparse(e->stream, e->U, e->halo_pairs, e->n_halo_pairs, e->ncube); break;
       default: return fail("unknown part");
     }
   }

@@ -59,7 +59,8 @@ void free_case(Engine* e) {
   free_graph(e);
   void* ptrs[] = {e->U, e->U1, e->U1q, e->U1p1, e->U1p2, e->Rku, e->Res, e->staging,
                   e->fws, e->cube_size, e->bc_cubes, e->bc_masks, e->ghost,
-                  e->cube_level_off, e->level_off, e->halo, e->partials, e->er_dev};
+                  e->cube_level_off, e->level_off, e->halo, e->partials, e->er_dev,
+                  e->halo_pairs};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (e->er_host) hipHostFree(e->er_host);
@@ -71,6 +72,8 @@ void free_case(Engine* e) {
   e->ghost = nullptr;
   e->cube_level_off = e->level_off = nullptr;
   e->halo = nullptr;
+  e->halo_pairs = nullptr;
+  e->n_halo_pairs = 0;
   e->partials = e->er_dev = e->er_host = nullptr;
   e->er_slots = 0;
   e->ncube = 0;
@@ -261,6 +264,38 @@ int engine_load_case(Engine* e, const Case& cs, std::string& err) {
         hs[c0].src[2 * d + 1] = c2;  // c0's plus face from c2's bottom layers
       }
     if (dev_upload(&e->halo, hs, err)) return 1;
+    // Cells of the IB lists that some neighbour gathers: the only interior
+    // cells that change between a post-update exchange and the next stage's
+    // pre-flux exchange.
+    std::vector<char> is_ghost((size_t)cs.ncube * PLANE, 0);
+    for (const IBTable* tb : {&cs.ib_minus, &cs.ib_plus})
+      for (int n = 1; n <= tb->n; ++n) {
+        const int* g = &tb->gcindex[(n - 1) * 4 + 1];
+        is_ghost[(size_t)g[0] * PLANE + g[1] * 400 + g[2] * 20 + g[3]] = 1;
+      }
+    std::vector<int2> pairs;
+    for (int n = 1; n < cs.ncube; ++n)
+      for (int f = 0; f < 6; ++f) {
+        const int sc = hs[n].src[f];
+        if (sc <= 0) continue;
+        const int ax = f >> 1;
+        const bool plus = f & 1;
+        for (int l = 0; l < 2; ++l)
+          for (int p = ILO; p <= IHI; ++p)
+            for (int r = ILO; r <= IHI; ++r) {
+              const int dl = plus ? 18 + l : l, sl = plus ? 2 + l : 16 + l;
+              const int si = ax == 0 ? sl : p, sj = ax == 1 ? sl : (ax == 0 ? p : r),
+                        sk = ax == 2 ? sl : r;
+              const int di = ax == 0 ? dl : p, dj = ax == 1 ? dl : (ax == 0 ? p : r),
+                        dk = ax == 2 ? dl : r;
+              const int scell = si * 400 + sj * 20 + sk;
+              if (!is_ghost[(size_t)sc * PLANE + scell]) continue;
+              pairs.push_back(make_int2(sc * PLANE + scell, n * PLANE + di * 400 + dj * 20 + dk));
+            }
+      }
+    e->n_halo_pairs = (int)pairs.size();
+    if (dev_upload(&e->halo_pairs, pairs, err)) return 1;
+    e->halo_full_next = false;
   }
   // State upload.
   const std::vector<double>* st[3] = {&cs.U1_, &cs.U1, &cs.U1q};
@@ -276,6 +311,7 @@ int engine_load_case(Engine* e, const Case& cs, std::string& err) {
 int engine_upload_state(Engine* e, int which, const double* aos) {
   double* d = state_ptr(e, which);
   if (!d) return 1;
+  if (which == 0) e->halo_full_next = true;
   if (hipMemcpyAsync(e->staging, aos, e->state_bytes, hipMemcpyHostToDevice, e->stream) != hipSuccess)
     return 1;
   launch_aos_to_soa(e->stream, e->staging, d, e->ncube);
@@ -399,10 +435,13 @@ void engine_update(Engine* e, int RK, int er_slot) {
 
 void engine_shift(Engine* e) { launch_shift(e->stream, e->U, e->U1, e->U1q, e->ncube); }
 
-void engine_stage(Engine* e, int RK, int er_slot) {
+void engine_stage(Engine* e, int RK, int er_slot, bool full_pre_halo) {
   engine_bc(e);
   engine_ghost(e);
-  engine_interface(e);
+  if (full_pre_halo || !e->halo_pairs)
+    engine_interface(e);
+  else
+    launch_halo_sparse(e->stream, e->U, e->halo_pairs, e->n_halo_pairs, e->ncube);
   engine_flux_update(e, RK, er_slot);
   engine_interface(e);
 }
@@ -422,9 +461,9 @@ int ensure_er(Engine* e, int slots, std::string& err) {
 
 namespace {
 
-void enqueue_step(Engine* e, int n_inner) {
+void enqueue_step(Engine* e, int n_inner, bool full_first) {
   for (int it = 0; it < n_inner; ++it)
-    for (int RK = 1; RK <= 3; ++RK) engine_stage(e, RK, it);
+    for (int RK = 1; RK <= 3; ++RK) engine_stage(e, RK, it, full_first && it == 0 && RK == 1);
   engine_shift(e);
 }
 
@@ -445,17 +484,22 @@ int engine_run(Engine* e, int n_steps, int n_inner, double* er_hist, bool use_gr
   if (use_graph && (!e->graph_valid || e->graph_inner != n_inner)) {
     free_graph(e);
     HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step(e, n_inner);
+    enqueue_step(e, n_inner, false);
     HIP_TRY(hipStreamEndCapture(e->stream, &e->graph));
     HIP_TRY(hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0));
     e->graph_valid = true;
     e->graph_inner = n_inner;
   }
   for (int s = 0; s < n_steps; ++s) {
-    if (use_graph) {
+    if (e->halo_full_next) {
+      // The inner iterate was set from outside: this step starts with the
+      // full pre-flux exchange.
+      enqueue_step(e, n_inner, true);
+      e->halo_full_next = false;
+    } else if (use_graph) {
       HIP_TRY(hipGraphLaunch(e->graph_exec, e->stream));
     } else {
-      enqueue_step(e, n_inner);
+      enqueue_step(e, n_inner, false);
     }
     if (er_hist) {
       HIP_TRY(hipMemcpyAsync(e->er_host, e->er_dev, (size_t)n_inner * 8 * sizeof(double),

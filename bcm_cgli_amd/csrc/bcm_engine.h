@@ -48,6 +48,13 @@ struct Engine {
   int n_ghost_entries = 0;
   int ghost_levels = 0;
   HaloSrc* halo = nullptr;
+  // Pre-flux halo refresh: (source, destination) cells of the IB ghost cells
+  // that lie in a neighbour's gather source layers.
+  int2* halo_pairs = nullptr;
+  int n_halo_pairs = 0;
+  // Set when the inner iterate was replaced from outside: the next pre-flux
+  // exchange must then be the full one.
+  bool halo_full_next = false;
   double* partials = nullptr;
   double* er_dev = nullptr;     // 8 doubles per inner iteration slot
   int er_slots = 0;
@@ -91,7 +98,7 @@ void engine_update(Engine* e, int RK, int er_slot);
 void engine_flux_update(Engine* e, int RK, int er_slot);
 bool engine_fused(const Engine* e);
 void engine_shift(Engine* e);
-void engine_stage(Engine* e, int RK, int er_slot);
+void engine_stage(Engine* e, int RK, int er_slot, bool full_pre_halo = false);
 // n_steps physical steps of n_inner inner iterations; er_hist (may be null)
 // receives er[1..7] per inner iteration.
 int engine_run(Engine* e, int n_steps, int n_inner, double* er_hist, bool use_graph,

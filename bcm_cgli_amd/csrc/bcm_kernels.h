@@ -35,6 +35,9 @@ void launch_bc(hipStream_t s, double* U, const int* cubes, const uint8_t* masks,
 void launch_ghost(hipStream_t s, double* U, const GhostEntry* e, const int* cube_level_off,
                   const int* level_off, const int* cubes, int ncubes_with_ghosts, DevPhys ph);
 void launch_halo_edge(hipStream_t s, double* U, const HaloSrc* src, int ncube);
+// Pre-flux refresh: copy (source, destination) cell pairs (cube*8000 + cell),
+// then the edge passes of every cube.
+void launch_halo_sparse(hipStream_t s, double* U, const int2* pairs, int npairs, int ncube);
 void launch_flux(hipStream_t s, const double* U, const double* U1, const double* U1q,
                  const int8_t* fws, const double* cube_size, double* Rku, double* Res,
                  int ncube, DevPhys ph, double deltaT, double deltaTau, double e);

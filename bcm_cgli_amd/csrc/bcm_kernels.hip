@@ -1072,37 +1072,11 @@ __global__ __launch_bounds__(WG) void k_ghost(double* __restrict__ U,
   }
 }
 
-// ---------------------------------------------------------------------------
-// Cube-to-cube halo of equal-size neighbours as a gather into each cube's own
-// face layers (Interface.cpp:1745-1782, 2604-2641, 3474-3511), then the
-// cube's edge-ghost averaging in the reference's three sequential passes
-// (:5640-5732).  Sources are interior layers, which nothing in this kernel
-// writes, so cubes proceed independently.
-__global__ __launch_bounds__(WG) void k_halo_edge(double* __restrict__ U,
-                                                  const HaloSrc* __restrict__ srcs) {
-  const int c = blockIdx.x + 1;
-  const HaloSrc hs = srcs[c];
+// Edge-ghost averaging of one cube in the reference's three sequential passes
+// (Interface.cpp:5640-5732): i-edges, then j-edges, then k-edges.  Called by a
+// whole workgroup; the face ghosts it reads must be complete.
+__device__ __forceinline__ void edge_fill(double* __restrict__ U, int c) {
   const int t = threadIdx.x;
-  const int p = (t >> 4) + 2, r = (t & 15) + 2;
-#pragma unroll
-  for (int f = 0; f < 6; ++f) {
-    const int sc = hs.src[f];
-    if (sc <= 0) continue;
-    const int ax = f >> 1;
-    const bool plus = f & 1;
-#pragma unroll
-    for (int l = 0; l < 2; ++l) {
-      const int dl = plus ? 18 + l : l;      // destination layer
-      const int sl = plus ? 2 + l : 16 + l;  // source layer
-      const int di = ax == 0 ? dl : p, dj = ax == 1 ? dl : (ax == 0 ? p : r),
-                dk = ax == 2 ? dl : r;
-      const int si = ax == 0 ? sl : p, sj = ax == 1 ? sl : (ax == 0 ? p : r),
-                sk = ax == 2 ? sl : r;
-#pragma unroll
-      for (int v = 0; v < 5; ++v) U[sidx(c, v, di, dj, dk)] = U[sidx(sc, v, si, sj, sk)];
-    }
-  }
-  __syncthreads();
   // i-edges at (j,k) in {1,18}^2, i = 2..17.
   for (int w = t; w < 16 * 4 * 5; w += WG) {
     const int i = (w >> 2) / 5 + 2, e = w & 3, v = (w >> 2) % 5;
@@ -1138,6 +1112,61 @@ __global__ __launch_bounds__(WG) void k_halo_edge(double* __restrict__ U,
     const int ii = (e >= 2) ? 18 : 1, jj = (e == 1 || e == 3) ? 18 : 1;
     U[sidx(c, v, ii, jj, k)] = x;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Cube-to-cube halo of equal-size neighbours as a gather into each cube's own
+// face layers (Interface.cpp:1745-1782, 2604-2641, 3474-3511), then the
+// cube's edge-ghost averaging in the reference's three sequential passes
+// (:5640-5732).  Sources are interior layers, which nothing in this kernel
+// writes, so cubes proceed independently.
+__global__ __launch_bounds__(WG) void k_halo_edge(double* __restrict__ U,
+                                                  const HaloSrc* __restrict__ srcs) {
+  const int c = blockIdx.x + 1;
+  const HaloSrc hs = srcs[c];
+  const int t = threadIdx.x;
+  const int p = (t >> 4) + 2, r = (t & 15) + 2;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    const int sc = hs.src[f];
+    if (sc <= 0) continue;
+    const int ax = f >> 1;
+    const bool plus = f & 1;
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+      const int dl = plus ? 18 + l : l;      // destination layer
+      const int sl = plus ? 2 + l : 16 + l;  // source layer
+      const int di = ax == 0 ? dl : p, dj = ax == 1 ? dl : (ax == 0 ? p : r),
+                dk = ax == 2 ? dl : r;
+      const int si = ax == 0 ? sl : p, sj = ax == 1 ? sl : (ax == 0 ? p : r),
+                sk = ax == 2 ? sl : r;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) U[sidx(c, v, di, dj, dk)] = U[sidx(sc, v, si, sj, sk)];
+    }
+  }
+  __syncthreads();
+  edge_fill(U, c);
+}
+
+// Pre-flux halo refresh.  Between the post-update exchange and the next
+// stage's pre-flux exchange the only interior cells that change are the IB
+// ghost cells written by the ghost fill, and the domain BCs only write ghost
+// layers that no gather writes.  The full gather therefore reduces to copying
+// those ghost cells that lie in a neighbour's source layers, followed by the
+// edge passes, which read face ghosts the BCs may have changed.
+__global__ __launch_bounds__(WG) void k_halo_sparse(double* __restrict__ U,
+                                                    const int2* __restrict__ pairs, int n) {
+  const int g = blockIdx.x * WG + threadIdx.x;
+  if (g >= n) return;
+  const int2 pr = pairs[g];  // (source flat cell, destination flat cell), cube*8000 + cell
+  const size_t so = (size_t)(pr.x / PLANE) * CUBE_STRIDE + pr.x % PLANE;
+  const size_t dd = (size_t)(pr.y / PLANE) * CUBE_STRIDE + pr.y % PLANE;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) U[dd + (size_t)v * PLANE] = U[so + (size_t)v * PLANE];
+}
+
+__global__ __launch_bounds__(WG) void k_edge(double* __restrict__ U) {
+  edge_fill(U, blockIdx.x + 1);
 }
 
 // Physical-time shift of interior cells (main.cpp:1292-1315).
@@ -1195,6 +1224,13 @@ void launch_ghost(hipStream_t s, double* U, const GhostEntry* e, const int* cube
 
 void launch_halo_edge(hipStream_t s, double* U, const HaloSrc* src, int ncube) {
   if (ncube > 1) hipLaunchKernelGGL(k_halo_edge, dim3(ncube - 1), dim3(WG), 0, s, U, src);
+}
+
+void launch_halo_sparse(hipStream_t s, double* U, const int2* pairs, int npairs, int ncube) {
+  if (npairs > 0)
+    hipLaunchKernelGGL(k_halo_sparse, dim3((npairs + WG - 1) / WG), dim3(WG), 0, s, U, pairs,
+                       npairs);
+  if (ncube > 1) hipLaunchKernelGGL(k_edge, dim3(ncube - 1), dim3(WG), 0, s, U);
 }
 
 void launch_flux(hipStream_t s, const double* U, const double* U1, const double* U1q,

@@ -56,7 +56,8 @@ def test_each_kernel_bitwise(B, O, engine, name):
     for RK in (1, 2, 3):
         engine.part("bc", RK); orc.bc(); compare(engine, orc, ("U1_",))
         engine.part("ghost", RK); orc.ghost(); compare(engine, orc, ("U1_",))
-        engine.part("interface", RK); orc.interface(); compare(engine, orc, ("U1_",))
+        # pre-flux exchange as the engine runs it: IB ghost copies + edge passes
+        engine.part("halo_refresh", RK); orc.interface(); compare(engine, orc, ("U1_",))
         engine.part("flux_update", RK); orc.flux_update(RK)
         compare(engine, orc, ("U1_", "Rku1", "Residual1", "U1p1", "U1p2"))
         engine.part("interface", RK); orc.interface(); compare(engine, orc, ("U1_",))
@@ -105,6 +106,22 @@ def test_autotune_choice_is_bitwise(B, O, engine):
     orc = O.OracleRun(case, nthreads=8)
     engine.run(1, 1)
     orc.run(1, 1)
+    compare(engine, orc, STATE)
+
+
+def test_full_exchange_after_external_state(B, O, engine):
+    # After the inner iterate is replaced from outside, the first pre-flux
+    # exchange must be the full one; the oracle always runs the full one.
+    case = B.Case(B.case_params("tiny"))
+    engine.load_case(case)
+    rng = np.random.default_rng(5)
+    U = case.array("U1_").copy()
+    U[1:, 2:18, 2:18, 2:18, 0] *= 1 + 1e-3 * rng.uniform(-1, 1, size=U[1:, 2:18, 2:18, 2:18, 0].shape)
+    engine.set_state("U1_", U)
+    orc = O.OracleRun(case, nthreads=8)
+    orc.state["U1_"][...] = U
+    engine.run(1, 2)
+    orc.run(1, 2)
     compare(engine, orc, STATE)
 
 
