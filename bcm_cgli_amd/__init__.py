@@ -75,6 +75,14 @@ def load_library(path=LIB_PATH):
     lib.bcm_engine_ghost_stats.argtypes = [c_void_p, c_void_p]
     lib.bcm_engine_set_flux.argtypes = [c_void_p, c_int, c_int, c_int]
     lib.bcm_engine_autotune.argtypes = [c_void_p, c_void_p]
+    lib.bcm_bytes_per_cube.restype = ctypes.c_longlong
+    lib.bcm_bytes_per_cube.argtypes = []
+    lib.bcm_engine_alloc_pool.argtypes = [c_void_p, c_int, c_int]
+    lib.bcm_engine_add_case.argtypes = [c_void_p, c_void_p, P(c_int)]
+    lib.bcm_engine_remove_case.argtypes = [c_void_p, c_int]
+    lib.bcm_engine_case_state.argtypes = [c_void_p, c_int, c_int, c_void_p, c_int]
+    lib.bcm_engine_case_er.argtypes = [c_void_p, c_int, c_int, c_void_p]
+    lib.bcm_engine_pool_info.argtypes = [c_void_p, c_void_p]
     _lib = lib
     return lib
 
@@ -152,8 +160,14 @@ class Case:
             pass
 
 
+def bytes_per_cube():
+    """Device memory one pool cube takes."""
+    return int(load_library().bcm_bytes_per_cube())
+
+
 class Engine:
-    """One device-resident case on one GPU."""
+    """Device-resident cases on one GPU: one case (load_case) or a pool of
+    independent cases that run together (alloc_pool / add_case)."""
 
     def __init__(self, device=0):
         lib = load_library()
@@ -161,19 +175,71 @@ class Engine:
         _check(lib.bcm_engine_create(int(device), ctypes.byref(h)))
         self._h = h
         self.case = None
+        self.max_cases = 1
+        self.slots = {}
 
     def load_case(self, case):
         _check(_lib.bcm_engine_load_case(self._h, case._h))
         self.case = case
+        self.max_cases = 1
+        self.slots = {0: case}
+
+    def alloc_pool(self, cap_cubes, max_cases):
+        _check(_lib.bcm_engine_alloc_pool(self._h, int(cap_cubes), int(max_cases)))
+        self.case = None
+        self.max_cases = int(max_cases)
+        self.slots = {}
+
+    def add_case(self, case):
+        slot = ctypes.c_int()
+        _check(_lib.bcm_engine_add_case(self._h, case._h, ctypes.byref(slot)))
+        self.slots[slot.value] = case
+        return slot.value
+
+    def remove_case(self, slot):
+        _check(_lib.bcm_engine_remove_case(self._h, int(slot)))
+        self.slots.pop(slot, None)
+
+    def case_state(self, slot, name):
+        case = self.slots[slot]
+        out = np.empty((case.ncube, 20, 20, 20, 5))
+        _check(_lib.bcm_engine_case_state(self._h, int(slot), STATE_NAMES.index(name),
+                                          out.ctypes.data, 0))
+        return out
+
+    def set_case_state(self, slot, name, arr):
+        case = self.slots[slot]
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.size == case.ncube * 40000
+        _check(_lib.bcm_engine_case_state(self._h, int(slot), STATE_NAMES.index(name),
+                                          a.ctypes.data, 1))
+
+    def case_er(self, slot, inner=0):
+        out = np.zeros(8)
+        _check(_lib.bcm_engine_case_er(self._h, int(slot), int(inner), out.ctypes.data))
+        return out
+
+    def pool_info(self):
+        v = (ctypes.c_int * 6)()
+        _check(_lib.bcm_engine_pool_info(self._h, v))
+        keys = ("cap_cubes", "span_cubes", "active", "max_cases", "free_range", "used_cubes")
+        return dict(zip(keys, list(v)))
 
     def set_time(self, deltaT=1e-4, deltaTau=1e-4 / 200.0, e=0.1):
         _check(_lib.bcm_engine_set_time(self._h, deltaT, deltaTau, e))
 
     def run(self, n_steps, n_inner, record_er=True, use_graph=True):
-        er = np.zeros((max(n_steps, 1) * n_inner, 7)) if record_er else None
+        """Runs every resident case.  Returns er[1..7] per inner iteration:
+        shape (n_steps*n_inner, 7) for a single case, (n_steps*n_inner,
+        max_cases, 7) for a pool."""
+        mc = self.max_cases
+        er = np.zeros((max(n_steps, 1) * n_inner, mc, 7)) if record_er else None
         ptr = er.ctypes.data if record_er else None
         _check(_lib.bcm_engine_run(self._h, int(n_steps), int(n_inner), ptr, int(bool(use_graph))))
-        return er[: n_steps * n_inner] if record_er else None
+        if not record_er:
+            return None
+        er = er[: n_steps * n_inner]
+        return er[:, 0, :] if self.case is not None else er
 
     def part(self, name, RK=1):
         _check(_lib.bcm_engine_part(self._h, PARTS[name], int(RK)))

@@ -21,7 +21,32 @@ CONFIGS = {
 }
 
 # Stepping of each case: (physical steps, inner iterations per step).
-STEPPING = {"tiny": (2, 3), "cfg1": (20, 10), "cfg2": (200, 10)}
+STEPPING = {"tiny": (2, 3), "cfg1": (20, 10), "cfg2": (200, 10), "batch": (50, 10)}
+
+# Cube lattices of the batched workload: n^3 cubes on the unit box.
+BATCH_LATTICES = (2, 3, 4, 5, 6, 8)
+
+
+def _rng(seed):
+    import numpy as np
+    return np.random.default_rng(seed)
+
+
+def batch_cases(n_cases=64, seed=20260502, lattices=BATCH_LATTICES, ico_level=4):
+    """Independent sphere cases of the batched workload, drawn from seed:
+    an n^3 lattice of cubes on the unit box with n uniform in lattices,
+    M uniform in [0.1, 0.5], Re uniform in [50, 1000] and the sphere centre
+    offset from the box centre by up to +-0.1 per axis.  Each case gets its
+    own density-noise seed."""
+    r = _rng(seed)
+    out = []
+    for c in range(n_cases):
+        n = int(r.choice(lattices))
+        centre = tuple(float(0.5 + r.uniform(-0.1, 0.1)) for _ in range(3))
+        out.append(dict(_BASE, lattice=(n, n, n), cube_size=1.0 / n, sphere_center=centre,
+                        ico_level=ico_level, mach=float(r.uniform(0.1, 0.5)),
+                        reynolds=float(r.uniform(50.0, 1000.0)), seed=int(seed * 1000 + c)))
+    return out
 
 
 def case_params(name, **overrides):

@@ -120,7 +120,7 @@ void run_bc_axis(const char* where, int axis, int nl, int nu, int* l, int* u, do
     if (!b.cubes || !b.masks) fatal(where, "device allocation failed");
     it = d.bc.emplace(key, b).first;
   }
-  launch_bc(h->e->stream, h->e->U, it->second.cubes, it->second.masks, it->second.n, h->e->ph);
+  launch_bc(h->e->stream, h->e->U, it->second.cubes, it->second.masks, it->second.n, h->e->pt());
   check(where, hipGetLastError());
 }
 
@@ -157,8 +157,8 @@ void run_ghost(const char* where, int* NNBC, double* weight, int* GCindex, int* 
     if (!g.ent || !g.clo || !g.lo) fatal(where, "device allocation failed");
     it = d.gh.emplace(key, g).first;
   }
-  launch_ghost(h->e->stream, h->e->U, it->second.ent, it->second.clo, it->second.lo, nullptr,
-               it->second.ncubes, h->e->ph);
+  launch_ghost(h->e->stream, h->e->U, it->second.ent, it->second.clo, it->second.lo,
+               it->second.ncubes, h->e->pt());
   check(where, hipGetLastError());
 }
 }  // namespace
@@ -306,6 +306,71 @@ int bcm_engine_load_case(bcm_engine* h, const bcm_case* c) {
   return 0;
 }
 
+int bcm_engine_alloc_pool(bcm_engine* h, int cap_cubes, int max_cases) {
+  if (!h) return fail("null engine");
+  if (g_current == h) g_current = nullptr;
+  if (h->di) {
+    for (void* p : h->di->owned) hipFree(p);
+    h->di.reset();
+  }
+  std::string err;
+  if (engine_alloc(h->e, cap_cubes + 1, max_cases, err)) return fail(err);
+  h->e->flux_tuned = false;
+  return 0;
+}
+
+int bcm_engine_add_case(bcm_engine* h, const bcm_case* c, int* slot) {
+  if (!h || !c || !slot) return fail("null argument");
+  if (h->di) return fail("engine is bound to host arrays");
+  if (h->e->cap == 0) return fail("no pool allocated");
+  std::string err;
+  if (engine_add_case(h->e, c->cs, slot, err)) return fail(err);
+  return 0;
+}
+
+int bcm_engine_remove_case(bcm_engine* h, int slot) {
+  if (!h) return fail("null engine");
+  std::string err;
+  if (engine_remove_case(h->e, slot, err)) return fail(err);
+  return 0;
+}
+
+int bcm_engine_case_state(bcm_engine* h, int slot, int which, double* aos, int upload) {
+  if (!h || !aos) return fail("null argument");
+  if (which < 0 || which > 6) return fail("unknown state array");
+  std::string err;
+  if (engine_case_state(h->e, slot, which, aos, upload != 0, err)) return fail(err);
+  return 0;
+}
+
+int bcm_engine_case_er(bcm_engine* h, int slot, int inner, double* er8) {
+  if (!h || !er8) return fail("null argument");
+  Engine* e = h->e;
+  if (slot < 0 || slot >= e->max_cases) return fail("no such case slot");
+  if (inner < 0 || inner >= e->er_slots) return fail("no such inner iteration record");
+  if (hipMemcpy(er8, e->er_dev + ((size_t)inner * e->max_cases + slot) * 8, 8 * sizeof(double),
+                hipMemcpyDeviceToHost) != hipSuccess)
+    return fail("er copy failed");
+  return 0;
+}
+
+int bcm_engine_pool_info(bcm_engine* h, int* info6) {
+  if (!h || !info6) return fail("null argument");
+  const Engine* e = h->e;
+  info6[0] = e->cap > 0 ? e->cap - 1 : 0;
+  info6[1] = e->ncube > 0 ? e->ncube - 1 : 0;
+  info6[2] = e->n_active;
+  info6[3] = e->max_cases;
+  info6[4] = e->cap > 0 ? engine_free_range(e) : 0;
+  int used = 0;
+  for (const CaseSlot& s : e->cases)
+    if (s.active) used += s.tb.n;
+  info6[5] = used;
+  return 0;
+}
+
+long long bcm_bytes_per_cube(void) { return (long long)engine_bytes_per_cube(); }
+
 int bcm_engine_set_time(bcm_engine* h, double deltaT, double deltaTau, double e_floor) {
   if (!h) return fail("null engine");
   if (!(deltaT > 0) || !(deltaTau > 0)) return fail("time steps must be positive");
@@ -328,7 +393,7 @@ int bcm_engine_run(bcm_engine* h, int n_steps, int n_inner, double* er_hist, int
 int bcm_engine_part(bcm_engine* h, int part, int RK) {
   if (!h) return fail("null engine");
   Engine* e = h->e;
-  if (e->ncube <= 1) return fail("no case loaded");
+  if (e->n_active == 0) return fail("no case loaded");
   if (RK < 1 || RK > 3) return fail("RK must be 1, 2 or 3");
   switch (part) {
     case 0: engine_bc(e); break;
@@ -339,7 +404,7 @@ int bcm_engine_part(bcm_engine* h, int part, int RK) {
     case 5: engine_shift(e); break;
     case 6: engine_stage(e, RK, 0, e->halo_full_next); e->halo_full_next = false; break;
     case 7: engine_flux_update(e, RK, 0); break;
-    case 8: launch_halo_sparse(e->stream, e->U, e->halo_pairs, e->n_halo_pairs, e->ncube); break;
+    case 8: engine_halo_refresh(e); break;
     default: return fail("unknown part");
   }
   std::string err;
@@ -379,7 +444,7 @@ int bcm_engine_sync(bcm_engine* h) {
 int bcm_engine_time_part(bcm_engine* h, int part, int RK, int reps, double* ms) {
   if (!h || !ms) return fail("null argument");
   Engine* e = h->e;
-  if (e->ncube <= 1) return fail("no case loaded");
+  if (e->n_active == 0) return fail("no case loaded");
   if (reps < 1) return fail("reps must be positive");
   hipEventRecord(e->ev0, e->stream);
   for (int r = 0; r < reps; ++r) {
@@ -392,7 +457,7 @@ int bcm_engine_time_part(bcm_engine* h, int part, int RK, int reps, double* ms) 
       case 5: engine_shift(e); break;
       case 6: engine_stage(e, RK, 0); break;
       case 7: engine_flux_update(e, RK, 0); break;
-      case 8: launch_halo_sparse(e->stream, e->U, e->halo_pairs, e->n_halo_pairs, e->ncube); break;
+      case 8: engine_halo_refresh(e); break;
       default: return fail("unknown part");
     }
   }
@@ -452,11 +517,11 @@ int bcm_bind(bcm_engine* h, int ncube, int n_cubes_norm, double* U1_, double* U1
   if (h->di)
     for (void* p : h->di->owned) hipFree(p);
   h->di.reset(new DropIn());
-  if (engine_alloc(h->e, ncube, err)) return fail(err);
+  if (engine_alloc(h->e, ncube, 1, err)) return fail(err);
   Engine* e = h->e;
-  e->mpi_ncube = n_cubes_norm;
   const double rho0 = phys7[0], P0 = phys7[5];
-  e->ph = DevPhys{rho0, phys7[1], phys7[2], phys7[3], phys7[4], P0, P0 / rho0 / 287.0, phys7[6]};
+  const DevPhys ph{rho0, phys7[1], phys7[2], phys7[3], phys7[4], P0, P0 / rho0 / 287.0, phys7[6]};
+  if (engine_bind_single(e, n_cubes_norm, ph, err)) return fail(err);
   DropIn& d = *h->di;
   d.ncube = ncube;
   d.U1_ = U1_; d.U1 = U1; d.U1q = U1q; d.U1p1 = U1p1; d.U1p2 = U1p2;
@@ -613,7 +678,7 @@ void BCM_Interface(int myid, int ncube, int mPI_Nadj, int ncpu_bs, int ncpu_eq, 
     if (!dp) fatal(where, "device allocation failed");
     it = d.halo.emplace(key, dp).first;
   }
-  launch_halo_edge(h->e->stream, h->e->U, it->second, ncube);
+  launch_halo_edge(h->e->stream, h->e->U, it->second, ncube, h->e->cube_case);
   check(where, hipGetLastError());
 }
 

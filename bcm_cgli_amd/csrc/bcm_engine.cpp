@@ -55,33 +55,49 @@ void free_graph(Engine* e) {
   e->graph_valid = false;
 }
 
-void free_case(Engine* e) {
-  free_graph(e);
-  void* ptrs[] = {e->U, e->U1, e->U1q, e->U1p1, e->U1p2, e->Rku, e->Res, e->staging,
-                  e->fws, e->cube_size, e->bc_cubes, e->bc_masks, e->ghost,
-                  e->cube_level_off, e->level_off, e->halo, e->partials, e->er_dev,
-                  e->halo_pairs};
+// Tables of the resident cases (rebuilt whenever a case comes or goes).
+void free_tables(Engine* e) {
+  void* ptrs[] = {e->bc_cubes, e->bc_masks,  e->ghost,      e->cube_level_off,
+                  e->level_off, e->halo,     e->halo_pairs, e->case_desc};
   for (void* p : ptrs)
     if (p) hipFree(p);
-  if (e->er_host) hipHostFree(e->er_host);
-  e->U = e->U1 = e->U1q = e->U1p1 = e->U1p2 = e->Rku = e->Res = e->staging = nullptr;
-  e->fws = nullptr;
-  e->cube_size = nullptr;
   e->bc_cubes = nullptr;
   e->bc_masks = nullptr;
   e->ghost = nullptr;
   e->cube_level_off = e->level_off = nullptr;
   e->halo = nullptr;
   e->halo_pairs = nullptr;
-  e->n_halo_pairs = 0;
+  e->case_desc = nullptr;
+  e->n_halo_pairs = e->n_bc = e->n_ghost_cubes = e->n_ghost_entries = 0;
+}
+
+void free_case(Engine* e) {
+  free_graph(e);
+  free_tables(e);
+  void* ptrs[] = {e->U,   e->U1,        e->U1q,      e->U1p1,     e->U1p2,   e->Rku,
+                  e->Res, e->staging,   e->fws,      e->cube_size, e->partials, e->er_dev,
+                  e->phys_dev, e->cube_case};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  if (e->er_host) hipHostFree(e->er_host);
+  e->U = e->U1 = e->U1q = e->U1p1 = e->U1p2 = e->Rku = e->Res = e->staging = nullptr;
+  e->fws = nullptr;
+  e->cube_size = nullptr;
   e->partials = e->er_dev = e->er_host = nullptr;
+  e->phys_dev = nullptr;
+  e->cube_case = nullptr;
   e->er_slots = 0;
   e->ncube = 0;
+  e->cap = 0;
+  e->n_active = 0;
+  e->cases.clear();
+  e->cube_case_host.clear();
 }
 
 }  // namespace
 
 int ensure_er(Engine* e, int slots, std::string& err);
+int rebuild_tables(Engine* e, std::string& err);
 
 // ---------------------------------------------------------------------------
 GhostSchedule build_ghost_schedule(const std::vector<GhostEntry>& ordered) {
@@ -132,6 +148,11 @@ GhostSchedule build_ghost_schedule(const std::vector<GhostEntry>& ordered) {
   return gs;
 }
 
+size_t engine_bytes_per_cube() {
+  return (size_t)8 * CUBE_STRIDE * sizeof(double) + PLANE + sizeof(double) + sizeof(int) +
+         (size_t)16 * 8 * sizeof(double);
+}
+
 int engine_create(int device, Engine** out, std::string& err) {
   *out = nullptr;
   int ndev = 0;
@@ -164,65 +185,74 @@ void engine_destroy(Engine* e) {
   delete e;
 }
 
-int engine_alloc(Engine* e, int ncube, std::string& err) {
+int engine_alloc(Engine* e, int cap, int max_cases, std::string& err) {
   HIP_TRY(hipSetDevice(e->device));
   free_case(e);
-  if (ncube < 2) {
-    err = "a case needs at least one cube";
+  if (cap < 2) {
+    err = "a pool needs at least one cube";
     return 1;
   }
-  e->ncube = ncube;
-  e->mpi_ncube = ncube - 1;
-  const size_t ncells = (size_t)ncube * PLANE;
-  e->state_bytes = ncells * 5 * sizeof(double);
+  if (max_cases < 1) {
+    err = "a pool needs at least one case slot";
+    return 1;
+  }
+  e->cap = cap;
+  e->ncube = 1;
+  e->max_cases = max_cases;
+  e->cases.assign(max_cases, CaseSlot());
+  e->cube_case_host.assign(cap, -1);
+  e->n_active = 0;
+  const size_t ncells = (size_t)cap * PLANE;
+  const size_t bytes = ncells * 5 * sizeof(double);
   for (double** p : {&e->U, &e->U1, &e->U1q, &e->U1p1, &e->U1p2, &e->Rku, &e->Res, &e->staging}) {
     if (dev_alloc(p, ncells * 5, err)) return 1;
-    HIP_TRY(hipMemsetAsync(*p, 0, e->state_bytes, e->stream));
+    HIP_TRY(hipMemsetAsync(*p, 0, bytes, e->stream));
   }
   if (dev_alloc(&e->fws, ncells, err)) return 1;
   HIP_TRY(hipMemsetAsync(e->fws, 0, ncells, e->stream));
-  if (dev_alloc(&e->cube_size, (size_t)ncube, err)) return 1;
-  HIP_TRY(hipMemsetAsync(e->cube_size, 0, ncube * sizeof(double), e->stream));
-  if (dev_alloc(&e->partials, (size_t)update_blocks(ncube) * 8, err)) return 1;
+  if (dev_alloc(&e->cube_size, (size_t)cap, err)) return 1;
+  HIP_TRY(hipMemsetAsync(e->cube_size, 0, cap * sizeof(double), e->stream));
+  if (dev_alloc(&e->partials, (size_t)update_blocks(cap) * 8, err)) return 1;
+  if (dev_alloc(&e->phys_dev, (size_t)max_cases, err)) return 1;
+  if (dev_alloc(&e->cube_case, (size_t)cap, err)) return 1;
   if (ensure_er(e, 1, err)) return 1;
+  if (rebuild_tables(e, err)) return 1;
   HIP_TRY(hipStreamSynchronize(e->stream));
   return 0;
 }
 
-int engine_load_case(Engine* e, const Case& cs, std::string& err) {
-  if (engine_alloc(e, cs.ncube, err)) return 1;
-  e->flux_tuned = false;
-  e->ph = DevPhys{cs.phys.rho0, cs.phys.U0, cs.phys.Uin, cs.phys.V0,
+int build_case_tables(const Case& cs, CaseTables& tb, std::string& err) {
+  tb = CaseTables();
+  tb.n = cs.ncube - 1;
+  tb.mpi_ncube = cs.ncube - 1;
+  tb.ph = DevPhys{cs.phys.rho0, cs.phys.U0, cs.phys.Uin, cs.phys.V0,
                   cs.phys.W0,   cs.phys.P0, cs.phys.T0,  cs.phys.mu_L};
-  if (engine_upload_flags(e, cs.fws.data())) {
-    err = "flag upload failed";
-    return 1;
-  }
-  HIP_TRY(hipMemcpy(e->cube_size, cs.cube_size.data(), cs.ncube * sizeof(double),
-                    hipMemcpyHostToDevice));
   // Boundary cubes with their side masks.
   {
     std::vector<uint8_t> mask(cs.ncube, 0);
     for (int s = 0; s < 6; ++s)
-      for (size_t n = 1; n < cs.bc_lists[s].size(); ++n) mask[cs.bc_lists[s][n]] |= (uint8_t)(1 << s);
-    std::vector<int> cubes;
-    std::vector<uint8_t> m;
+      for (size_t n = 1; n < cs.bc_lists[s].size(); ++n) {
+        const int c = cs.bc_lists[s][n];
+        if (c < 1 || c >= cs.ncube) {
+          err = "boundary list names a missing cube";
+          return 1;
+        }
+        mask[c] |= (uint8_t)(1 << s);
+      }
     for (int c = 1; c < cs.ncube; ++c)
       if (mask[c]) {
-        cubes.push_back(c);
-        m.push_back(mask[c]);
+        tb.bc_cubes.push_back(c);
+        tb.bc_masks.push_back(mask[c]);
       }
-    e->n_bc = (int)cubes.size();
-    if (dev_upload(&e->bc_cubes, cubes, err) || dev_upload(&e->bc_masks, m, err)) return 1;
   }
   // IB ghost schedule: minus list, then plus list, each in list order.
   {
     std::vector<GhostEntry> ordered;
-    for (const IBTable* tb : {&cs.ib_minus, &cs.ib_plus})
-      for (int n = 1; n <= tb->n; ++n) {
+    for (const IBTable* t : {&cs.ib_minus, &cs.ib_plus})
+      for (int n = 1; n <= t->n; ++n) {
         GhostEntry E;
-        const int* g = &tb->gcindex[(n - 1) * 4 + 1];
-        const int* q = &tb->ipsur[(n - 1) * 4 + 1];
+        const int* g = &t->gcindex[(n - 1) * 4 + 1];
+        const int* q = &t->ipsur[(n - 1) * 4 + 1];
         if (g[0] != q[0]) {
           err = "IB entry interpolates from another cube";
           return 1;
@@ -236,83 +266,273 @@ int engine_load_case(Engine* e, const Case& cs, std::string& err) {
         E.cube = g[0];
         E.gi = (int16_t)g[1]; E.gj = (int16_t)g[2]; E.gk = (int16_t)g[3];
         E.pi = (int16_t)q[1]; E.pj = (int16_t)q[2]; E.pk = (int16_t)q[3];
-        for (int m = 0; m < 8; ++m) E.w[m] = tb->weight[(n - 1) * 8 + 1 + m];
+        for (int m = 0; m < 8; ++m) E.w[m] = t->weight[(n - 1) * 8 + 1 + m];
         ordered.push_back(E);
       }
-    GhostSchedule gs = build_ghost_schedule(ordered);
-    e->n_ghost_cubes = (int)gs.cube_level_off.size() - 1;
-    e->n_ghost_entries = (int)gs.entries.size();
-    e->ghost_levels = gs.max_levels;
-    if (dev_upload(&e->ghost, gs.entries, err) || dev_upload(&e->cube_level_off, gs.cube_level_off, err) ||
-        dev_upload(&e->level_off, gs.level_off, err))
-      return 1;
+    tb.ghost = build_ghost_schedule(ordered);
   }
   // Face halo sources from the equal-size interface lists.
-  {
-    std::vector<HaloSrc> hs(cs.ncube);
-    for (auto& h : hs) memset(&h, 0, sizeof h);
-    const int dirs[3] = {2, 4, 6};
-    for (int d = 0; d < 3; ++d)
-      for (int a = 1; a <= cs.nadj[d]; ++a) {
-        int c0 = cs.adj[d][a];
-        int c2 = cs.adj_number[adj_index(c0, 1, dirs[d])];
-        if (c2 <= 0 || c2 >= cs.ncube) {
-          err = "interface list names a missing neighbour";
-          return 1;
-        }
-        hs[c2].src[2 * d] = c0;      // c2's minus face from c0's top layers
-        hs[c0].src[2 * d + 1] = c2;  // c0's plus face from c2's bottom layers
+  tb.halo.resize(cs.ncube);
+  for (auto& h : tb.halo) memset(&h, 0, sizeof h);
+  const int dirs[3] = {2, 4, 6};
+  for (int d = 0; d < 3; ++d)
+    for (int a = 1; a <= cs.nadj[d]; ++a) {
+      int c0 = cs.adj[d][a];
+      int c2 = c0 >= 1 && c0 < cs.ncube ? cs.adj_number[adj_index(c0, 1, dirs[d])] : 0;
+      if (c0 < 1 || c0 >= cs.ncube || c2 <= 0 || c2 >= cs.ncube) {
+        err = "interface list names a missing neighbour";
+        return 1;
       }
-    if (dev_upload(&e->halo, hs, err)) return 1;
-    // Cells of the IB lists that some neighbour gathers: the only interior
-    // cells that change between a post-update exchange and the next stage's
-    // pre-flux exchange.
-    std::vector<char> is_ghost((size_t)cs.ncube * PLANE, 0);
-    for (const IBTable* tb : {&cs.ib_minus, &cs.ib_plus})
-      for (int n = 1; n <= tb->n; ++n) {
-        const int* g = &tb->gcindex[(n - 1) * 4 + 1];
-        is_ghost[(size_t)g[0] * PLANE + g[1] * 400 + g[2] * 20 + g[3]] = 1;
-      }
-    std::vector<int2> pairs;
-    for (int n = 1; n < cs.ncube; ++n)
-      for (int f = 0; f < 6; ++f) {
-        const int sc = hs[n].src[f];
-        if (sc <= 0) continue;
-        const int ax = f >> 1;
-        const bool plus = f & 1;
-        for (int l = 0; l < 2; ++l)
-          for (int p = ILO; p <= IHI; ++p)
-            for (int r = ILO; r <= IHI; ++r) {
-              const int dl = plus ? 18 + l : l, sl = plus ? 2 + l : 16 + l;
-              const int si = ax == 0 ? sl : p, sj = ax == 1 ? sl : (ax == 0 ? p : r),
-                        sk = ax == 2 ? sl : r;
-              const int di = ax == 0 ? dl : p, dj = ax == 1 ? dl : (ax == 0 ? p : r),
-                        dk = ax == 2 ? dl : r;
-              const int scell = si * 400 + sj * 20 + sk;
-              if (!is_ghost[(size_t)sc * PLANE + scell]) continue;
-              pairs.push_back(make_int2(sc * PLANE + scell, n * PLANE + di * 400 + dj * 20 + dk));
-            }
-      }
-    e->n_halo_pairs = (int)pairs.size();
-    if (dev_upload(&e->halo_pairs, pairs, err)) return 1;
-    e->halo_full_next = false;
+      tb.halo[c2].src[2 * d] = c0;      // c2's minus face from c0's top layers
+      tb.halo[c0].src[2 * d + 1] = c2;  // c0's plus face from c2's bottom layers
+    }
+  // Cells of the IB lists that some neighbour gathers: the only interior
+  // cells that change between a post-update exchange and the next stage's
+  // pre-flux exchange.
+  std::vector<char> is_ghost((size_t)cs.ncube * PLANE, 0);
+  for (const IBTable* t : {&cs.ib_minus, &cs.ib_plus})
+    for (int n = 1; n <= t->n; ++n) {
+      const int* g = &t->gcindex[(n - 1) * 4 + 1];
+      is_ghost[(size_t)g[0] * PLANE + g[1] * 400 + g[2] * 20 + g[3]] = 1;
+    }
+  for (int n = 1; n < cs.ncube; ++n)
+    for (int f = 0; f < 6; ++f) {
+      const int sc = tb.halo[n].src[f];
+      if (sc <= 0) continue;
+      const int ax = f >> 1;
+      const bool plus = f & 1;
+      for (int l = 0; l < 2; ++l)
+        for (int p = ILO; p <= IHI; ++p)
+          for (int r = ILO; r <= IHI; ++r) {
+            const int dl = plus ? 18 + l : l, sl = plus ? 2 + l : 16 + l;
+            const int si = ax == 0 ? sl : p, sj = ax == 1 ? sl : (ax == 0 ? p : r),
+                      sk = ax == 2 ? sl : r;
+            const int di = ax == 0 ? dl : p, dj = ax == 1 ? dl : (ax == 0 ? p : r),
+                      dk = ax == 2 ? dl : r;
+            const int scell = si * 400 + sj * 20 + sk;
+            if (!is_ghost[(size_t)sc * PLANE + scell]) continue;
+            tb.halo_pairs.push_back(
+                make_int2(sc * PLANE + scell, n * PLANE + di * 400 + dj * 20 + dk));
+          }
+    }
+  return 0;
+}
+
+// Concatenates the tables of all active cases in pool numbering and uploads
+// them with the cube -> case map and the per-case physics and norm records.
+int rebuild_tables(Engine* e, std::string& err) {
+  free_tables(e);
+  std::vector<int> order;
+  for (int s = 0; s < (int)e->cases.size(); ++s)
+    if (e->cases[s].active) order.push_back(s);
+  std::sort(order.begin(), order.end(),
+            [&](int a, int b) { return e->cases[a].first < e->cases[b].first; });
+  std::vector<int> bc;
+  std::vector<uint8_t> bm;
+  GhostSchedule gs;
+  std::vector<int2> pairs;
+  std::vector<CaseDesc> desc;
+  std::vector<DevPhys> phys(e->max_cases);
+  for (int s = 0; s < e->max_cases; ++s) phys[s] = e->cases[s].tb.ph;
+  int top = 1;
+  for (int s : order) {
+    const CaseSlot& cs = e->cases[s];
+    const CaseTables& tb = cs.tb;
+    const int off = cs.first - 1;
+    top = std::max(top, cs.first + tb.n);
+    for (size_t a = 0; a < tb.bc_cubes.size(); ++a) {
+      bc.push_back(tb.bc_cubes[a] + off);
+      bm.push_back(tb.bc_masks[a]);
+    }
+    const int e0 = (int)gs.entries.size(), l0 = (int)gs.level_off.size();
+    for (GhostEntry E : tb.ghost.entries) {
+      E.cube += off;
+      gs.entries.push_back(E);
+    }
+    for (size_t a = 0; a + 1 < tb.ghost.level_off.size(); ++a)
+      gs.level_off.push_back(tb.ghost.level_off[a] + e0);
+    for (size_t a = 0; a + 1 < tb.ghost.cube_level_off.size(); ++a)
+      gs.cube_level_off.push_back(tb.ghost.cube_level_off[a] + l0);
+    gs.max_levels = std::max(gs.max_levels, tb.ghost.max_levels);
+    for (int2 pr : tb.halo_pairs) pairs.push_back(make_int2(pr.x + off * PLANE, pr.y + off * PLANE));
+    desc.push_back(CaseDesc{cs.first, tb.n, tb.mpi_ncube, s});
   }
-  // State upload.
+  gs.cube_level_off.push_back((int)gs.level_off.size());
+  gs.level_off.push_back((int)gs.entries.size());
+  std::vector<HaloSrc> hs(top);
+  for (auto& h : hs) memset(&h, 0, sizeof h);
+  for (int s : order) {
+    const CaseSlot& cs = e->cases[s];
+    const int off = cs.first - 1;
+    for (int c = 1; c <= cs.tb.n && c < (int)cs.tb.halo.size(); ++c)
+      for (int f = 0; f < 6; ++f) {
+        const int src = cs.tb.halo[c].src[f];
+        hs[c + off].src[f] = src > 0 ? src + off : 0;
+      }
+  }
+  e->ncube = top;
+  e->n_active = (int)order.size();
+  e->n_bc = (int)bc.size();
+  e->n_ghost_cubes = (int)gs.cube_level_off.size() - 1;
+  e->n_ghost_entries = (int)gs.entries.size();
+  e->ghost_levels = gs.max_levels;
+  e->n_halo_pairs = (int)pairs.size();
+  if (dev_upload(&e->bc_cubes, bc, err) || dev_upload(&e->bc_masks, bm, err) ||
+      dev_upload(&e->ghost, gs.entries, err) ||
+      dev_upload(&e->cube_level_off, gs.cube_level_off, err) ||
+      dev_upload(&e->level_off, gs.level_off, err) || dev_upload(&e->halo, hs, err) ||
+      dev_upload(&e->halo_pairs, pairs, err) || dev_upload(&e->case_desc, desc, err))
+    return 1;
+  HIP_TRY(hipMemcpy(e->phys_dev, phys.data(), phys.size() * sizeof(DevPhys),
+                    hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(e->cube_case, e->cube_case_host.data(), e->cap * sizeof(int),
+                    hipMemcpyHostToDevice));
+  free_graph(e);
+  return 0;
+}
+
+int engine_free_range(const Engine* e) {
+  int best = 0, run = 0;
+  for (int c = 1; c < e->cap; ++c) {
+    run = e->cube_case_host[c] < 0 ? run + 1 : 0;
+    best = std::max(best, run);
+  }
+  return best;
+}
+
+namespace {
+// First pool cube of a free run of n cubes, or 0.
+int find_range(const Engine* e, int n) {
+  int run = 0;
+  for (int c = 1; c < e->cap; ++c) {
+    run = e->cube_case_host[c] < 0 ? run + 1 : 0;
+    if (run == n) return c - n + 1;
+  }
+  return 0;
+}
+}  // namespace
+
+int engine_add_case(Engine* e, const Case& cs, int* slot_out, std::string& err) {
+  HIP_TRY(hipSetDevice(e->device));
+  if (slot_out) *slot_out = -1;
+  int slot = -1;
+  for (int s = 0; s < e->max_cases; ++s)
+    if (!e->cases[s].active) {
+      slot = s;
+      break;
+    }
+  if (slot < 0) {
+    err = "all case slots are in use";
+    return 1;
+  }
+  CaseTables tb;
+  if (build_case_tables(cs, tb, err)) return 1;
+  const int first = find_range(e, tb.n);
+  if (first == 0) {
+    err = "no free range of " + std::to_string(tb.n) + " cubes in the pool";
+    return 1;
+  }
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  CaseSlot& S = e->cases[slot];
+  S.first = first;
+  S.tb = std::move(tb);
+  S.active = true;
+  const int n = S.tb.n;
+  // Flags, cube sizes and state of the range; RK scratch, increments and
+  // force densities start at zero as in the reference's fresh arrays.
+  {
+    std::vector<int8_t> f((size_t)n * PLANE);
+    for (size_t a = 0; a < f.size(); ++a) f[a] = (int8_t)cs.fws[PLANE + a];
+    HIP_TRY(hipMemcpy(e->fws + (size_t)first * PLANE, f.data(), f.size(), hipMemcpyHostToDevice));
+  }
+  HIP_TRY(hipMemcpy(e->cube_size + first, cs.cube_size.data() + 1, n * sizeof(double),
+                    hipMemcpyHostToDevice));
+  for (double* p : {e->U1p1, e->U1p2, e->Rku, e->Res})
+    HIP_TRY(hipMemsetAsync(p + (size_t)first * CUBE_STRIDE, 0,
+                           (size_t)n * CUBE_STRIDE * sizeof(double), e->stream));
+  for (int c = first; c < first + n; ++c) e->cube_case_host[c] = slot;
+  if (rebuild_tables(e, err)) return 1;
   const std::vector<double>* st[3] = {&cs.U1_, &cs.U1, &cs.U1q};
   for (int w = 0; w < 3; ++w)
-    if (engine_upload_state(e, w, st[w]->data())) {
-      err = "state upload failed";
-      return 1;
-    }
+    if (engine_case_state(e, slot, w, const_cast<double*>(st[w]->data()), true, err)) return 1;
+  e->halo_full_next = true;
+  if (slot_out) *slot_out = slot;
+  return 0;
+}
+
+int engine_remove_case(Engine* e, int slot, std::string& err) {
+  if (slot < 0 || slot >= e->max_cases || !e->cases[slot].active) {
+    err = "no such case slot";
+    return 1;
+  }
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  CaseSlot& S = e->cases[slot];
+  for (int c = S.first; c < S.first + S.tb.n; ++c) e->cube_case_host[c] = -1;
+  S = CaseSlot();
+  return rebuild_tables(e, err);
+}
+
+int engine_bind_single(Engine* e, int mpi_ncube, const DevPhys& ph, std::string& err) {
+  for (int s = 0; s < e->max_cases; ++s) e->cases[s] = CaseSlot();
+  CaseSlot& S = e->cases[0];
+  S.active = true;
+  S.first = 1;
+  S.tb.n = e->cap - 1;
+  S.tb.mpi_ncube = mpi_ncube;
+  S.tb.ph = ph;
+  for (int c = 0; c < e->cap; ++c) e->cube_case_host[c] = c == 0 ? -1 : 0;
+  return rebuild_tables(e, err);
+}
+
+int engine_load_case(Engine* e, const Case& cs, std::string& err) {
+  if (engine_alloc(e, cs.ncube, 1, err)) return 1;
+  e->flux_tuned = false;
+  int slot;
+  if (engine_add_case(e, cs, &slot, err)) return 1;
+  e->halo_full_next = false;
   HIP_TRY(hipStreamSynchronize(e->stream));
   return 0;
 }
+
+int engine_case_state(Engine* e, int slot, int which, double* aos, bool upload,
+                      std::string& err) {
+  if (slot < 0 || slot >= e->max_cases || !e->cases[slot].active) {
+    err = "no such case slot";
+    return 1;
+  }
+  double* d = state_ptr(e, which);
+  if (!d) {
+    err = "unknown state array";
+    return 1;
+  }
+  const CaseSlot& S = e->cases[slot];
+  const size_t cube_bytes = (size_t)CUBE_STRIDE * sizeof(double);
+  const size_t n = S.tb.n;
+  double* dst = d + (size_t)S.first * CUBE_STRIDE;
+  if (upload) {
+    if (which == 0) e->halo_full_next = true;
+    HIP_TRY(hipMemcpyAsync(e->staging, aos + CUBE_STRIDE, n * cube_bytes, hipMemcpyHostToDevice,
+                           e->stream));
+    launch_aos_to_soa(e->stream, e->staging, dst, (int)n);
+  } else {
+    launch_soa_to_aos(e->stream, dst, e->staging, (int)n);
+    HIP_TRY(hipMemcpyAsync(aos + CUBE_STRIDE, e->staging, n * cube_bytes, hipMemcpyDeviceToHost,
+                           e->stream));
+    memset(aos, 0, cube_bytes);
+  }
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  return 0;
+}
+
+namespace {
+size_t pool_bytes(const Engine* e) { return (size_t)e->ncube * CUBE_STRIDE * sizeof(double); }
+}  // namespace
 
 int engine_upload_state(Engine* e, int which, const double* aos) {
   double* d = state_ptr(e, which);
   if (!d) return 1;
   if (which == 0) e->halo_full_next = true;
-  if (hipMemcpyAsync(e->staging, aos, e->state_bytes, hipMemcpyHostToDevice, e->stream) != hipSuccess)
+  if (hipMemcpyAsync(e->staging, aos, pool_bytes(e), hipMemcpyHostToDevice, e->stream) != hipSuccess)
     return 1;
   launch_aos_to_soa(e->stream, e->staging, d, e->ncube);
   return hipStreamSynchronize(e->stream) == hipSuccess ? 0 : 1;
@@ -322,7 +542,7 @@ int engine_download_state(Engine* e, int which, double* aos) {
   double* d = state_ptr(e, which);
   if (!d) return 1;
   launch_soa_to_aos(e->stream, d, e->staging, e->ncube);
-  if (hipMemcpyAsync(aos, e->staging, e->state_bytes, hipMemcpyDeviceToHost, e->stream) != hipSuccess)
+  if (hipMemcpyAsync(aos, e->staging, pool_bytes(e), hipMemcpyDeviceToHost, e->stream) != hipSuccess)
     return 1;
   return hipStreamSynchronize(e->stream) == hipSuccess ? 0 : 1;
 }
@@ -335,22 +555,32 @@ int engine_upload_flags(Engine* e, const int* fws_aos) {
   return hipStreamSynchronize(e->stream) == hipSuccess ? 0 : 1;
 }
 
-void engine_bc(Engine* e) { launch_bc(e->stream, e->U, e->bc_cubes, e->bc_masks, e->n_bc, e->ph); }
+void engine_bc(Engine* e) { launch_bc(e->stream, e->U, e->bc_cubes, e->bc_masks, e->n_bc, e->pt()); }
 
 void engine_ghost(Engine* e) {
-  launch_ghost(e->stream, e->U, e->ghost, e->cube_level_off, e->level_off, nullptr,
-               e->n_ghost_cubes, e->ph);
+  launch_ghost(e->stream, e->U, e->ghost, e->cube_level_off, e->level_off, e->n_ghost_cubes,
+               e->pt());
 }
 
-void engine_interface(Engine* e) { launch_halo_edge(e->stream, e->U, e->halo, e->ncube); }
+void engine_interface(Engine* e) {
+  launch_halo_edge(e->stream, e->U, e->halo, e->ncube, e->cube_case);
+}
+
+void engine_halo_refresh(Engine* e) {
+  launch_halo_sparse(e->stream, e->U, e->halo_pairs, e->n_halo_pairs, e->ncube, e->cube_case);
+}
+
+namespace {
+double* er_row(Engine* e, int er_slot) { return e->er_dev + (size_t)er_slot * e->max_cases * 8; }
+}  // namespace
 
 void engine_flux(Engine* e, int RK) {
   if (e->flux_variant == 0)
     launch_flux(e->stream, e->U, e->U1, e->U1q, e->fws, e->cube_size, e->Rku, e->Res, e->ncube,
-                e->ph, e->deltaT, e->deltaTau, e->e);
+                e->pt(), e->deltaT, e->deltaTau, e->e);
   else
     launch_flux_march(e->stream, e->U, e->U1, e->U1q, e->fws, e->cube_size, e->Rku, e->Res,
-                      e->ncube, e->ph, e->deltaT, e->deltaTau, e->e, e->flux_nseg, RK, 0,
+                      e->ncube, e->pt(), e->deltaT, e->deltaTau, e->e, e->flux_nseg, RK, 0,
                       e->U1p1, e->U1p2, e->partials);
 }
 
@@ -365,16 +595,16 @@ void engine_flux_update(Engine* e, int RK, int er_slot) {
     return;
   }
   launch_flux_march(e->stream, e->U, e->U1, e->U1q, e->fws, e->cube_size, e->Rku, e->Res,
-                    e->ncube, e->ph, e->deltaT, e->deltaTau, e->e, 1, RK, 1, e->U1p1, e->U1p2,
+                    e->ncube, e->pt(), e->deltaT, e->deltaTau, e->e, 1, RK, 1, e->U1p1, e->U1p2,
                     e->partials);
   if (RK == 3)
-    launch_reduce(e->stream, e->partials, e->ncube - 1, e->er_dev + 8 * er_slot, e->mpi_ncube,
-                  e->ph);
+    launch_reduce(e->stream, e->partials, 1, e->case_desc, e->n_active, e->phys_dev,
+                  er_row(e, er_slot));
 }
 
 int engine_autotune(Engine* e, int* variant, int* nseg, std::string& err) {
   HIP_TRY(hipSetDevice(e->device));
-  if (e->ncube <= 1) {
+  if (e->n_active == 0) {
     err = "no case loaded";
     return 1;
   }
@@ -386,8 +616,8 @@ int engine_autotune(Engine* e, int* variant, int* nseg, std::string& err) {
   double* saved[3] = {nullptr, nullptr, nullptr};
   double* live[3] = {e->U, e->U1p1, e->U1p2};
   for (int a = 0; a < 3; ++a) {
-    HIP_TRY(hipMalloc((void**)&saved[a], e->state_bytes));
-    HIP_TRY(hipMemcpyAsync(saved[a], live[a], e->state_bytes, hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(hipMalloc((void**)&saved[a], pool_bytes(e)));
+    HIP_TRY(hipMemcpyAsync(saved[a], live[a], pool_bytes(e), hipMemcpyDeviceToDevice, e->stream));
   }
   float best = 1e30f;
   int bv = e->flux_variant, bs = e->flux_nseg;
@@ -411,7 +641,7 @@ int engine_autotune(Engine* e, int* variant, int* nseg, std::string& err) {
     }
   }
   for (int a = 0; a < 3; ++a) {
-    HIP_TRY(hipMemcpyAsync(live[a], saved[a], e->state_bytes, hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(live[a], saved[a], pool_bytes(e), hipMemcpyDeviceToDevice, e->stream));
   }
   HIP_TRY(hipStreamSynchronize(e->stream));
   for (int a = 0; a < 3; ++a) hipFree(saved[a]);
@@ -427,21 +657,23 @@ int engine_autotune(Engine* e, int* variant, int* nseg, std::string& err) {
 
 void engine_update(Engine* e, int RK, int er_slot) {
   launch_update(e->stream, RK, e->U, e->U1p1, e->U1p2, e->Rku, e->Res, e->fws, e->ncube,
-                e->partials);
+                e->cube_case, e->partials);
   if (RK == 3)
-    launch_reduce(e->stream, e->partials, update_blocks(e->ncube), e->er_dev + 8 * er_slot,
-                  e->mpi_ncube, e->ph);
+    launch_reduce(e->stream, e->partials, 16, e->case_desc, e->n_active, e->phys_dev,
+                  er_row(e, er_slot));
 }
 
-void engine_shift(Engine* e) { launch_shift(e->stream, e->U, e->U1, e->U1q, e->ncube); }
+void engine_shift(Engine* e) {
+  launch_shift(e->stream, e->U, e->U1, e->U1q, e->ncube, e->cube_case);
+}
 
 void engine_stage(Engine* e, int RK, int er_slot, bool full_pre_halo) {
   engine_bc(e);
   engine_ghost(e);
-  if (full_pre_halo || !e->halo_pairs)
+  if (full_pre_halo)
     engine_interface(e);
   else
-    launch_halo_sparse(e->stream, e->U, e->halo_pairs, e->n_halo_pairs, e->ncube);
+    engine_halo_refresh(e);
   engine_flux_update(e, RK, er_slot);
   engine_interface(e);
 }
@@ -453,8 +685,10 @@ int ensure_er(Engine* e, int slots, std::string& err) {
   e->er_dev = nullptr;
   e->er_host = nullptr;
   free_graph(e);
-  HIP_TRY(hipMalloc((void**)&e->er_dev, (size_t)slots * 8 * sizeof(double)));
-  HIP_TRY(hipHostMalloc((void**)&e->er_host, (size_t)slots * 8 * sizeof(double)));
+  const size_t n = (size_t)slots * e->max_cases * 8;
+  HIP_TRY(hipMalloc((void**)&e->er_dev, n * sizeof(double)));
+  HIP_TRY(hipMemset(e->er_dev, 0, n * sizeof(double)));
+  HIP_TRY(hipHostMalloc((void**)&e->er_host, n * sizeof(double)));
   e->er_slots = slots;
   return 0;
 }
@@ -472,7 +706,7 @@ void enqueue_step(Engine* e, int n_inner, bool full_first) {
 int engine_run(Engine* e, int n_steps, int n_inner, double* er_hist, bool use_graph,
                std::string& err) {
   HIP_TRY(hipSetDevice(e->device));
-  if (e->ncube <= 1) {
+  if (e->n_active == 0) {
     err = "no case loaded";
     return 1;
   }
@@ -502,12 +736,15 @@ int engine_run(Engine* e, int n_steps, int n_inner, double* er_hist, bool use_gr
       enqueue_step(e, n_inner, false);
     }
     if (er_hist) {
-      HIP_TRY(hipMemcpyAsync(e->er_host, e->er_dev, (size_t)n_inner * 8 * sizeof(double),
+      const size_t row = (size_t)e->max_cases * 8;
+      HIP_TRY(hipMemcpyAsync(e->er_host, e->er_dev, n_inner * row * sizeof(double),
                              hipMemcpyDeviceToHost, e->stream));
       HIP_TRY(hipStreamSynchronize(e->stream));
       for (int it = 0; it < n_inner; ++it)
-        for (int m = 0; m < 7; ++m)
-          er_hist[((size_t)s * n_inner + it) * 7 + m] = e->er_host[it * 8 + 1 + m];
+        for (int c = 0; c < e->max_cases; ++c)
+          for (int m = 0; m < 7; ++m)
+            er_hist[(((size_t)s * n_inner + it) * e->max_cases + c) * 7 + m] =
+                e->er_host[it * row + c * 8 + 1 + m];
     }
   }
   HIP_TRY(hipGetLastError());

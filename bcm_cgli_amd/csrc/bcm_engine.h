@@ -1,6 +1,7 @@
-// Device-resident solver engine: one HIP stream, device mirrors of the state
-// and table arrays of one flow case, and the stage sequence of the reference
-// main loop (main.cpp:1157-1245) as kernel launches.
+// Device-resident solver engine: one HIP stream and a pool of cubes in HBM
+// that holds one or more independent flow cases, each in a contiguous range
+// of pool cubes, and the stage sequence of the reference main loop
+// (main.cpp:1157-1245) as kernel launches over every cube of every case.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -25,19 +26,48 @@ struct GhostSchedule {
 // given in reference order (minus list, then plus list).
 GhostSchedule build_ghost_schedule(const std::vector<GhostEntry>& ordered);
 
+// Device tables of one case in its own cube numbering (cubes 1..n).
+struct CaseTables {
+  int n = 0;            // cubes
+  int mpi_ncube = 0;    // cube count of the norm scaling (MPI_Ncube)
+  DevPhys ph{};
+  std::vector<int> bc_cubes;
+  std::vector<uint8_t> bc_masks;
+  GhostSchedule ghost;
+  std::vector<HaloSrc> halo;       // [n + 1]
+  std::vector<int2> halo_pairs;    // (source, destination) flat cells, cube * 8000 + cell
+};
+int build_case_tables(const Case& cs, CaseTables& tb, std::string& err);
+
+struct CaseSlot {
+  bool active = false;
+  int first = 0;        // pool cube of the case's cube 1
+  CaseTables tb;
+};
+
 struct Engine {
   int device = 0;
   hipStream_t stream = nullptr;
-  int ncube = 0;       // cubes + 1 (slot 0 unused)
-  int mpi_ncube = 0;   // cube count used by the norm scaling
-  DevPhys ph{};
+  int cap = 0;         // pool cubes allocated (slot 0 included)
+  int ncube = 0;       // one past the last pool cube in use; launches cover 1..ncube-1
   double deltaT = 0.0001, deltaTau = 0.0001 / 200., e = 0.1;
 
-  // Device arrays.
+  // Resident cases.
+  int max_cases = 1;
+  std::vector<CaseSlot> cases;
+  std::vector<int> cube_case_host;    // [cap], -1 = free
+  int n_active = 0;
+  DevPhys* phys_dev = nullptr;        // [max_cases]
+  int* cube_case = nullptr;           // [cap]
+  CaseDesc* case_desc = nullptr;      // [n_active], ordered by first cube
+  PhysTab pt() const { return PhysTab{phys_dev, cube_case}; }
+
+  // Device arrays (SoA per cube, cap cubes each).
   double *U = nullptr, *U1 = nullptr, *U1q = nullptr, *U1p1 = nullptr, *U1p2 = nullptr;
   double *Rku = nullptr, *Res = nullptr, *staging = nullptr;
   int8_t* fws = nullptr;
   double* cube_size = nullptr;
+  // Tables of all resident cases in pool numbering.
   int* bc_cubes = nullptr;
   uint8_t* bc_masks = nullptr;
   int n_bc = 0;
@@ -52,14 +82,13 @@ struct Engine {
   // that lie in a neighbour's gather source layers.
   int2* halo_pairs = nullptr;
   int n_halo_pairs = 0;
-  // Set when the inner iterate was replaced from outside: the next pre-flux
-  // exchange must then be the full one.
+  // Set when an inner iterate was replaced from outside (state upload, new
+  // case): the next pre-flux exchange must then be the full one.
   bool halo_full_next = false;
   double* partials = nullptr;
-  double* er_dev = nullptr;     // 8 doubles per inner iteration slot
+  double* er_dev = nullptr;     // [er_slots][max_cases][8]
   int er_slots = 0;
   double* er_host = nullptr;    // pinned mirror of er_dev
-  size_t state_bytes = 0;
 
   // Flux kernel choice: variant 0 = one workgroup per cell plane (k_flux),
   // 1 = marching workgroups over nseg segments of each cube (k_flux_march).
@@ -75,22 +104,38 @@ struct Engine {
   int graph_inner = 0;
 };
 
+// Bytes of device memory per pool cube.
+size_t engine_bytes_per_cube();
+
 int engine_create(int device, Engine** out, std::string& err);
-// Allocates zeroed device arrays for ncube (cubes + 1) cubes.
-int engine_alloc(Engine* e, int ncube, std::string& err);
+// Allocates a pool of cap cubes (slot 0 included) for up to max_cases cases.
+int engine_alloc(Engine* e, int cap, int max_cases, std::string& err);
 int ensure_er(Engine* e, int slots, std::string& err);
 void engine_destroy(Engine* e);
-// Uploads a case built on the host (tables and state).
+// Single-case use: a pool of exactly cs.ncube cubes holding cs in slot 0.
 int engine_load_case(Engine* e, const Case& cs, std::string& err);
-// Uploads tables only (flags, cube sizes, lists) for drop-in use.
+// Adds a case to the pool in the first free range of its size; the slot
+// index it gets is returned through slot.
+int engine_add_case(Engine* e, const Case& cs, int* slot, std::string& err);
+int engine_remove_case(Engine* e, int slot, std::string& err);
+// Largest free contiguous range of pool cubes.
+int engine_free_range(const Engine* e);
+// Registers cubes 1..ncube-1 as one case with no tables (drop-in binding).
+int engine_bind_single(Engine* e, int mpi_ncube, const DevPhys& ph, std::string& err);
+// Whole-pool state transfers, AoS layout of cubes 0..ncube-1.
 int engine_upload_state(Engine* e, int which, const double* aos);
 int engine_download_state(Engine* e, int which, double* aos);
 int engine_upload_flags(Engine* e, const int* fws_aos);
+// One case's state, AoS layout of its cubes 0..n (cube 0 is zeroed on download
+// and ignored on upload).
+int engine_case_state(Engine* e, int slot, int which, double* aos, bool upload,
+                      std::string& err);
 
 // Stage pieces (asynchronous on the engine stream).
 void engine_bc(Engine* e);
 void engine_ghost(Engine* e);
 void engine_interface(Engine* e);
+void engine_halo_refresh(Engine* e);
 void engine_flux(Engine* e, int RK);
 void engine_update(Engine* e, int RK, int er_slot);
 // Flux and RK update of one stage, fused into one kernel when the flux
@@ -99,12 +144,12 @@ void engine_flux_update(Engine* e, int RK, int er_slot);
 bool engine_fused(const Engine* e);
 void engine_shift(Engine* e);
 void engine_stage(Engine* e, int RK, int er_slot, bool full_pre_halo = false);
-// n_steps physical steps of n_inner inner iterations; er_hist (may be null)
-// receives er[1..7] per inner iteration.
+// n_steps physical steps of n_inner inner iterations for every resident case.
+// er_hist (may be null) receives er[1..7] as [step][inner][max_cases][7].
 int engine_run(Engine* e, int n_steps, int n_inner, double* er_hist, bool use_graph,
                std::string& err);
 int engine_sync(Engine* e, std::string& err);
-// Times every flux variant on the resident case and keeps the fastest.
+// Times every flux variant on the resident cases and keeps the fastest.
 // Returns the chosen (variant, nseg) through the pointers.
 int engine_autotune(Engine* e, int* variant, int* nseg, std::string& err);
 

@@ -18,6 +18,20 @@ struct DevPhys {
   double rho0, U0, Uin, V0, W0, P0, T0, mu_L;
 };
 
+// Physics of every cube of a pool that holds several independent cases:
+// phys[cube_case[c]] is the case constants of cube c; cube_case[c] < 0 marks
+// a cube that belongs to no case (kernels skip it).
+struct PhysTab {
+  const DevPhys* phys;
+  const int* cube_case;
+};
+
+// One resident case in the per-case norm reduction: cubes [first, first+n)
+// of the pool, norm scaling cube count mpi_ncube, norm slot and physics index.
+struct CaseDesc {
+  int first, n, mpi_ncube, slot;
+};
+
 // One IB ghost-fill entry in level order (see the ghost schedule in the engine).
 struct GhostEntry {
   int cube;
@@ -30,31 +44,38 @@ struct HaloSrc {
   int src[6];   // -X, +X, -Y, +Y, -Z, +Z
 };
 
+// Launches cover pool cubes 1..ncube-1; "ncube" is one past the last cube in use.
 void launch_bc(hipStream_t s, double* U, const int* cubes, const uint8_t* masks, int n,
-               DevPhys ph);
+               PhysTab pt);
 void launch_ghost(hipStream_t s, double* U, const GhostEntry* e, const int* cube_level_off,
-                  const int* level_off, const int* cubes, int ncubes_with_ghosts, DevPhys ph);
-void launch_halo_edge(hipStream_t s, double* U, const HaloSrc* src, int ncube);
+                  const int* level_off, int ncubes_with_ghosts, PhysTab pt);
+void launch_halo_edge(hipStream_t s, double* U, const HaloSrc* src, int ncube, const int* cube_case);
 // Pre-flux refresh: copy (source, destination) cell pairs (cube*8000 + cell),
 // then the edge passes of every cube.
-void launch_halo_sparse(hipStream_t s, double* U, const int2* pairs, int npairs, int ncube);
+void launch_halo_sparse(hipStream_t s, double* U, const int2* pairs, int npairs, int ncube,
+                        const int* cube_case);
 void launch_flux(hipStream_t s, const double* U, const double* U1, const double* U1q,
                  const int8_t* fws, const double* cube_size, double* Rku, double* Res,
-                 int ncube, DevPhys ph, double deltaT, double deltaTau, double e);
+                 int ncube, PhysTab pt, double deltaT, double deltaTau, double e);
 // Marching variant of the flux kernel: one workgroup per cube segment
 // (nseg in {1, 2, 4, 8, 16} segments of the cube's 16 i-planes).  With
 // fuse != 0 and nseg == 1 it also applies the RK update in place and, at
 // RK 3, writes one row of 8 norm partials per cube (as k_update does per block).
 void launch_flux_march(hipStream_t s, double* U, const double* U1, const double* U1q,
                        const int8_t* fws, const double* cube_size, double* Rku, double* Res,
-                       int ncube, DevPhys ph, double deltaT, double deltaTau, double e, int nseg,
+                       int ncube, PhysTab pt, double deltaT, double deltaTau, double e, int nseg,
                        int RK, int fuse, double* U1p1, double* U1p2, double* partials);
 void launch_update(hipStream_t s, int RK, double* U, double* U1p1, double* U1p2,
                    const double* Rku, const double* Res, const int8_t* fws, int ncube,
-                   double* partials);
-void launch_reduce(hipStream_t s, const double* partials, int nblocks, double* er,
-                   int mpi_ncube, DevPhys ph);
-void launch_shift(hipStream_t s, const double* U, double* U1, double* U1q, int ncube);
+                   const int* cube_case, double* partials);
+// Norms er[0..7] of each case in cases[0..ncases): rows_per_cube partial rows
+// per cube (1 after the fused flux kernel, 16 after k_update); case d writes
+// er[d.slot*8 .. d.slot*8+7].
+void launch_reduce(hipStream_t s, const double* partials, int rows_per_cube,
+                   const CaseDesc* cases, int ncases, const DevPhys* phys, double* er);
+void launch_shift(hipStream_t s, const double* U, double* U1, double* U1q, int ncube,
+                  const int* cube_case);
+// AoS <-> SoA transposes of ncube whole cubes (cube 0 of both arrays included).
 void launch_aos_to_soa(hipStream_t s, const double* aos, double* soa, int ncube);
 void launch_soa_to_aos(hipStream_t s, const double* soa, double* aos, int ncube);
 int update_blocks(int ncube);

@@ -237,9 +237,12 @@ __global__ __launch_bounds__(WG) void k_flux(const double* __restrict__ U,
                                              const int8_t* __restrict__ fws,
                                              const double* __restrict__ cube_size,
                                              double* __restrict__ Rku, double* __restrict__ Res,
-                                             DevPhys ph, double deltaT, double deltaTau,
+                                             PhysTab pt, double deltaT, double deltaTau,
                                              double e) {
   const int c = blockIdx.y + 1;
+  const int cc = pt.cube_case[c];
+  if (cc < 0) return;
+  const DevPhys ph = pt.phys[cc];
   const int i = blockIdx.x + 2;
   const int j = (threadIdx.x >> 4) + 2;
   const int k = (threadIdx.x & 15) + 2;
@@ -454,7 +457,7 @@ constexpr int NFACE = 17 * 16;      // faces of one axis in one i-plane
 __global__ __launch_bounds__(WG, 2) void k_flux_march(
     double* __restrict__ U, const double* __restrict__ U1, const double* __restrict__ U1q,
     const int8_t* __restrict__ fws, const double* __restrict__ cube_size,
-    double* __restrict__ Rku, double* __restrict__ Res, DevPhys ph, double deltaT,
+    double* __restrict__ Rku, double* __restrict__ Res, PhysTab pt, double deltaT,
     double deltaTau, double e, int nseg, int RK, int fuse, double* __restrict__ U1p1,
     double* __restrict__ U1p2, double* __restrict__ partials) {
   __shared__ double pr[4][3 * PR_PLANE];   // u, v, w, T = P/(rho R) of 3 planes
@@ -462,6 +465,9 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
   __shared__ double fz[6][NFACE];          // flux + face pressure average
   const int seg = blockIdx.x % nseg;
   const int c = blockIdx.x / nseg + 1;
+  const int cc = pt.cube_case[c];
+  if (cc < 0) return;
+  const DevPhys ph = pt.phys[cc];
   const int len = 16 / nseg;
   const int i0 = 2 + seg * len, i1 = i0 + len - 1;
   const int t = threadIdx.x;
@@ -804,8 +810,10 @@ __global__ __launch_bounds__(WG) void k_update(int RK, double* __restrict__ U,
                                                const double* __restrict__ Rku,
                                                const double* __restrict__ Res,
                                                const int8_t* __restrict__ fws,
+                                               const int* __restrict__ cube_case,
                                                double* __restrict__ partials) {
   const int c = blockIdx.y + 1;
+  if (cube_case[c] < 0) return;
   const int i = blockIdx.x + 2;
   const int j = (threadIdx.x >> 4) + 2;
   const int k = (threadIdx.x & 15) + 2;
@@ -870,18 +878,28 @@ __global__ __launch_bounds__(WG) void k_update(int RK, double* __restrict__ U,
   }
 }
 
-// Final norm reduction (Flux.cpp:2627-2652 at one rank): er[1..7].
-__global__ __launch_bounds__(WG) void k_reduce(const double* __restrict__ partials, int nblocks,
-                                               double* __restrict__ er, int mpi_ncube,
-                                               DevPhys ph) {
+// Final norm reduction (Flux.cpp:2627-2652 at one rank): er[1..7] of one
+// case per block.  The partial rows of a case are summed in an order fixed
+// relative to the case's first row, so a case gives the same bits alone and
+// next to other cases.
+__global__ __launch_bounds__(WG) void k_reduce(const double* __restrict__ partials,
+                                               int rows_per_cube,
+                                               const CaseDesc* __restrict__ cases,
+                                               const DevPhys* __restrict__ phys,
+                                               double* __restrict__ er_all) {
+  const CaseDesc cd = cases[blockIdx.x];
+  const DevPhys ph = phys[cd.slot];
+  const double* rows = partials + (size_t)(cd.first - 1) * rows_per_cube * 8;
+  const int nrows = cd.n * rows_per_cube;
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int bi = threadIdx.x; bi < nblocks; bi += WG)
+  for (int bi = threadIdx.x; bi < nrows; bi += WG)
 #pragma unroll
-    for (int m = 0; m < 7; ++m) acc[m] += partials[(size_t)bi * 8 + m];
+    for (int m = 0; m < 7; ++m) acc[m] += rows[(size_t)bi * 8 + m];
   __shared__ double sh[(WG / 64) * 7];
   block_sum<7>(acc, sh);
   if (threadIdx.x == 0) {
-    const double DN = 1. / (mpi_ncube * 16 * 16 * 16);
+    double* er = er_all + (size_t)cd.slot * 8;
+    const double DN = 1. / (cd.mpi_ncube * 16 * 16 * 16);
     er[0] = 0;
     for (int m = 0; m < 5; ++m) er[m + 1] = sqrt(acc[m]) * DN;
     er[6] = acc[5] / (0.5 * ph.rho0 * ph.U0 * ph.U0 * 2.602);
@@ -895,8 +913,9 @@ __global__ __launch_bounds__(WG) void k_reduce(const double* __restrict__ partia
 // (BCM_Z_boundary_condition.cpp:42-92).  mask bits: 1 x-min, 2 x-max, 4 y-min,
 // 8 y-max, 16 z-min, 32 z-max.
 __global__ __launch_bounds__(WG) void k_bc(double* __restrict__ U, const int* __restrict__ cubes,
-                                           const uint8_t* __restrict__ masks, DevPhys ph) {
+                                           const uint8_t* __restrict__ masks, PhysTab pt) {
   const int c = cubes[blockIdx.x];
+  const DevPhys ph = pt.phys[pt.cube_case[c]];
   const int m = masks[blockIdx.x];
   const int t = threadIdx.x;
   {
@@ -971,14 +990,15 @@ __global__ __launch_bounds__(WG) void k_bc(double* __restrict__ U, const int* __
 __global__ __launch_bounds__(WG) void k_ghost(double* __restrict__ U,
                                               const GhostEntry* __restrict__ ent,
                                               const int* __restrict__ cube_level_off,
-                                              const int* __restrict__ level_off, DevPhys ph) {
+                                              const int* __restrict__ level_off, PhysTab pt) {
   const int l0 = cube_level_off[blockIdx.x], l1 = cube_level_off[blockIdx.x + 1];
-  const double rho0 = ph.rho0, P0 = ph.P0;
   for (int lv = l0; lv < l1; ++lv) {
     const int e0 = level_off[lv], e1 = level_off[lv + 1];
     for (int n = e0 + threadIdx.x; n < e1; n += WG) {
       const GhostEntry& E = ent[n];
       const int c = E.cube;
+      const DevPhys& ph = pt.phys[pt.cube_case[c]];
+      const double rho0 = ph.rho0, P0 = ph.P0;
       double rc[8], uc[8], vc[8], wc[8], pc[8];
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
@@ -1121,8 +1141,10 @@ __device__ __forceinline__ void edge_fill(double* __restrict__ U, int c) {
 // (:5640-5732).  Sources are interior layers, which nothing in this kernel
 // writes, so cubes proceed independently.
 __global__ __launch_bounds__(WG) void k_halo_edge(double* __restrict__ U,
-                                                  const HaloSrc* __restrict__ srcs) {
+                                                  const HaloSrc* __restrict__ srcs,
+                                                  const int* __restrict__ cube_case) {
   const int c = blockIdx.x + 1;
+  if (cube_case[c] < 0) return;
   const HaloSrc hs = srcs[c];
   const int t = threadIdx.x;
   const int p = (t >> 4) + 2, r = (t & 15) + 2;
@@ -1165,15 +1187,19 @@ __global__ __launch_bounds__(WG) void k_halo_sparse(double* __restrict__ U,
   for (int v = 0; v < 5; ++v) U[dd + (size_t)v * PLANE] = U[so + (size_t)v * PLANE];
 }
 
-__global__ __launch_bounds__(WG) void k_edge(double* __restrict__ U) {
+__global__ __launch_bounds__(WG) void k_edge(double* __restrict__ U,
+                                             const int* __restrict__ cube_case) {
+  if (cube_case[blockIdx.x + 1] < 0) return;
   edge_fill(U, blockIdx.x + 1);
 }
 
 // Physical-time shift of interior cells (main.cpp:1292-1315).
 __global__ __launch_bounds__(WG) void k_shift(const double* __restrict__ U,
                                               double* __restrict__ U1,
-                                              double* __restrict__ U1q) {
+                                              double* __restrict__ U1q,
+                                              const int* __restrict__ cube_case) {
   const int c = blockIdx.y + 1;
+  if (cube_case[c] < 0) return;
   const int i = blockIdx.x + 2;
   const int j = (threadIdx.x >> 4) + 2;
   const int k = (threadIdx.x & 15) + 2;
@@ -1210,62 +1236,68 @@ __global__ __launch_bounds__(WG) void k_soa_to_aos(const double* __restrict__ so
 int update_blocks(int ncube) { return 16 * (ncube - 1); }
 
 void launch_bc(hipStream_t s, double* U, const int* cubes, const uint8_t* masks, int n,
-               DevPhys ph) {
-  if (n > 0) hipLaunchKernelGGL(k_bc, dim3(n), dim3(WG), 0, s, U, cubes, masks, ph);
+               PhysTab pt) {
+  if (n > 0) hipLaunchKernelGGL(k_bc, dim3(n), dim3(WG), 0, s, U, cubes, masks, pt);
 }
 
 void launch_ghost(hipStream_t s, double* U, const GhostEntry* e, const int* cube_level_off,
-                  const int* level_off, const int* cubes, int ncubes_with_ghosts, DevPhys ph) {
-  (void)cubes;
+                  const int* level_off, int ncubes_with_ghosts, PhysTab pt) {
   if (ncubes_with_ghosts > 0)
     hipLaunchKernelGGL(k_ghost, dim3(ncubes_with_ghosts), dim3(WG), 0, s, U, e, cube_level_off,
-                       level_off, ph);
+                       level_off, pt);
 }
 
-void launch_halo_edge(hipStream_t s, double* U, const HaloSrc* src, int ncube) {
-  if (ncube > 1) hipLaunchKernelGGL(k_halo_edge, dim3(ncube - 1), dim3(WG), 0, s, U, src);
+void launch_halo_edge(hipStream_t s, double* U, const HaloSrc* src, int ncube,
+                      const int* cube_case) {
+  if (ncube > 1)
+    hipLaunchKernelGGL(k_halo_edge, dim3(ncube - 1), dim3(WG), 0, s, U, src, cube_case);
 }
 
-void launch_halo_sparse(hipStream_t s, double* U, const int2* pairs, int npairs, int ncube) {
+void launch_halo_sparse(hipStream_t s, double* U, const int2* pairs, int npairs, int ncube,
+                        const int* cube_case) {
   if (npairs > 0)
     hipLaunchKernelGGL(k_halo_sparse, dim3((npairs + WG - 1) / WG), dim3(WG), 0, s, U, pairs,
                        npairs);
-  if (ncube > 1) hipLaunchKernelGGL(k_edge, dim3(ncube - 1), dim3(WG), 0, s, U);
+  if (ncube > 1) hipLaunchKernelGGL(k_edge, dim3(ncube - 1), dim3(WG), 0, s, U, cube_case);
 }
 
 void launch_flux(hipStream_t s, const double* U, const double* U1, const double* U1q,
                  const int8_t* fws, const double* cube_size, double* Rku, double* Res,
-                 int ncube, DevPhys ph, double deltaT, double deltaTau, double e) {
+                 int ncube, PhysTab pt, double deltaT, double deltaTau, double e) {
   if (ncube > 1)
     hipLaunchKernelGGL(k_flux, dim3(16, ncube - 1), dim3(WG), 0, s, U, U1, U1q, fws, cube_size,
-                       Rku, Res, ph, deltaT, deltaTau, e);
+                       Rku, Res, pt, deltaT, deltaTau, e);
 }
 
 void launch_flux_march(hipStream_t s, double* U, const double* U1, const double* U1q,
                        const int8_t* fws, const double* cube_size, double* Rku, double* Res,
-                       int ncube, DevPhys ph, double deltaT, double deltaTau, double e, int nseg,
+                       int ncube, PhysTab pt, double deltaT, double deltaTau, double e, int nseg,
                        int RK, int fuse, double* U1p1, double* U1p2, double* partials) {
   if (ncube > 1)
     hipLaunchKernelGGL(k_flux_march, dim3((ncube - 1) * nseg), dim3(WG), 0, s, U, U1, U1q, fws,
-                       cube_size, Rku, Res, ph, deltaT, deltaTau, e, nseg, RK, fuse && nseg == 1,
+                       cube_size, Rku, Res, pt, deltaT, deltaTau, e, nseg, RK, fuse && nseg == 1,
                        U1p1, U1p2, partials);
 }
 
 void launch_update(hipStream_t s, int RK, double* U, double* U1p1, double* U1p2,
                    const double* Rku, const double* Res, const int8_t* fws, int ncube,
-                   double* partials) {
+                   const int* cube_case, double* partials) {
   if (ncube > 1)
     hipLaunchKernelGGL(k_update, dim3(16, ncube - 1), dim3(WG), 0, s, RK, U, U1p1, U1p2, Rku,
-                       Res, fws, partials);
+                       Res, fws, cube_case, partials);
 }
 
-void launch_reduce(hipStream_t s, const double* partials, int nblocks, double* er,
-                   int mpi_ncube, DevPhys ph) {
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(WG), 0, s, partials, nblocks, er, mpi_ncube, ph);
+void launch_reduce(hipStream_t s, const double* partials, int rows_per_cube,
+                   const CaseDesc* cases, int ncases, const DevPhys* phys, double* er) {
+  if (ncases > 0)
+    hipLaunchKernelGGL(k_reduce, dim3(ncases), dim3(WG), 0, s, partials, rows_per_cube, cases,
+                       phys, er);
 }
 
-void launch_shift(hipStream_t s, const double* U, double* U1, double* U1q, int ncube) {
-  if (ncube > 1) hipLaunchKernelGGL(k_shift, dim3(16, ncube - 1), dim3(WG), 0, s, U, U1, U1q);
+void launch_shift(hipStream_t s, const double* U, double* U1, double* U1q, int ncube,
+                  const int* cube_case) {
+  if (ncube > 1)
+    hipLaunchKernelGGL(k_shift, dim3(16, ncube - 1), dim3(WG), 0, s, U, U1, U1q, cube_case);
 }
 
 void launch_aos_to_soa(hipStream_t s, const double* aos, double* soa, int ncube) {

@@ -152,9 +152,11 @@ int bcm_engine_load_case(bcm_engine* e, const bcm_case* c);
 /* Physical and pseudo time steps and the preconditioning floor e
  * (main.cpp:57-59 defaults 1e-4, 1e-4/200, 0.1). */
 int bcm_engine_set_time(bcm_engine* e, double deltaT, double deltaTau, double e_floor);
-/* n_steps physical steps of n_inner inner iterations (3 RK stages each),
- * each step closed by the physical-time shift.  er_hist, if not null,
- * receives er[1..7] after every inner iteration (7 doubles each). */
+/* n_steps physical steps of n_inner inner iterations (3 RK stages each) of
+ * every resident case, each step closed by the physical-time shift.
+ * er_hist, if not null, receives er[1..7] of every case slot after every
+ * inner iteration: [n_steps][n_inner][max_cases][7] doubles (max_cases is 1
+ * after bcm_engine_load_case). */
 int bcm_engine_run(bcm_engine* e, int n_steps, int n_inner, double* er_hist, int use_graph);
 /* One piece of a stage: 0 domain BCs, 1 IB ghost fill, 2 halo + edges,
  * 3 flux, 4 RK update (with norms at RK 3), 5 time shift, 6 whole stage,
@@ -183,6 +185,31 @@ int bcm_engine_set_flux(bcm_engine* e, int variant, int nseg, int fuse);
 int bcm_engine_autotune(bcm_engine* e, int* out3);
 /* Ghost schedule statistics: entries, cubes with entries, maximum levels. */
 int bcm_engine_ghost_stats(bcm_engine* e, int* stats3);
+
+/* ------------------------------------------------------------------------ */
+/* 4. Case pool: several independent cases resident on one device.           */
+/* Not a reference interface: the reference runs one case per process.  The
+ * cases of a pool run together, every kernel launch covering all their
+ * cubes; each case keeps its own physics, tables and norms, and its results
+ * are bitwise the ones it gives alone. */
+
+/* Device memory per pool cube, bytes. */
+long long bcm_bytes_per_cube(void);
+/* Replaces whatever the engine holds by an empty pool of cap_cubes cubes
+ * with max_cases case slots. */
+int bcm_engine_alloc_pool(bcm_engine* e, int cap_cubes, int max_cases);
+/* Adds a case in the first free range of its cube count; *slot receives its
+ * slot index.  Fails when no slot or no range is free. */
+int bcm_engine_add_case(bcm_engine* e, const bcm_case* c, int* slot);
+int bcm_engine_remove_case(bcm_engine* e, int slot);
+/* One case's state array in the reference layout, (ncube+1) cubes of
+ * 20^3 x 5 doubles (cube 0 zeroed on download, ignored on upload). */
+int bcm_engine_case_state(bcm_engine* e, int slot, int which, double* aos, int upload);
+/* er[0..7] of one case after inner iteration `inner` of the last step. */
+int bcm_engine_case_er(bcm_engine* e, int slot, int inner, double* er8);
+/* info6: pool cubes, cubes spanned, active cases, case slots, largest free
+ * range, cubes in use. */
+int bcm_engine_pool_info(bcm_engine* e, int* info6);
 
 #ifdef __cplusplus
 }
