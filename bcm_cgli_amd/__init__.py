@@ -76,8 +76,9 @@ def load_library(path=LIB_PATH):
     lib.bcm_engine_set_flux.argtypes = [c_void_p, c_int, c_int, c_int]
     lib.bcm_engine_autotune.argtypes = [c_void_p, c_void_p]
     lib.bcm_bytes_per_cube.restype = ctypes.c_longlong
-    lib.bcm_bytes_per_cube.argtypes = []
-    lib.bcm_engine_alloc_pool.argtypes = [c_void_p, c_int, c_int]
+    lib.bcm_bytes_per_cube.argtypes = [c_int]
+    lib.bcm_engine_alloc_pool.argtypes = [c_void_p, c_int, c_int, c_int]
+    lib.bcm_engine_precision.argtypes = [c_void_p]
     lib.bcm_engine_add_case.argtypes = [c_void_p, c_void_p, P(c_int)]
     lib.bcm_engine_remove_case.argtypes = [c_void_p, c_int]
     lib.bcm_engine_case_state.argtypes = [c_void_p, c_int, c_int, c_void_p, c_int]
@@ -160,9 +161,12 @@ class Case:
             pass
 
 
-def bytes_per_cube():
+PRECISIONS = {"fp64": 0, "mixed": 1}
+
+
+def bytes_per_cube(precision="fp64"):
     """Device memory one pool cube takes."""
-    return int(load_library().bcm_bytes_per_cube())
+    return int(load_library().bcm_bytes_per_cube(PRECISIONS[precision]))
 
 
 class Engine:
@@ -184,8 +188,9 @@ class Engine:
         self.max_cases = 1
         self.slots = {0: case}
 
-    def alloc_pool(self, cap_cubes, max_cases):
-        _check(_lib.bcm_engine_alloc_pool(self._h, int(cap_cubes), int(max_cases)))
+    def alloc_pool(self, cap_cubes, max_cases, precision="fp64"):
+        _check(_lib.bcm_engine_alloc_pool(self._h, int(cap_cubes), int(max_cases),
+                                          PRECISIONS[precision]))
         self.case = None
         self.max_cases = int(max_cases)
         self.slots = {}

@@ -34,7 +34,7 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-SOURCES = ["bcm_kernels.hip", "bcm_engine.cpp", "bcm_capi.cpp", "bcm_setup.cpp"]
+SOURCES = ["bcm_kernels.hip", "bcm_kernels_mixed.hip", "bcm_engine.cpp", "bcm_capi.cpp", "bcm_setup.cpp"]
 
 
 def _hipcc():

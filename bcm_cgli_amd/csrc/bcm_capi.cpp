@@ -306,7 +306,7 @@ int bcm_engine_load_case(bcm_engine* h, const bcm_case* c) {
   return 0;
 }
 
-int bcm_engine_alloc_pool(bcm_engine* h, int cap_cubes, int max_cases) {
+int bcm_engine_alloc_pool(bcm_engine* h, int cap_cubes, int max_cases, int precision) {
   if (!h) return fail("null engine");
   if (g_current == h) g_current = nullptr;
   if (h->di) {
@@ -314,7 +314,7 @@ int bcm_engine_alloc_pool(bcm_engine* h, int cap_cubes, int max_cases) {
     h->di.reset();
   }
   std::string err;
-  if (engine_alloc(h->e, cap_cubes + 1, max_cases, err)) return fail(err);
+  if (engine_alloc(h->e, cap_cubes + 1, max_cases, err, precision)) return fail(err);
   h->e->flux_tuned = false;
   return 0;
 }
@@ -369,7 +369,11 @@ int bcm_engine_pool_info(bcm_engine* h, int* info6) {
   return 0;
 }
 
-long long bcm_bytes_per_cube(void) { return (long long)engine_bytes_per_cube(); }
+long long bcm_bytes_per_cube(int precision) {
+  return (long long)engine_bytes_per_cube(precision == 1 ? 1 : 0);
+}
+
+int bcm_engine_precision(bcm_engine* h) { return h ? h->e->prec : -1; }
 
 int bcm_engine_set_time(bcm_engine* h, double deltaT, double deltaTau, double e_floor) {
   if (!h) return fail("null engine");
@@ -395,6 +399,8 @@ int bcm_engine_part(bcm_engine* h, int part, int RK) {
   Engine* e = h->e;
   if (e->n_active == 0) return fail("no case loaded");
   if (RK < 1 || RK > 3) return fail("RK must be 1, 2 or 3");
+  if (e->prec == 1 && (part == 3 || part == 4))
+    return fail("the mixed path fuses flux and update (use part 7)");
   switch (part) {
     case 0: engine_bc(e); break;
     case 1: engine_ghost(e); break;
@@ -446,6 +452,8 @@ int bcm_engine_time_part(bcm_engine* h, int part, int RK, int reps, double* ms) 
   Engine* e = h->e;
   if (e->n_active == 0) return fail("no case loaded");
   if (reps < 1) return fail("reps must be positive");
+  if (e->prec == 1 && (part == 3 || part == 4))
+    return fail("the mixed path fuses flux and update (use part 7)");
   hipEventRecord(e->ev0, e->stream);
   for (int r = 0; r < reps; ++r) {
     switch (part) {
@@ -473,6 +481,7 @@ int bcm_engine_time_part(bcm_engine* h, int part, int RK, int reps, double* ms) 
 
 int bcm_engine_set_flux(bcm_engine* h, int variant, int nseg, int fuse) {
   if (!h) return fail("null engine");
+  if (h->e->prec == 1) return fail("the mixed path has a single flux kernel");
   h->e->flux_fuse = fuse != 0;
   if (variant != 0 && variant != 1) return fail("flux variant must be 0 or 1");
   if (nseg != 1 && nseg != 2 && nseg != 4 && nseg != 8 && nseg != 16)

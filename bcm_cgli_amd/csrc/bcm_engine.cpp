@@ -79,6 +79,8 @@ void free_case(Engine* e) {
                   e->phys_dev, e->cube_case};
   for (void* p : ptrs)
     if (p) hipFree(p);
+  if (e->Uf) hipFree(e->Uf);
+  e->Uf = nullptr;
   if (e->er_host) hipHostFree(e->er_host);
   e->U = e->U1 = e->U1q = e->U1p1 = e->U1p2 = e->Rku = e->Res = e->staging = nullptr;
   e->fws = nullptr;
@@ -148,9 +150,10 @@ GhostSchedule build_ghost_schedule(const std::vector<GhostEntry>& ordered) {
   return gs;
 }
 
-size_t engine_bytes_per_cube() {
-  return (size_t)8 * CUBE_STRIDE * sizeof(double) + PLANE + sizeof(double) + sizeof(int) +
-         (size_t)16 * 8 * sizeof(double);
+size_t engine_bytes_per_cube(int prec) {
+  const size_t state = prec == 1 ? (size_t)CUBE_STRIDE * (sizeof(float) + 3 * sizeof(double))
+                                 : (size_t)8 * CUBE_STRIDE * sizeof(double);
+  return state + PLANE + sizeof(double) + sizeof(int) + (size_t)16 * 8 * sizeof(double);
 }
 
 int engine_create(int device, Engine** out, std::string& err) {
@@ -185,7 +188,7 @@ void engine_destroy(Engine* e) {
   delete e;
 }
 
-int engine_alloc(Engine* e, int cap, int max_cases, std::string& err) {
+int engine_alloc(Engine* e, int cap, int max_cases, std::string& err, int prec) {
   HIP_TRY(hipSetDevice(e->device));
   free_case(e);
   if (cap < 2) {
@@ -196,6 +199,11 @@ int engine_alloc(Engine* e, int cap, int max_cases, std::string& err) {
     err = "a pool needs at least one case slot";
     return 1;
   }
+  if (prec != 0 && prec != 1) {
+    err = "precision must be 0 (FP64) or 1 (mixed)";
+    return 1;
+  }
+  e->prec = prec;
   e->cap = cap;
   e->ncube = 1;
   e->max_cases = max_cases;
@@ -204,9 +212,18 @@ int engine_alloc(Engine* e, int cap, int max_cases, std::string& err) {
   e->n_active = 0;
   const size_t ncells = (size_t)cap * PLANE;
   const size_t bytes = ncells * 5 * sizeof(double);
-  for (double** p : {&e->U, &e->U1, &e->U1q, &e->U1p1, &e->U1p2, &e->Rku, &e->Res, &e->staging}) {
-    if (dev_alloc(p, ncells * 5, err)) return 1;
-    HIP_TRY(hipMemsetAsync(*p, 0, bytes, e->stream));
+  if (prec == 1) {
+    if (dev_alloc(&e->Uf, ncells * 5, err)) return 1;
+    HIP_TRY(hipMemsetAsync(e->Uf, 0, ncells * 5 * sizeof(float), e->stream));
+    for (double** p : {&e->U1, &e->U1q, &e->staging}) {
+      if (dev_alloc(p, ncells * 5, err)) return 1;
+      HIP_TRY(hipMemsetAsync(*p, 0, bytes, e->stream));
+    }
+  } else {
+    for (double** p : {&e->U, &e->U1, &e->U1q, &e->U1p1, &e->U1p2, &e->Rku, &e->Res, &e->staging}) {
+      if (dev_alloc(p, ncells * 5, err)) return 1;
+      HIP_TRY(hipMemsetAsync(*p, 0, bytes, e->stream));
+    }
   }
   if (dev_alloc(&e->fws, ncells, err)) return 1;
   HIP_TRY(hipMemsetAsync(e->fws, 0, ncells, e->stream));
@@ -448,8 +465,9 @@ int engine_add_case(Engine* e, const Case& cs, int* slot_out, std::string& err) 
   HIP_TRY(hipMemcpy(e->cube_size + first, cs.cube_size.data() + 1, n * sizeof(double),
                     hipMemcpyHostToDevice));
   for (double* p : {e->U1p1, e->U1p2, e->Rku, e->Res})
-    HIP_TRY(hipMemsetAsync(p + (size_t)first * CUBE_STRIDE, 0,
-                           (size_t)n * CUBE_STRIDE * sizeof(double), e->stream));
+    if (p)
+      HIP_TRY(hipMemsetAsync(p + (size_t)first * CUBE_STRIDE, 0,
+                             (size_t)n * CUBE_STRIDE * sizeof(double), e->stream));
   for (int c = first; c < first + n; ++c) e->cube_case_host[c] = slot;
   if (rebuild_tables(e, err)) return 1;
   const std::vector<double>* st[3] = {&cs.U1_, &cs.U1, &cs.U1q};
@@ -500,22 +518,30 @@ int engine_case_state(Engine* e, int slot, int which, double* aos, bool upload,
     err = "no such case slot";
     return 1;
   }
-  double* d = state_ptr(e, which);
-  if (!d) {
-    err = "unknown state array";
+  const bool f32 = e->prec == 1 && which == 0;
+  double* d = f32 ? nullptr : state_ptr(e, which);
+  if (!d && !f32) {
+    err = e->prec == 1 ? "the mixed path keeps only U1_, U1 and U1q" : "unknown state array";
     return 1;
   }
   const CaseSlot& S = e->cases[slot];
   const size_t cube_bytes = (size_t)CUBE_STRIDE * sizeof(double);
   const size_t n = S.tb.n;
-  double* dst = d + (size_t)S.first * CUBE_STRIDE;
+  double* dst = f32 ? nullptr : d + (size_t)S.first * CUBE_STRIDE;
+  float* dstf = f32 ? e->Uf + (size_t)S.first * CUBE_STRIDE : nullptr;
   if (upload) {
     if (which == 0) e->halo_full_next = true;
     HIP_TRY(hipMemcpyAsync(e->staging, aos + CUBE_STRIDE, n * cube_bytes, hipMemcpyHostToDevice,
                            e->stream));
-    launch_aos_to_soa(e->stream, e->staging, dst, (int)n);
+    if (f32)
+      launch_aos_to_soa_f(e->stream, e->staging, dstf, (int)n);
+    else
+      launch_aos_to_soa(e->stream, e->staging, dst, (int)n);
   } else {
-    launch_soa_to_aos(e->stream, dst, e->staging, (int)n);
+    if (f32)
+      launch_soa_to_aos_f(e->stream, dstf, e->staging, (int)n);
+    else
+      launch_soa_to_aos(e->stream, dst, e->staging, (int)n);
     HIP_TRY(hipMemcpyAsync(aos + CUBE_STRIDE, e->staging, n * cube_bytes, hipMemcpyDeviceToHost,
                            e->stream));
     memset(aos, 0, cube_bytes);
@@ -529,6 +555,7 @@ size_t pool_bytes(const Engine* e) { return (size_t)e->ncube * CUBE_STRIDE * siz
 }  // namespace
 
 int engine_upload_state(Engine* e, int which, const double* aos) {
+  if (e->prec != 0) return 1;
   double* d = state_ptr(e, which);
   if (!d) return 1;
   if (which == 0) e->halo_full_next = true;
@@ -539,6 +566,7 @@ int engine_upload_state(Engine* e, int which, const double* aos) {
 }
 
 int engine_download_state(Engine* e, int which, double* aos) {
+  if (e->prec != 0) return 1;
   double* d = state_ptr(e, which);
   if (!d) return 1;
   launch_soa_to_aos(e->stream, d, e->staging, e->ncube);
@@ -555,19 +583,34 @@ int engine_upload_flags(Engine* e, const int* fws_aos) {
   return hipStreamSynchronize(e->stream) == hipSuccess ? 0 : 1;
 }
 
-void engine_bc(Engine* e) { launch_bc(e->stream, e->U, e->bc_cubes, e->bc_masks, e->n_bc, e->pt()); }
+void engine_bc(Engine* e) {
+  if (e->prec == 1)
+    launch_bc(e->stream, e->Uf, e->bc_cubes, e->bc_masks, e->n_bc, e->pt());
+  else
+    launch_bc(e->stream, e->U, e->bc_cubes, e->bc_masks, e->n_bc, e->pt());
+}
 
 void engine_ghost(Engine* e) {
-  launch_ghost(e->stream, e->U, e->ghost, e->cube_level_off, e->level_off, e->n_ghost_cubes,
-               e->pt());
+  if (e->prec == 1)
+    launch_ghost(e->stream, e->Uf, e->ghost, e->cube_level_off, e->level_off, e->n_ghost_cubes,
+                 e->pt());
+  else
+    launch_ghost(e->stream, e->U, e->ghost, e->cube_level_off, e->level_off, e->n_ghost_cubes,
+                 e->pt());
 }
 
 void engine_interface(Engine* e) {
-  launch_halo_edge(e->stream, e->U, e->halo, e->ncube, e->cube_case);
+  if (e->prec == 1)
+    launch_halo_edge(e->stream, e->Uf, e->halo, e->ncube, e->cube_case);
+  else
+    launch_halo_edge(e->stream, e->U, e->halo, e->ncube, e->cube_case);
 }
 
 void engine_halo_refresh(Engine* e) {
-  launch_halo_sparse(e->stream, e->U, e->halo_pairs, e->n_halo_pairs, e->ncube, e->cube_case);
+  if (e->prec == 1)
+    launch_halo_sparse(e->stream, e->Uf, e->halo_pairs, e->n_halo_pairs, e->ncube, e->cube_case);
+  else
+    launch_halo_sparse(e->stream, e->U, e->halo_pairs, e->n_halo_pairs, e->ncube, e->cube_case);
 }
 
 namespace {
@@ -585,10 +628,18 @@ void engine_flux(Engine* e, int RK) {
 }
 
 bool engine_fused(const Engine* e) {
-  return e->flux_variant == 1 && e->flux_nseg == 1 && e->flux_fuse;
+  return e->prec == 1 || (e->flux_variant == 1 && e->flux_nseg == 1 && e->flux_fuse);
 }
 
 void engine_flux_update(Engine* e, int RK, int er_slot) {
+  if (e->prec == 1) {
+    launch_flux_mixed(e->stream, e->Uf, e->U1, e->U1q, e->fws, e->cube_size, e->ncube, e->pt(),
+                      e->deltaT, e->deltaTau, e->e, RK, e->partials);
+    if (RK == 3)
+      launch_reduce(e->stream, e->partials, 1, e->case_desc, e->n_active, e->phys_dev,
+                    er_row(e, er_slot));
+    return;
+  }
   if (!engine_fused(e)) {
     engine_flux(e, RK);
     engine_update(e, RK, er_slot);
@@ -607,6 +658,13 @@ int engine_autotune(Engine* e, int* variant, int* nseg, std::string& err) {
   if (e->n_active == 0) {
     err = "no case loaded";
     return 1;
+  }
+  if (e->prec == 1) {
+    // The mixed path has one kernel shape (one workgroup per cube, update fused).
+    e->flux_tuned = true;
+    if (variant) *variant = 2;
+    if (nseg) *nseg = 1;
+    return 0;
   }
   // Candidates (variant, nseg, fuse): the plane kernel, the marching kernel
   // with 1..8 segments, and the marching kernel with the RK update fused.
@@ -664,7 +722,10 @@ void engine_update(Engine* e, int RK, int er_slot) {
 }
 
 void engine_shift(Engine* e) {
-  launch_shift(e->stream, e->U, e->U1, e->U1q, e->ncube, e->cube_case);
+  if (e->prec == 1)
+    launch_shift_mixed(e->stream, e->Uf, e->U1, e->U1q, e->ncube, e->cube_case);
+  else
+    launch_shift(e->stream, e->U, e->U1, e->U1q, e->ncube, e->cube_case);
 }
 
 void engine_stage(Engine* e, int RK, int er_slot, bool full_pre_halo) {

@@ -62,7 +62,13 @@ struct Engine {
   CaseDesc* case_desc = nullptr;      // [n_active], ordered by first cube
   PhysTab pt() const { return PhysTab{phys_dev, cube_case}; }
 
-  // Device arrays (SoA per cube, cap cubes each).
+  // Arithmetic of the pool: 0 plain FP64 (bitwise the reference), 1 mixed
+  // (FP32 inner iterate and stage arithmetic, FP64 time levels, source and norms).
+  int prec = 0;
+  float* Uf = nullptr;          // inner iterate of the mixed path
+
+  // Device arrays (SoA per cube, cap cubes each).  The mixed path keeps U1,
+  // U1q, staging, flags and sizes; U, U1p1, U1p2, Rku and Res stay null.
   double *U = nullptr, *U1 = nullptr, *U1q = nullptr, *U1p1 = nullptr, *U1p2 = nullptr;
   double *Rku = nullptr, *Res = nullptr, *staging = nullptr;
   int8_t* fws = nullptr;
@@ -104,12 +110,13 @@ struct Engine {
   int graph_inner = 0;
 };
 
-// Bytes of device memory per pool cube.
-size_t engine_bytes_per_cube();
+// Bytes of device memory per pool cube for a precision (0 FP64, 1 mixed).
+size_t engine_bytes_per_cube(int prec);
 
 int engine_create(int device, Engine** out, std::string& err);
-// Allocates a pool of cap cubes (slot 0 included) for up to max_cases cases.
-int engine_alloc(Engine* e, int cap, int max_cases, std::string& err);
+// Allocates a pool of cap cubes (slot 0 included) for up to max_cases cases
+// in precision prec (0 FP64, 1 mixed).
+int engine_alloc(Engine* e, int cap, int max_cases, std::string& err, int prec = 0);
 int ensure_er(Engine* e, int slots, std::string& err);
 void engine_destroy(Engine* e);
 // Single-case use: a pool of exactly cs.ncube cubes holding cs in slot 0.

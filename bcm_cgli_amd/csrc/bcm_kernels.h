@@ -45,14 +45,23 @@ struct HaloSrc {
 };
 
 // Launches cover pool cubes 1..ncube-1; "ncube" is one past the last cube in use.
+// The BC, IB fill and halo kernels exist for an FP64 inner iterate and for the
+// FP32 one of the mixed-precision path (arithmetic in FP64 either way).
 void launch_bc(hipStream_t s, double* U, const int* cubes, const uint8_t* masks, int n,
+               PhysTab pt);
+void launch_bc(hipStream_t s, float* U, const int* cubes, const uint8_t* masks, int n,
                PhysTab pt);
 void launch_ghost(hipStream_t s, double* U, const GhostEntry* e, const int* cube_level_off,
                   const int* level_off, int ncubes_with_ghosts, PhysTab pt);
+void launch_ghost(hipStream_t s, float* U, const GhostEntry* e, const int* cube_level_off,
+                  const int* level_off, int ncubes_with_ghosts, PhysTab pt);
 void launch_halo_edge(hipStream_t s, double* U, const HaloSrc* src, int ncube, const int* cube_case);
+void launch_halo_edge(hipStream_t s, float* U, const HaloSrc* src, int ncube, const int* cube_case);
 // Pre-flux refresh: copy (source, destination) cell pairs (cube*8000 + cell),
 // then the edge passes of every cube.
 void launch_halo_sparse(hipStream_t s, double* U, const int2* pairs, int npairs, int ncube,
+                        const int* cube_case);
+void launch_halo_sparse(hipStream_t s, float* U, const int2* pairs, int npairs, int ncube,
                         const int* cube_case);
 void launch_flux(hipStream_t s, const double* U, const double* U1, const double* U1q,
                  const int8_t* fws, const double* cube_size, double* Rku, double* Res,
@@ -79,5 +88,19 @@ void launch_shift(hipStream_t s, const double* U, double* U1, double* U1q, int n
 void launch_aos_to_soa(hipStream_t s, const double* aos, double* soa, int ncube);
 void launch_soa_to_aos(hipStream_t s, const double* soa, double* aos, int ncube);
 int update_blocks(int ncube);
+
+// ---- Mixed-precision path (bcm_kernels_mixed.hip) ----
+// Inner iterate U in FP32 SoA; U1, U1q (physical time levels) in FP64.
+// One workgroup per cube: flux, residual, preconditioned increment and the RK
+// update of stage RK, in place; at RK 3 one row of 8 norm partials per cube.
+void launch_flux_mixed(hipStream_t s, float* U, const double* U1, const double* U1q,
+                       const int8_t* fws, const double* cube_size, int ncube, PhysTab pt,
+                       double deltaT, double deltaTau, double e, int RK, double* partials);
+// U1q <- U1, U1 <- U (widened) on interior cells.
+void launch_shift_mixed(hipStream_t s, const float* U, double* U1, double* U1q, int ncube,
+                        const int* cube_case);
+// FP64 AoS <-> FP32 SoA of ncube whole cubes.
+void launch_aos_to_soa_f(hipStream_t s, const double* aos, float* soa, int ncube);
+void launch_soa_to_aos_f(hipStream_t s, const float* soa, double* aos, int ncube);
 
 }  // namespace bcm

@@ -912,7 +912,8 @@ __global__ __launch_bounds__(WG) void k_reduce(const double* __restrict__ partia
 // .cpp:43-118), then Y (BCM_Y_boundary_condition.cpp:36-80), then Z
 // (BCM_Z_boundary_condition.cpp:42-92).  mask bits: 1 x-min, 2 x-max, 4 y-min,
 // 8 y-max, 16 z-min, 32 z-max.
-__global__ __launch_bounds__(WG) void k_bc(double* __restrict__ U, const int* __restrict__ cubes,
+template <typename T>
+__global__ __launch_bounds__(WG) void k_bc(T* __restrict__ U, const int* __restrict__ cubes,
                                            const uint8_t* __restrict__ masks, PhysTab pt) {
   const int c = cubes[blockIdx.x];
   const DevPhys ph = pt.phys[pt.cube_case[c]];
@@ -987,7 +988,8 @@ __global__ __launch_bounds__(WG) void k_bc(double* __restrict__ U, const int* __
 // whose corners it overwrites), so all entries of one level run in parallel
 // and levels are separated by workgroup barriers.  Interpolation corners
 // always lie in the ghost cell's own padded cube.
-__global__ __launch_bounds__(WG) void k_ghost(double* __restrict__ U,
+template <typename T>
+__global__ __launch_bounds__(WG) void k_ghost(T* __restrict__ U,
                                               const GhostEntry* __restrict__ ent,
                                               const int* __restrict__ cube_level_off,
                                               const int* __restrict__ level_off, PhysTab pt) {
@@ -1095,7 +1097,8 @@ __global__ __launch_bounds__(WG) void k_ghost(double* __restrict__ U,
 // Edge-ghost averaging of one cube in the reference's three sequential passes
 // (Interface.cpp:5640-5732): i-edges, then j-edges, then k-edges.  Called by a
 // whole workgroup; the face ghosts it reads must be complete.
-__device__ __forceinline__ void edge_fill(double* __restrict__ U, int c) {
+template <typename T>
+__device__ __forceinline__ void edge_fill(T* __restrict__ U, int c) {
   const int t = threadIdx.x;
   // i-edges at (j,k) in {1,18}^2, i = 2..17.
   for (int w = t; w < 16 * 4 * 5; w += WG) {
@@ -1140,7 +1143,8 @@ __device__ __forceinline__ void edge_fill(double* __restrict__ U, int c) {
 // cube's edge-ghost averaging in the reference's three sequential passes
 // (:5640-5732).  Sources are interior layers, which nothing in this kernel
 // writes, so cubes proceed independently.
-__global__ __launch_bounds__(WG) void k_halo_edge(double* __restrict__ U,
+template <typename T>
+__global__ __launch_bounds__(WG) void k_halo_edge(T* __restrict__ U,
                                                   const HaloSrc* __restrict__ srcs,
                                                   const int* __restrict__ cube_case) {
   const int c = blockIdx.x + 1;
@@ -1176,7 +1180,8 @@ __global__ __launch_bounds__(WG) void k_halo_edge(double* __restrict__ U,
 // layers that no gather writes.  The full gather therefore reduces to copying
 // those ghost cells that lie in a neighbour's source layers, followed by the
 // edge passes, which read face ghosts the BCs may have changed.
-__global__ __launch_bounds__(WG) void k_halo_sparse(double* __restrict__ U,
+template <typename T>
+__global__ __launch_bounds__(WG) void k_halo_sparse(T* __restrict__ U,
                                                     const int2* __restrict__ pairs, int n) {
   const int g = blockIdx.x * WG + threadIdx.x;
   if (g >= n) return;
@@ -1187,7 +1192,8 @@ __global__ __launch_bounds__(WG) void k_halo_sparse(double* __restrict__ U,
   for (int v = 0; v < 5; ++v) U[dd + (size_t)v * PLANE] = U[so + (size_t)v * PLANE];
 }
 
-__global__ __launch_bounds__(WG) void k_edge(double* __restrict__ U,
+template <typename T>
+__global__ __launch_bounds__(WG) void k_edge(T* __restrict__ U,
                                              const int* __restrict__ cube_case) {
   if (cube_case[blockIdx.x + 1] < 0) return;
   edge_fill(U, blockIdx.x + 1);
@@ -1235,30 +1241,64 @@ __global__ __launch_bounds__(WG) void k_soa_to_aos(const double* __restrict__ so
 // ---------------------------------------------------------------------------
 int update_blocks(int ncube) { return 16 * (ncube - 1); }
 
+template <typename T>
+void launch_bc_t(hipStream_t s, T* U, const int* cubes, const uint8_t* masks, int n, PhysTab pt) {
+  if (n > 0) hipLaunchKernelGGL(k_bc<T>, dim3(n), dim3(WG), 0, s, U, cubes, masks, pt);
+}
 void launch_bc(hipStream_t s, double* U, const int* cubes, const uint8_t* masks, int n,
                PhysTab pt) {
-  if (n > 0) hipLaunchKernelGGL(k_bc, dim3(n), dim3(WG), 0, s, U, cubes, masks, pt);
+  launch_bc_t(s, U, cubes, masks, n, pt);
+}
+void launch_bc(hipStream_t s, float* U, const int* cubes, const uint8_t* masks, int n,
+               PhysTab pt) {
+  launch_bc_t(s, U, cubes, masks, n, pt);
 }
 
+template <typename T>
+void launch_ghost_t(hipStream_t s, T* U, const GhostEntry* e, const int* cube_level_off,
+                    const int* level_off, int ncubes_with_ghosts, PhysTab pt) {
+  if (ncubes_with_ghosts > 0)
+    hipLaunchKernelGGL(k_ghost<T>, dim3(ncubes_with_ghosts), dim3(WG), 0, s, U, e,
+                       cube_level_off, level_off, pt);
+}
 void launch_ghost(hipStream_t s, double* U, const GhostEntry* e, const int* cube_level_off,
                   const int* level_off, int ncubes_with_ghosts, PhysTab pt) {
-  if (ncubes_with_ghosts > 0)
-    hipLaunchKernelGGL(k_ghost, dim3(ncubes_with_ghosts), dim3(WG), 0, s, U, e, cube_level_off,
-                       level_off, pt);
+  launch_ghost_t(s, U, e, cube_level_off, level_off, ncubes_with_ghosts, pt);
+}
+void launch_ghost(hipStream_t s, float* U, const GhostEntry* e, const int* cube_level_off,
+                  const int* level_off, int ncubes_with_ghosts, PhysTab pt) {
+  launch_ghost_t(s, U, e, cube_level_off, level_off, ncubes_with_ghosts, pt);
 }
 
+template <typename T>
+void launch_halo_edge_t(hipStream_t s, T* U, const HaloSrc* src, int ncube, const int* cube_case) {
+  if (ncube > 1)
+    hipLaunchKernelGGL(k_halo_edge<T>, dim3(ncube - 1), dim3(WG), 0, s, U, src, cube_case);
+}
 void launch_halo_edge(hipStream_t s, double* U, const HaloSrc* src, int ncube,
                       const int* cube_case) {
-  if (ncube > 1)
-    hipLaunchKernelGGL(k_halo_edge, dim3(ncube - 1), dim3(WG), 0, s, U, src, cube_case);
+  launch_halo_edge_t(s, U, src, ncube, cube_case);
+}
+void launch_halo_edge(hipStream_t s, float* U, const HaloSrc* src, int ncube,
+                      const int* cube_case) {
+  launch_halo_edge_t(s, U, src, ncube, cube_case);
 }
 
+template <typename T>
+void launch_halo_sparse_t(hipStream_t s, T* U, const int2* pairs, int npairs, int ncube,
+                          const int* cube_case) {
+  if (npairs > 0)
+    hipLaunchKernelGGL(k_halo_sparse<T>, dim3((npairs + WG - 1) / WG), dim3(WG), 0, s, U, pairs,
+                       npairs);
+  if (ncube > 1) hipLaunchKernelGGL(k_edge<T>, dim3(ncube - 1), dim3(WG), 0, s, U, cube_case);
+}
 void launch_halo_sparse(hipStream_t s, double* U, const int2* pairs, int npairs, int ncube,
                         const int* cube_case) {
-  if (npairs > 0)
-    hipLaunchKernelGGL(k_halo_sparse, dim3((npairs + WG - 1) / WG), dim3(WG), 0, s, U, pairs,
-                       npairs);
-  if (ncube > 1) hipLaunchKernelGGL(k_edge, dim3(ncube - 1), dim3(WG), 0, s, U, cube_case);
+  launch_halo_sparse_t(s, U, pairs, npairs, ncube, cube_case);
+}
+void launch_halo_sparse(hipStream_t s, float* U, const int2* pairs, int npairs, int ncube,
+                        const int* cube_case) {
+  launch_halo_sparse_t(s, U, pairs, npairs, ncube, cube_case);
 }
 
 void launch_flux(hipStream_t s, const double* U, const double* U1, const double* U1q,

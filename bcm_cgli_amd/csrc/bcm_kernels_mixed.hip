@@ -1,0 +1,563 @@
+// Mixed-precision form of the per-iteration hot path for gfx950.
+//
+// FP32: the inner iterate U1_ (storage and arithmetic), the inviscid and
+// viscous fluxes, the residual, the preconditioner and the RK stages.
+// FP64: the physical time levels U1 and U1q, the dual-time source
+// -(3u - 4U1 + U1q)/(2 dT) (a difference of nearly equal states), the norms
+// and force sums, and the IB interpolation (bcm_kernels.hip, FP64 arithmetic on
+// the FP32 iterate).  The algorithm is the reference's
+// (BCM_Flux_XYZ_Viscous_Runge_kutta.cpp:326-2655, cited as Flux.cpp); the
+// operation order is free here, so divisions become reciprocal multiplies and
+// square roots the hardware approximations.
+//
+// Kernel shape: one workgroup of 256 lanes per cube, lane (j,k), marching
+// along i.  Each lane keeps its own i-column of conserved values (planes
+// i-3..i+3) and flags (i-2..i+2) in registers for the x-stencils; the plane
+// being processed is staged in LDS for the y- and z-stencils; primitives of
+// three planes sit in an LDS ring for the viscous terms.  After a plane has
+// been staged nothing reads its interior cells from memory again, so each
+// cell's RK update is written in place as soon as it is computed: no stage
+// increment array, no RK scratch arrays.  The reference's RK2/RK3 read U1p1
+// and U1p2, which equal U1_ at every fluid cell when those stages run (only
+// ghost and non-fluid cells change in between), so the stages use U1_.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "bcm_kernels.h"
+
+namespace bcm {
+namespace {
+
+constexpr int WG = 256;
+constexpr float KGf = 1.4f;
+constexpr float KM1f = 0.4f;
+constexpr float IKM1f = 2.5f;          // 1/(K-1)
+constexpr float IRGf = 1.0f / 287.0f;  // 1/R
+constexpr float HEATf = 717.5f * 1.4f / 0.72f;   // Cv K / Pr
+constexpr double KM1d = 1.4 - 1;
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fmaxr(float a, float b) { return (a > b) ? a : b; }
+
+// Preconditioned Roe-type flux at a face with normal axis d (Flux.cpp:611-797).
+__device__ __forceinline__ void roe_f(const float (&L)[5], const float (&R)[5], int d, float e,
+                                      float (&F)[5], float& dP) {
+  const float lr = L[0], ilr = frcp(lr);
+  const float lq[3] = {L[1] * ilr, L[2] * ilr, L[3] * ilr};
+  const float lVV = lq[0] * lq[0] + lq[1] * lq[1] + lq[2] * lq[2];
+  const float lP = (L[4] - 0.5f * lr * lVV) * KM1f;
+  const float lH = 0.5f * lVV + KGf * lP * ilr * IKM1f;
+  const float rr = R[0], irr = frcp(rr);
+  const float rq[3] = {R[1] * irr, R[2] * irr, R[3] * irr};
+  const float rVV = rq[0] * rq[0] + rq[1] * rq[1] + rq[2] * rq[2];
+  const float rP = (R[4] - 0.5f * rr * rVV) * KM1f;
+  const float rH = 0.5f * rVV + KGf * rP * irr * IKM1f;
+
+  const float t5 = fsqrt(lr), t6 = fsqrt(rr), it4 = frcp(t5 + t6);
+  const float rho = t5 * t6;
+  float q[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) q[m] = (t5 * lq[m] + t6 * rq[m]) * it4;
+  const float VV = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+  const float H = (t5 * lH + t6 * rH) * it4;
+  const float C = (H - 0.5f * VV) * KM1f;
+  const float beta = fmaxr(VV * frcp(C), e);
+  const float un = d == 0 ? q[0] : (d == 1 ? q[1] : q[2]);
+  const float lun = d == 0 ? lq[0] : (d == 1 ? lq[1] : lq[2]);
+  const float run = d == 0 ? rq[0] : (d == 1 ? rq[1] : rq[2]);
+  const float temp = 0.5f * (1 + beta) * un;
+  const float S = 0.5f * fsqrt(4 * beta * C + un * un * (1 - beta) * (1 - beta));
+  const float iS = frcp(S);
+  const float dPr = rP - lP, dun = run - lun;
+  const float temp1 = dPr * frcp(rho * beta * C);
+  const float tS = temp * iS;
+  const float temp3 = 0.5f * (1 - beta) * un * tS;
+  const float an = fabsf(un);
+  const float deltaU = (S - temp3 - beta * an) * temp1 + tS * dun;
+  const float deltaP = tS * dPr + (S - an + temp3) * rho * dun;
+  const float dUr = deltaU * rho;
+  F[0] = 0.5f * ((lr * lun + rr * run) - (an * (R[0] - L[0]) + dUr));
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const float fav = an * (R[m + 1] - L[m + 1]) + dUr * q[m] + (m == d ? deltaP : 0.f);
+    const float pf = m == d ? lP + rP : 0.f;
+    F[m + 1] = 0.5f * ((lr * lq[m] * lun + rr * rq[m] * run + pf) - fav);
+  }
+  const float fav4 = an * (R[4] - L[4]) + dUr * H + deltaP * un;
+  F[4] = 0.5f * ((lun * (3.5f * lP + 0.5f * lr * lVV) + run * (3.5f * rP + 0.5f * rr * rVV)) - fav4);
+  dP = 0.5f * (lP + rP);
+}
+
+// Face states of order 5, 3 or 1 (Flux.cpp:499-596) from six cells s0..s5 =
+// p-2..p+3 of the face between p and p+1.
+__device__ __forceinline__ void recon(int order, float s0, float s1, float s2, float s3, float s4,
+                                      float s5, float& L, float& R) {
+  if (order == 5) {
+    L = (1.f / 60) * (2 * s0 - 13 * s1 + 47 * s2 + 27 * s3 - 3 * s4);
+    R = (1.f / 60) * (-3 * s1 + 27 * s2 + 47 * s3 - 13 * s4 + 2 * s5);
+  } else if (order == 3) {
+    L = (1.f / 6) * (-s1 + 5 * s2 + 2 * s3);
+    R = (1.f / 6) * (2 * s2 + 5 * s3 - s4);
+  } else {
+    L = s2;
+    R = s3;
+  }
+}
+
+__device__ __forceinline__ int order_fwd(int p, int fm1, int fp1, int fp2) {
+  if (fp1 == 1 && fm1 == 1) return (p == 17 || fp2 != 1) ? 3 : 5;
+  return 1;
+}
+__device__ __forceinline__ int order_bwd(int q, int fm2, int fm1, int fp1) {
+  if (fp1 == 1 && fm1 == 1) return (q == 2 || fm2 != 1) ? 3 : 5;
+  return 1;
+}
+
+constexpr int PR = 18 * 18;       // primitive plane, j,k in [1,18]
+constexpr int NF = 17 * 16;       // faces of one axis in one plane
+
+// Flux of the face between p and p+1 along axis d (1 or 2) of the staged
+// plane: base is the plane index of cell p, s the index stride of the axis.
+__device__ __forceinline__ void plane_face(const float (*cp)[400], int base, int s, int d,
+                                           int order, float e, float (&F)[5], float& dP) {
+  float L[5], R[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const float* q = cp[v] + base;
+    const float s1 = q[-s], s2 = q[0], s3 = q[s], s4 = q[2 * s];
+    const float s0 = order == 5 ? q[-2 * s] : 0.f;
+    const float s5 = order == 5 ? q[3 * s] : 0.f;
+    recon(order, s0, s1, s2, s3, s4, s5, L[v], R[v]);
+  }
+  roe_f(L, R, d, e, F, dP);
+}
+
+// Border cells of a 20x20 plane (j or k in {0,1,18,19}): 144 of them.
+__device__ __forceinline__ int border20(int t) {
+  const int edge[4] = {0, 1, 18, 19};
+  if (t < 80) return edge[t / 20] * 20 + t % 20;
+  const int u = t - 80;
+  return (2 + u / 4) * 20 + edge[u % 4];
+}
+// Border cells of the 18x18 primitive plane (j or k in {1,18}): 68 of them,
+// as (j,k) in padded-plane numbering.
+__device__ __forceinline__ int border18(int t) {
+  if (t < 18) return 1 * 20 + (1 + t);
+  if (t < 36) return 18 * 20 + (1 + t - 18);
+  if (t < 52) return (2 + t - 36) * 20 + 1;
+  return (2 + t - 52) * 20 + 18;
+}
+
+__global__ __launch_bounds__(WG, 2) void k_flux_mixed(
+    float* __restrict__ U, const double* __restrict__ U1, const double* __restrict__ U1q,
+    const int8_t* __restrict__ fws, const double* __restrict__ cube_size, PhysTab pt,
+    double deltaT, double deltaTau, double e_d, int RK, double* __restrict__ partials) {
+  __shared__ float cp[2][5][400];     // conserved values of plane i (double-buffered)
+  __shared__ int8_t fl[2][400];       // flags of plane i
+  __shared__ float pr[4][4][PR];      // u, v, w, T of planes (ring of 4)
+  __shared__ float fy[5][NF];
+  __shared__ float fz[6][NF];         // flux + face pressure average
+  __shared__ double red[(WG / 64) * 7];
+
+  const int c = blockIdx.x + 1;
+  const int cc = pt.cube_case[c];
+  if (cc < 0) return;
+  const DevPhys ph = pt.phys[cc];
+  const int t = threadIdx.x;
+  const int j = (t >> 4) + 2, k = (t & 15) + 2;
+  const int col = j * 20 + k;
+  const double dxd = cube_size[c] / 16;
+  const float dx = (float)dxd, inv = (float)(1.0 / dxd), vol = (float)(dxd * dxd * dxd);
+  const float e = (float)e_d;
+  const float T0 = (float)ph.T0, iT0 = (float)(1.0 / ph.T0), muL = (float)ph.mu_L;
+  const float sut = (float)(ph.T0 + 110.);
+  const double i2dT = 1.0 / (2 * deltaT);
+  const float dT = (float)deltaT, dtau = (float)deltaTau;
+  float* __restrict__ Uc = U + (size_t)c * CUBE_STRIDE;
+  const int8_t* __restrict__ Fc = fws + (size_t)c * PLANE;
+  const size_t gcol = (size_t)c * CUBE_STRIDE + col;
+
+  // Own column: w[m] = plane i-3+m, fw[m] = flag of plane i-2+m.
+  float w[7][5];
+  int fw[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) w[0][v] = 1.f;
+#pragma unroll
+  for (int m = 1; m < 7; ++m)
+#pragma unroll
+    for (int v = 0; v < 5; ++v) w[m][v] = Uc[v * PLANE + (m - 1) * 400 + col];
+#pragma unroll
+  for (int m = 0; m < 5; ++m) fw[m] = Fc[m * 400 + col];
+
+  auto prims = [](const float (&q)[5], float& u, float& v, float& ww, float& T) {
+    const float ir = frcp(q[0]);
+    u = q[1] * ir;
+    v = q[2] * ir;
+    ww = q[3] * ir;
+    const float P = (q[4] - 0.5f * q[0] * (u * u + v * v + ww * ww)) * KM1f;
+    T = P * ir * IRGf;
+  };
+  // Primitive plane p into ring slot p & 3: the own cell from the column
+  // window (own holds plane p), the 68 border cells from memory.
+  auto fill_prims = [&](int p, const float (&own)[5]) {
+    float* dst = &pr[p & 3][0][0];
+    float u, v, ww, T;
+    prims(own, u, v, ww, T);
+    const int o = (j - 1) * 18 + (k - 1);
+    dst[o] = u;
+    dst[PR + o] = v;
+    dst[2 * PR + o] = ww;
+    dst[3 * PR + o] = T;
+    if (t < 68) {
+      const int cell = border18(t);
+      float q[5];
+#pragma unroll
+      for (int vv = 0; vv < 5; ++vv) q[vv] = Uc[vv * PLANE + p * 400 + cell];
+      prims(q, u, v, ww, T);
+      const int ob = (cell / 20 - 1) * 18 + (cell % 20 - 1);
+      dst[ob] = u;
+      dst[PR + ob] = v;
+      dst[2 * PR + ob] = ww;
+      dst[3 * PR + ob] = T;
+    }
+  };
+  fill_prims(1, w[2]);
+  fill_prims(2, w[3]);
+
+  float cF[5] = {0, 0, 0, 0, 0}, cdP = 0.f;
+  int corder = -1;
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+
+#pragma unroll 1
+  for (int i = 2; i <= 17; ++i) {
+    const int b = i & 1;
+    // ---- stage plane i (conserved, flags) and primitives of plane i+1 ----
+#pragma unroll
+    for (int v = 0; v < 5; ++v) cp[b][v][col] = w[3][v];
+    fl[b][col] = (int8_t)fw[2];
+    if (t < 144) {
+      const int cell = border20(t);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) cp[b][v][cell] = Uc[v * PLANE + i * 400 + cell];
+      fl[b][cell] = Fc[i * 400 + cell];
+    }
+    fill_prims(i + 1, w[4]);
+    __syncthreads();
+
+    // ---- y- and z-faces of plane i: 272 + 272 faces over 256 lanes ----
+    for (int f = t; f < 2 * NF; f += WG) {
+      const bool yface = f < NF;
+      const int g = yface ? f : f - NF;
+      const int p = yface ? g / 16 + 1 : g % 17 + 1;
+      const int base = yface ? p * 20 + (g % 16 + 2) : (g / 17 + 2) * 20 + p;
+      const int s = yface ? 20 : 1;
+      const int8_t* fb = fl[b] + base;
+      const int o = p >= 2 ? order_fwd(p, fb[-s], fb[s], fb[2 * s])
+                           : order_bwd(2, fb[-s], fb[0], fb[2 * s]);
+      float F[5], dP;
+      plane_face(cp[b], base, s, yface ? 1 : 2, o, e, F, dP);
+      if (yface) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) fy[v][g] = F[v];
+      } else {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) fz[v][g] = F[v];
+        fz[5][g] = dP;
+      }
+    }
+    __syncthreads();
+
+    const int IF = fw[2];
+    // ---- inviscid x: forward face from the column, backward carried ----
+    float Rx[5], Ry[5], Rz[5], dPxb, dPxf, dPzb, dPzf;
+    {
+      const int of = order_fwd(i, fw[1], fw[3], fw[4]);
+      float L[5], R[5], Ff[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v)
+        recon(of, w[1][v], w[2][v], w[3][v], w[4][v], w[5][v], w[6][v], L[v], R[v]);
+      roe_f(L, R, 0, e, Ff, dPxf);
+      const int ob = order_bwd(i, fw[0], fw[1], fw[3]);
+      if (i != 2 && ob == corder) {
+        dPxb = cdP;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Rx[v] = Ff[v] - cF[v];
+      } else {
+        float Fb[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+          recon(ob, w[0][v], w[1][v], w[2][v], w[3][v], w[4][v], w[5][v], L[v], R[v]);
+        roe_f(L, R, 0, e, Fb, dPxb);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Rx[v] = Ff[v] - Fb[v];
+      }
+#pragma unroll
+      for (int v = 0; v < 5; ++v) cF[v] = Ff[v];
+      cdP = dPxf;
+      corder = of;
+    }
+    const int8_t* fc = fl[b] + col;
+    // ---- inviscid y ----
+    {
+      const int fm2 = fc[-40], fm1 = fc[-20], fp1 = fc[20];
+      const int ff = (j - 1) * 16 + (k - 2);
+      const int ob = order_bwd(j, fm2, fm1, fp1);
+      if (j == 2 || ob == order_fwd(j - 1, fm2, fc[0], fp1)) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Ry[v] = fy[v][ff] - fy[v][ff - 16];
+      } else {
+        float Fb[5], dPu;
+        plane_face(cp[b], col - 20, 20, 1, ob, e, Fb, dPu);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Ry[v] = fy[v][ff] - Fb[v];
+      }
+    }
+    // ---- inviscid z ----
+    {
+      const int fm2 = fc[-2], fm1 = fc[-1], fp1 = fc[1];
+      const int ff = (j - 2) * 17 + (k - 1);
+      const int ob = order_bwd(k, fm2, fm1, fp1);
+      dPzf = fz[5][ff];
+      if (k == 2 || ob == order_fwd(k - 1, fm2, fc[0], fp1)) {
+        dPzb = fz[5][ff - 1];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Rz[v] = fz[v][ff] - fz[v][ff - 1];
+      } else {
+        float Fb[5];
+        plane_face(cp[b], col - 1, 1, 2, ob, e, Fb, dPzb);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Rz[v] = fz[v][ff] - Fb[v];
+      }
+    }
+
+    // ---- viscous (Flux.cpp:1722-2279) from the primitive ring ----
+    auto Q = [&](int a, int bb, int cq, int m) -> float {
+      return pr[(i + a) & 3][m][(j + bb - 1) * 18 + (k + cq - 1)];
+    };
+    const float meu = Q(0, 0, 0, 0), mev = Q(0, 0, 0, 1), mew = Q(0, 0, 0, 2), meT = Q(0, 0, 0, 3);
+    float vF[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      float Lb[5], Lf[5];
+#pragma unroll
+      for (int side = -1; side <= 1; side += 2) {
+        const int o0 = d == 0 ? side : 0, o1 = d == 1 ? side : 0, o2 = d == 2 ? side : 0;
+        const float fu = Q(o0, o1, o2, 0), fv = Q(o0, o1, o2, 1), fwv = Q(o0, o1, o2, 2),
+                    fT = Q(o0, o1, o2, 3);
+        const float Tf = 0.5f * (meT + fT);
+        const float temp = Tf * iT0;
+        const float mu = muL * temp * fsqrt(temp) * sut * frcp(Tf + 110.f);
+        float g[3][4];
+#pragma unroll
+        for (int tt = 0; tt < 3; ++tt) {
+          if (tt == d) {
+            const float sg = side < 0 ? inv : -inv;
+            g[tt][0] = (meu - fu) * sg;
+            g[tt][1] = (mev - fv) * sg;
+            g[tt][2] = (mew - fwv) * sg;
+            g[tt][3] = (meT - fT) * sg;
+          } else {
+            const int p0 = tt == 0, p1 = tt == 1, p2 = tt == 2;
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+              g[tt][m] = (Q(p0, p1, p2, m) + Q(p0 + o0, p1 + o1, p2 + o2, m) - Q(-p0, -p1, -p2, m) -
+                          Q(-p0 + o0, -p1 + o1, -p2 + o2, m)) * (0.25f * inv);
+          }
+        }
+        const float uf = 0.5f * (meu + fu), vf = 0.5f * (mev + fv), wf = 0.5f * (mew + fwv);
+        const float div = g[0][0] + g[1][1] + g[2][2];
+        float sg[3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+          sg[m] = (m == d) ? mu * (2 * g[d][d] - (2.0f / 3.0f) * div) : mu * (g[m][d] + g[d][m]);
+        float* L = side < 0 ? Lb : Lf;
+        L[1] = sg[0];
+        L[2] = sg[1];
+        L[3] = sg[2];
+        L[4] = uf * sg[0] + vf * sg[1] + wf * sg[2] + mu * HEATf * g[d][3];
+      }
+#pragma unroll
+      for (int v = 1; v < 5; ++v) vF[v] += (Lf[v] - Lb[v]) * inv;
+    }
+
+    // ---- residual with the FP64 dual-time source ----
+    const float* u = w[3];
+    float Rk[5];
+    const size_t gb = gcol + (size_t)i * 400;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const double Rp = -(3.0 * (double)u[v] - 4.0 * U1[gb + (size_t)v * PLANE] +
+                          U1q[gb + (size_t)v * PLANE]) * i2dT;
+      const float Rf = -(Rx[v] + Ry[v] + Rz[v]) * inv + (v == 0 ? 0.f : vF[v]);
+      Rk[v] = (float)(Rp + (double)Rf);
+    }
+
+    // ---- preconditioned increment (Flux.cpp:2295-2375) and RK update ----
+    if (IF == 1) {
+      const float U_ = meu, V_ = mev, W_ = mew;
+      const float rho = u[0], irho = frcp(rho);
+      const float VV = U_ * U_ + V_ * V_ + W_ * W_;
+      const float P = (u[4] - 0.5f * rho * VV) * KM1f;
+      const float C = KGf * P * irho;
+      const float T = P * irho;
+      const float H = 0.5f * VV + C * IKM1f;
+      const float beta = fmaxr(VV * frcp(C), e);
+      const float bm1 = beta - 1;
+      const float itemp = frcp(KGf * T * (2 * dT + 3 * dtau) * (2 * dT + 3 * beta * dtau));
+      const float temp2 = KM1f * bm1 * dT * dT;
+      const float temp3 = H - H * KGf - VV + KGf * (T + VV);
+      const float kTbt = 3 * KGf * T * beta * dtau;
+      const float a4 = 4 * temp2 * itemp;              // 4 temp2 / temp
+      const float b4 = 4 * temp3 * bm1 * dT * dT * itemp;
+      const float d11 = 2 * dT * (-2 * (-KM1f * (H - VV) - temp3 * beta) * dT + kTbt) * itemp;
+      const float dmm[3] = {
+          2 * dT * (2 * KGf * (T - U_ * U_ * bm1) * dT + 2 * U_ * U_ * bm1 * dT + kTbt) * itemp,
+          2 * dT * (2 * KGf * (T - V_ * V_ * bm1) * dT + 2 * V_ * V_ * bm1 * dT + kTbt) * itemp,
+          2 * dT * (2 * KGf * (T - W_ * W_ * bm1) * dT + 2 * W_ * W_ * bm1 * dT + kTbt) * itemp};
+      const float d55 = 2 * dT * (2 * H * KM1f * bm1 * dT + KGf * T * (2 * dT + 3 * beta * dtau)) * itemp;
+      const float vel[3] = {U_, V_, W_};
+      // Row 0: d11, -a4 U, -a4 V, -a4 W, a4.  Rows 1..3 (velocity m): b4 vm,
+      // -a4 vm vn off-diagonal, dmm on the diagonal, a4 vm.  Row 4: b4 H,
+      // -a4 H U, -a4 H V, -a4 H W, d55.
+      const float sv = U_ * Rk[1] + V_ * Rk[2] + W_ * Rk[3];
+      float out[5];
+      out[0] = dtau * (d11 * Rk[0] - a4 * sv + a4 * Rk[4]);
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+        out[m + 1] = dtau * (b4 * vel[m] * Rk[0] - a4 * vel[m] * (sv - vel[m] * Rk[m + 1]) +
+                             dmm[m] * Rk[m + 1] + a4 * vel[m] * Rk[4]);
+      out[4] = dtau * (b4 * H * Rk[0] - a4 * H * sv + d55 * Rk[4]);
+      float un[5];
+      if (RK == 1) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) un[v] = u[v] + out[v];
+      } else if (RK == 2) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) un[v] = 0.75f * u[v] + 0.25f * u[v] + 0.25f * out[v];
+      } else {
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+          un[v] = (1.0f / 3.0f) * u[v] + (2.0f / 3.0f) * u[v] + (2.0f / 3.0f) * out[v];
+      }
+#pragma unroll
+      for (int v = 0; v < 5; ++v) Uc[v * PLANE + i * 400 + col] = un[v];
+      if (RK == 3) {
+        // Change norms in FP64 (Flux.cpp:2530-2566).
+        const double ro = u[0], iro = 1.0 / ro;
+        const double uo = u[1] * iro, vo = u[2] * iro, wo = u[3] * iro;
+        const double Po = ((double)u[4] - 0.5 * ro * (uo * uo + vo * vo + wo * wo)) * KM1d;
+        const double r = un[0], ir = 1.0 / r;
+        const double uu = un[1] * ir, vv = un[2] * ir, ww = un[3] * ir;
+        const double Pn = ((double)un[4] - 0.5 * r * (uu * uu + vv * vv + ww * ww)) * KM1d;
+        const double dTn = Pn * ir - Po * iro;
+        acc[0] += (Pn - Po) * (Pn - Po);
+        acc[1] += (uu - uo) * (uu - uo);
+        acc[2] += (vv - vo) * (vv - vo);
+        acc[3] += (ww - wo) * (ww - wo);
+        acc[4] += dTn * dTn;
+      }
+    } else if (IF < 1 && RK == 3) {
+      // Force density of non-fluid cells (Flux.cpp:2395-2398), summed at RK 3.
+      acc[5] += (double)(-(-(dPxf - dPxb) * inv + vF[1]) * vol);
+      acc[6] += (double)(-(-(dPzf - dPzb) * inv + vF[3]) * vol);
+    }
+
+    // ---- advance the column window ----
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+#pragma unroll
+      for (int v = 0; v < 5; ++v) w[m][v] = w[m + 1][v];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) fw[m] = fw[m + 1];
+    if (i + 4 <= 19) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) w[6][v] = Uc[v * PLANE + (i + 4) * 400 + col];
+    }
+    if (i + 3 <= 19) fw[4] = Fc[(i + 3) * 400 + col];
+  }
+
+  if (RK == 3) {
+#pragma unroll
+    for (int m = 0; m < 7; ++m)
+      for (int off = 32; off > 0; off >>= 1) acc[m] += __shfl_down(acc[m], off, 64);
+    const int lane = t & 63, wave = t >> 6;
+    if (lane == 0)
+#pragma unroll
+      for (int m = 0; m < 7; ++m) red[wave * 7 + m] = acc[m];
+    __syncthreads();
+    if (t == 0)
+#pragma unroll
+      for (int m = 0; m < 7; ++m) {
+        double s = red[m];
+        for (int wv = 1; wv < WG / 64; ++wv) s += red[wv * 7 + m];
+        partials[(size_t)blockIdx.x * 8 + m] = s;
+      }
+  }
+}
+
+__global__ __launch_bounds__(WG) void k_shift_mixed(const float* __restrict__ U,
+                                                    double* __restrict__ U1,
+                                                    double* __restrict__ U1q,
+                                                    const int* __restrict__ cube_case) {
+  const int c = blockIdx.y + 1;
+  if (cube_case[c] < 0) return;
+  const int i = blockIdx.x + 2;
+  const int j = (threadIdx.x >> 4) + 2;
+  const int k = (threadIdx.x & 15) + 2;
+  const size_t b = (size_t)c * CUBE_STRIDE + i * 400 + j * 20 + k;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const size_t o = b + (size_t)v * PLANE;
+    U1q[o] = U1[o];
+    U1[o] = (double)U[o];
+  }
+}
+
+__global__ __launch_bounds__(WG) void k_aos_to_soa_f(const double* __restrict__ aos,
+                                                     float* __restrict__ soa, int ncells) {
+  const int g = blockIdx.x * WG + threadIdx.x;
+  if (g >= ncells) return;
+  const int c = g / PLANE, cell = g % PLANE;
+#pragma unroll
+  for (int v = 0; v < 5; ++v)
+    soa[(size_t)c * CUBE_STRIDE + (size_t)v * PLANE + cell] = (float)aos[(size_t)g * 5 + v];
+}
+
+__global__ __launch_bounds__(WG) void k_soa_to_aos_f(const float* __restrict__ soa,
+                                                     double* __restrict__ aos, int ncells) {
+  const int g = blockIdx.x * WG + threadIdx.x;
+  if (g >= ncells) return;
+  const int c = g / PLANE, cell = g % PLANE;
+#pragma unroll
+  for (int v = 0; v < 5; ++v)
+    aos[(size_t)g * 5 + v] = (double)soa[(size_t)c * CUBE_STRIDE + (size_t)v * PLANE + cell];
+}
+
+}  // namespace
+
+void launch_flux_mixed(hipStream_t s, float* U, const double* U1, const double* U1q,
+                       const int8_t* fws, const double* cube_size, int ncube, PhysTab pt,
+                       double deltaT, double deltaTau, double e, int RK, double* partials) {
+  if (ncube > 1)
+    hipLaunchKernelGGL(k_flux_mixed, dim3(ncube - 1), dim3(WG), 0, s, U, U1, U1q, fws, cube_size,
+                       pt, deltaT, deltaTau, e, RK, partials);
+}
+
+void launch_shift_mixed(hipStream_t s, const float* U, double* U1, double* U1q, int ncube,
+                        const int* cube_case) {
+  if (ncube > 1)
+    hipLaunchKernelGGL(k_shift_mixed, dim3(16, ncube - 1), dim3(WG), 0, s, U, U1, U1q, cube_case);
+}
+
+void launch_aos_to_soa_f(hipStream_t s, const double* aos, float* soa, int ncube) {
+  const int n = ncube * PLANE;
+  hipLaunchKernelGGL(k_aos_to_soa_f, dim3((n + WG - 1) / WG), dim3(WG), 0, s, aos, soa, n);
+}
+
+void launch_soa_to_aos_f(hipStream_t s, const float* soa, double* aos, int ncube) {
+  const int n = ncube * PLANE;
+  hipLaunchKernelGGL(k_soa_to_aos_f, dim3((n + WG - 1) / WG), dim3(WG), 0, s, soa, aos, n);
+}
+
+}  // namespace bcm

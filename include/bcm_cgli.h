@@ -193,11 +193,18 @@ int bcm_engine_ghost_stats(bcm_engine* e, int* stats3);
  * cubes; each case keeps its own physics, tables and norms, and its results
  * are bitwise the ones it gives alone. */
 
-/* Device memory per pool cube, bytes. */
-long long bcm_bytes_per_cube(void);
+/* Device memory per pool cube, bytes, for a precision (0 FP64, 1 mixed). */
+long long bcm_bytes_per_cube(int precision);
 /* Replaces whatever the engine holds by an empty pool of cap_cubes cubes
- * with max_cases case slots. */
-int bcm_engine_alloc_pool(bcm_engine* e, int cap_cubes, int max_cases);
+ * with max_cases case slots.  precision 0: plain FP64, bitwise the
+ * reference.  precision 1: mixed; the inner iterate U1_, the fluxes, the
+ * residual, the preconditioner and the RK stages in FP32; the time levels U1
+ * and U1q, the dual-time source, the norms and the IB interpolation in FP64.
+ * A mixed pool keeps only U1_, U1 and U1q (state arrays 0..2) and runs flux
+ * and update as one kernel (part 7). */
+int bcm_engine_alloc_pool(bcm_engine* e, int cap_cubes, int max_cases, int precision);
+/* 0 FP64, 1 mixed. */
+int bcm_engine_precision(bcm_engine* e);
 /* Adds a case in the first free range of its cube count; *slot receives its
  * slot index.  Fails when no slot or no range is free. */
 int bcm_engine_add_case(bcm_engine* e, const bcm_case* c, int* slot);
