@@ -1,18 +1,28 @@
-"""Throughput benchmark of the plain FP64 per-iteration solver path on one MI355X.
+"""Throughput benchmark of the per-iteration solver path on one MI355X.
 
-Workload: the 16x16x8-cube sphere case (8,388,608 cells, level-5 icosphere,
-M 0.3, Re 300).  One step = one physical time step = 10 inner iterations of
-3 RK stages (domain BCs, IB ghost fill, halo, flux + residual + preconditioned
-update, halo) plus the physical-time shift, replayed as one captured HIP graph.
-The default 200 steps are the case's full run.
+Headline workload (value): the 16x16x8-cube sphere case (8,388,608 cells,
+level-5 icosphere, M 0.3, Re 300) in plain FP64.  One step = one physical time
+step = 10 inner iterations of 3 RK stages (domain BCs, IB ghost fill, halo,
+flux + residual + preconditioned update, halo) plus the physical-time shift,
+replayed as one captured HIP graph.  The default 200 steps are the case's
+full run.
 
-Prints one JSON line.  value = cells x inner iterations / wall seconds / 1e6
-(Mcell-iter/s), inputs resident in HBM before the timed region.
+value = cells x inner iterations / wall seconds / 1e6 (Mcell-iter/s), inputs
+resident in HBM before the timed region.
 
---workload batch runs the batched form instead: 64 independent sphere cases
-(n^3 cubes, n drawn from {2,3,4,5,6,8}; M, Re and sphere position drawn from a
-seed) resident together in one case pool, every kernel launch covering all
-their cubes, 10 inner iterations per physical step.
+The same line also reports, under "forms":
+  mixed_cfg2   the same case and steps in mixed precision, with the relative
+               L2 error of density (fluid cells) and the relative error of the
+               drag coefficient (normalised x-force of the last inner
+               iteration) against the FP64 run, which is bitwise the FP64
+               oracle;
+  batch_fp64 / batch_mixed
+               64 independent sphere cases (n^3 cubes, n drawn from
+               {2,3,4,5,6,8}; M, Re and sphere position drawn from a seed) in
+               one case pool, 50 steps of 10 inner iterations, every launch
+               covering all cases; the mixed form with per-case errors
+               against the FP64 form.
+--workload serve runs the serving loop instead (see bcm_cgli_amd/serve.py).
 """
 import argparse
 import json
@@ -21,40 +31,116 @@ import re
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Mcell-iter/s + %HBM roofline, mixed vs FP64 at ≤1e-4 Cd err; p99 latency"
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
 # Compulsory HBM traffic of one cell and one RK stage of the flux/update pass:
-# read the inner iterate, U1, U1q (40 B each) and the flag (4 B), write the
-# stage result (40 B).
-ALGO_BYTES_PER_CELL_STAGE = 164
+# read the inner iterate (FP64 40 B, FP32 20 B), U1 and U1q (40 B each) and the
+# flag (4 B), write the stage result (40 B or 20 B).
+ALGO_BYTES = {"fp64": 164, "mixed": 124}
+FLUX_KERNEL = {"fp64": "k_flux_march", "mixed": "k_flux_mixed"}
 
 
 def parse_args():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None,
-                    help="timed physical steps (default: 200 single, 50 batch)")
+    ap.add_argument("--steps", type=int, default=200, help="timed physical steps")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="cfg2")
-    ap.add_argument("--workload", default="single", choices=("single", "batch"))
-    ap.add_argument("--cases", type=int, default=64)
     ap.add_argument("--inner", type=int, default=10)
+    ap.add_argument("--config", default="cfg2")
+    ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed"))
+    ap.add_argument("--workload", default="single", choices=("single", "serve"))
+    ap.add_argument("--no-forms", action="store_true", help="skip the mixed and batched forms")
+    ap.add_argument("--batch-cases", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--flux-reps", type=int, default=30)
+    ap.add_argument("--requests", type=int, default=200)
+    ap.add_argument("--rate", type=float, default=None, help="serve: arrivals per second")
+    ap.add_argument("--max-steps", type=int, default=300)
     return ap.parse_args()
 
 
-def read_traffic(config):
-    """Measured HBM bytes per flux launch from the committed rocprofv3 counter summary."""
-    path = os.path.join(ROOT, "profiles", f"flux_{config}_traffic.txt")
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def read_traffic(kernel, config):
+    """Measured HBM bytes per launch from a committed rocprofv3 counter summary."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{config}_traffic.txt")
     if not os.path.exists(path):
         return None
     m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
     return float(m.group(1)) if m else None
+
+
+def run_pool(B, cases, precision, steps, warmup, n_inner, flux_reps, keep_state):
+    """Loads cases into one pool, autotunes, warms up, times `steps` steps.
+    Returns timing, the last inner iteration's er per case and (keep_state)
+    each case's fluid-cell density after warmup + steps steps."""
+    eng = B.Engine(0)
+    eng.alloc_pool(sum(c.ncube - 1 for c in cases), len(cases), precision)
+    slots = [eng.add_case(c) for c in cases]
+    choice = eng.autotune()
+    if warmup > 0:
+        eng.run(warmup, n_inner, record_er=False)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.run(steps, n_inner, record_er=False)
+    eng.sync()
+    wall = time.perf_counter() - t0
+    cells = sum(c.cells() for c in cases)
+    res = {
+        "value": cells * n_inner * steps / wall / 1e6,
+        "wall": wall,
+        "cells": cells,
+        "choice": choice,
+        "er": [eng.case_er(s, n_inner - 1)[1:] for s in slots],
+    }
+    if keep_state:
+        res["rho"] = []
+        for s, c in zip(slots, cases):
+            fluid = c.array("FWS")[1:, 2:18, 2:18, 2:18] == 1
+            res["rho"].append(eng.case_state(s, "U1")[1:, 2:18, 2:18, 2:18, 0][fluid])
+    res["flux_ms"] = eng.time_part("flux_update", RK=2, reps=flux_reps)
+    res["stage_ms"] = eng.time_part("stage", RK=1, reps=max(3, flux_reps // 3))
+    res["parts_ms"] = {p: round(eng.time_part(p, RK=1, reps=flux_reps), 4)
+                       for p in ("bc", "ghost", "halo_refresh", "interface")}
+    if precision == "fp64":
+        res["parts_ms"]["update_unfused"] = round(eng.time_part("update", RK=2, reps=flux_reps), 4)
+    eng.close()
+    return res
+
+
+def errors(mixed, ref):
+    """Per case: relative L2 error of fluid-cell density, relative Cd error."""
+    out = []
+    for rm, rr, em, er in zip(mixed["rho"], ref["rho"], mixed["er"], ref["er"]):
+        l2 = float(np.linalg.norm(rm - rr) / np.linalg.norm(rr))
+        cd = float(abs(em[5] - er[5]) / abs(er[5]))
+        out.append((l2, cd))
+    return out
+
+
+def roofline(precision, cells, flux_ms, config):
+    algo = ALGO_BYTES[precision] * cells
+    achieved = algo / (flux_ms * 1e-3) / 1e9
+    return {
+        "bound": "hbm",
+        "kernel": f"{FLUX_KERNEL[precision]} (flux + residual + preconditioned increment + RK "
+                  "update of one stage, all cubes)",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": read_traffic(FLUX_KERNEL[precision], config),
+        "algorithmic_bytes_per_launch": algo,
+        "kernel_ms": round(flux_ms, 4),
+    }
 
 
 def cpu_baseline(case, n_threads):
@@ -70,80 +156,6 @@ def cpu_baseline(case, n_threads):
     return case.cells() / dt / 1e6, dt
 
 
-def bench_batch(args):
-    import bcm_cgli_amd as B
-    from bcm_cgli_amd.configs import batch_cases
-    params = batch_cases(args.cases)
-    t_setup = time.perf_counter()
-    cases = [B.Case(p) for p in params]
-    setup_s = time.perf_counter() - t_setup
-    cubes = sum(c.ncube - 1 for c in cases)
-    cells = sum(c.cells() for c in cases)
-    print(f"[bench] {len(cases)} cases, {cubes} cubes built in {setup_s:.1f} s", file=sys.stderr,
-          flush=True)
-    eng = B.Engine(0)
-    eng.alloc_pool(cubes, len(cases))
-    for c in cases:
-        eng.add_case(c)
-    n_inner = args.inner
-    choice = eng.autotune()
-    if args.warmup > 0:
-        eng.run(args.warmup, n_inner, record_er=False)
-    eng.sync()
-    t0 = time.perf_counter()
-    eng.run(args.steps, n_inner, record_er=False)
-    eng.sync()
-    wall = time.perf_counter() - t0
-    value = cells * n_inner * args.steps / wall / 1e6
-    print(f"[bench] timed {args.steps} steps in {wall:.2f} s", file=sys.stderr, flush=True)
-    flux_ms = eng.time_part("flux_update", RK=2, reps=args.flux_reps)
-    algo_bytes = ALGO_BYTES_PER_CELL_STAGE * cells
-    achieved = algo_bytes / (flux_ms * 1e-3) / 1e9
-    sizes = {}
-    for p in params:
-        sizes[p["lattice"][0] ** 3] = sizes.get(p["lattice"][0] ** 3, 0) + 1
-    out = {
-        "metric": METRIC,
-        "value": round(value, 3),
-        "unit": "Mcell-iter/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (generated sphere cases, seeded parameters and density noise)",
-        "config": {
-            "workload": f"batch: {len(cases)} independent sphere cases in one case pool, "
-                        f"cube counts drawn from {{8,27,64,125,216,512}}, M in [0.1,0.5], "
-                        f"Re in [50,1000], centre offset +-0.1, {n_inner} inner iterations per step, "
-                        f"plain FP64",
-            "cases": len(cases),
-            "cubes": cubes,
-            "cells": cells,
-            "cases_per_cube_count": sizes,
-            "parallelism": "single GPU, every launch over all cases, hipGraph per physical step",
-            "setup_s": round(setup_s, 3),
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "k_flux_march over all cases (one stage)",
-            "achieved": round(achieved, 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "algorithmic_bytes_per_launch": algo_bytes,
-            "kernel_ms": round(flux_ms, 4),
-        },
-        "flux_choice": choice,
-    }
-    print(json.dumps(out), flush=True)
-    eng.close()
-
-
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -151,60 +163,42 @@ def main():
               "cases are independent and would shard by cube or by case, see DESIGN.md",
               file=sys.stderr)
         sys.exit(2)
-    if args.steps is None:
-        args.steps = 50 if args.workload == "batch" else 200
-    if args.workload == "batch":
-        return bench_batch(args)
+    if args.workload == "serve":
+        from bcm_cgli_amd import serve
+        print(json.dumps(serve.bench(args, METRIC)), flush=True)
+        return
     import bcm_cgli_amd as B
+    from bcm_cgli_amd.configs import batch_cases
     t_setup = time.perf_counter()
     case = B.Case(B.case_params(args.config))
     setup_s = time.perf_counter() - t_setup
-    print(f"[bench] case {args.config} built in {setup_s:.1f} s", file=sys.stderr, flush=True)
-    eng = B.Engine(0)
-    eng.load_case(case)
-    cells = case.cells()
+    log(f"case {args.config} built in {setup_s:.1f} s")
     n_inner = args.inner
-
-    eng.autotune_choice = eng.autotune()
-    # Warmup (also captures the graph).
-    if args.warmup > 0:
-        eng.run(args.warmup, n_inner, record_er=False)
-    eng.sync()
-    t0 = time.perf_counter()
-    eng.run(args.steps, n_inner, record_er=False)
-    eng.sync()
-    wall = time.perf_counter() - t0
-    value = cells * n_inner * args.steps / wall / 1e6
-    print(f"[bench] timed {args.steps} steps in {wall:.2f} s", file=sys.stderr, flush=True)
-
-    # Dominant kernel: flux + residual + preconditioner (+ the RK update when the
-    # tuned choice fuses it), timed with HIP events on the engine stream right
-    # after the timed region (same resident state).
-    flux_ms = eng.time_part("flux_update", RK=2, reps=args.flux_reps)
-    update_ms = eng.time_part("update", RK=2, reps=args.flux_reps)
-    halo_ms = eng.time_part("interface", RK=1, reps=args.flux_reps)
-    stage_ms = eng.time_part("stage", RK=1, reps=max(3, args.flux_reps // 3))
-    algo_bytes = ALGO_BYTES_PER_CELL_STAGE * cells
-    achieved = algo_bytes / (flux_ms * 1e-3) / 1e9
-    traffic = read_traffic(args.config)
+    forms = not args.no_forms
+    main_prec = args.precision
+    r = run_pool(B, [case], main_prec, args.steps, args.warmup, n_inner, args.flux_reps,
+                 keep_state=forms)
+    log(f"{main_prec}: timed {args.steps} steps in {r['wall']:.2f} s, {r['value']:.1f} Mcell-iter/s")
+    cells = case.cells()
+    p = case.params
     out = {
         "metric": METRIC,
-        "value": round(value, 3),
+        "value": round(r["value"], 3),
         "unit": "Mcell-iter/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "ms_per_step": round(r["wall"] / args.steps * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if main_prec == "fp64" else "f32 stages + f64 time levels",
         "data": "synthetic (generated sphere case, seeded density noise)",
         "config": {
-            "workload": f"{args.config}: sphere D=0.25, {case.params['lattice'][0]}x{case.params['lattice'][1]}x"
-                        f"{case.params['lattice'][2]} cubes of 16^3 cells, M={case.params['mach']}, "
-                        f"Re={case.params['reynolds']}, icosphere level {case.params['ico_level']}, "
-                        f"{n_inner} inner iterations per step, plain FP64",
+            "workload": f"{args.config}: sphere D=0.25, {p['lattice'][0]}x{p['lattice'][1]}x"
+                        f"{p['lattice'][2]} cubes of 16^3 cells, M={p['mach']}, Re={p['reynolds']}, "
+                        f"icosphere level {p['ico_level']}, {n_inner} inner iterations per step, "
+                        f"{'plain FP64' if main_prec == 'fp64' else 'mixed precision'}",
             "cells": cells,
             "cubes": case.ncube - 1,
             "inner_per_step": n_inner,
@@ -212,23 +206,66 @@ def main():
             "parallelism": "single GPU, hipGraph per physical step",
             "setup_s": round(setup_s, 3),
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "k_flux_march (flux + residual + preconditioned increment + RK update, one stage)",
-            "achieved": round(achieved, 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "algorithmic_bytes_per_launch": algo_bytes,
-            "kernel_ms": round(flux_ms, 4),
-        },
-        "flux_choice": eng.autotune_choice,
-        "kernels_ms": {"flux_update": round(flux_ms, 4), "update_unfused": round(update_ms, 4),
-                       "halo_edge": round(halo_ms, 4), "stage": round(stage_ms, 4)},
+        "roofline": roofline(main_prec, cells, r["flux_ms"], args.config),
+        "flux_choice": r["choice"],
+        "kernels_ms": {"flux_update": round(r["flux_ms"], 4), "stage": round(r["stage_ms"], 4),
+                       **r["parts_ms"]},
     }
+    if forms:
+        f = {}
+        other = "mixed" if main_prec == "fp64" else "fp64"
+        rm = run_pool(B, [case], other, args.steps, args.warmup, n_inner, args.flux_reps, True)
+        log(f"{other}: {rm['value']:.1f} Mcell-iter/s")
+        ref, mix = (r, rm) if main_prec == "fp64" else (rm, r)
+        (l2, cd), = errors(mix, ref)
+        f["mixed_cfg2"] = {
+            "value": round(mix["value"], 3), "unit": "Mcell-iter/s",
+            "speedup_vs_fp64": round(mix["value"] / ref["value"], 3),
+            "density_rel_l2": l2, "cd_rel_err": cd,
+            "cd_fp64": float(ref["er"][0][5]), "cd_mixed": float(mix["er"][0][5]),
+            "steps_compared": args.warmup + args.steps,
+            "roofline": roofline("mixed", cells, mix["flux_ms"], args.config),
+            "kernels_ms": {"flux_update": round(mix["flux_ms"], 4),
+                           "stage": round(mix["stage_ms"], 4), **mix["parts_ms"]},
+        }
+        params = batch_cases(args.batch_cases)
+        t0 = time.perf_counter()
+        bcases = [B.Case(q) for q in params]
+        bsetup = time.perf_counter() - t0
+        log(f"batch: {len(bcases)} cases built in {bsetup:.1f} s")
+        b64 = run_pool(B, bcases, "fp64", args.batch_steps, 1, n_inner, 10, True)
+        log(f"batch fp64: {b64['value']:.1f} Mcell-iter/s")
+        bmx = run_pool(B, bcases, "mixed", args.batch_steps, 1, n_inner, 10, True)
+        log(f"batch mixed: {bmx['value']:.1f} Mcell-iter/s")
+        errs = errors(bmx, b64)
+        sizes = {}
+        for q in params:
+            sizes[q["lattice"][0] ** 3] = sizes.get(q["lattice"][0] ** 3, 0) + 1
+        batch_cfg = {
+            "workload": f"{len(bcases)} independent sphere cases in one case pool, cube counts "
+                        "drawn from {8,27,64,125,216,512}, M in [0.1,0.5], Re in [50,1000], "
+                        f"centre offset +-0.1, {args.batch_steps} steps of {n_inner} inner iterations",
+            "cases": len(bcases), "cubes": sum(c.ncube - 1 for c in bcases),
+            "cells": b64["cells"], "cases_per_cube_count": sizes, "setup_s": round(bsetup, 3),
+        }
+        f["batch_fp64"] = {
+            "value": round(b64["value"], 3), "unit": "Mcell-iter/s", "config": batch_cfg,
+            "roofline": roofline("fp64", b64["cells"], b64["flux_ms"], "batch"),
+            "flux_choice": b64["choice"],
+        }
+        f["batch_mixed"] = {
+            "value": round(bmx["value"], 3), "unit": "Mcell-iter/s",
+            "speedup_vs_plain_fp64_cfg2": round(bmx["value"] / ref["value"], 3),
+            "speedup_vs_batch_fp64": round(bmx["value"] / b64["value"], 3),
+            "density_rel_l2_max": max(e[0] for e in errs),
+            "cd_rel_err_max": max(e[1] for e in errs),
+            "density_rel_l2_per_case": [float(f"{e[0]:.3e}") for e in errs],
+            "cd_rel_err_per_case": [float(f"{e[1]:.3e}") for e in errs],
+            "roofline": roofline("mixed", bmx["cells"], bmx["flux_ms"], "batch"),
+        }
+        out["forms"] = f
     if not args.no_cpu_baseline:
-        print("[bench] CPU baseline sample", file=sys.stderr, flush=True)
+        log("CPU baseline sample")
         n_threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
         n_threads = max(1, min(n_threads, int(os.environ.get("OMP_NUM_THREADS", n_threads))))
         v, dt = cpu_baseline(case, n_threads)
@@ -241,7 +278,6 @@ def main():
                       f"{dt:.1f} s, FP64 C oracle with OpenMP over cubes",
         }
     print(json.dumps(out), flush=True)
-    eng.close()
 
 
 if __name__ == "__main__":

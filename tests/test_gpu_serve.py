@@ -1,0 +1,64 @@
+"""Serving loop on the GPU: requests arriving while others run are admitted
+into the resident case pool and overlap on the device; each request's final
+state and norms are bitwise the ones the same case gives when run alone.
+"""
+import numpy as np
+import pytest
+
+from bcm_cgli_amd.serve import Server, poisson_requests
+
+pytestmark = pytest.mark.gpu
+
+
+def run_alone(B, params, n_steps, n_inner, precision):
+    case = B.Case(params)
+    e = B.Engine(0)
+    try:
+        e.alloc_pool(case.ncube - 1, 1, precision)
+        s = e.add_case(case)
+        er = e.run(n_steps, n_inner)[:, s, :]
+        return {n: e.case_state(s, n) for n in ("U1_", "U1", "U1q")}, er[-1]
+    finally:
+        e.close()
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_served_results_match_alone(B, precision):
+    reqs = poisson_requests(5, rate=40.0, seed=11, cubes=(27, 64), ico_levels=(3, 3))
+    steps = [6, 2, 3, 2, 4]
+    e = B.Engine(0)
+    srv = Server(e, B.Case, precision=precision, budget_bytes=150 * B.bytes_per_cube(precision),
+                 max_cases=2, n_inner=2, max_steps=100, tol=1e-30, keep_states=True)
+    for (t, p), n in zip(reqs, steps):
+        srv.submit(p, 0.0, max_steps=n)
+    st = srv.run()
+    e.close()
+    assert st["failed"] == 0 and st["requests"] == 5
+    # Budget and slots bind: never more than 2 cases or 150 cubes resident.
+    assert max(n for _, n, _ in srv.trace) <= 2
+    assert max(c for _, _, c in srv.trace) <= 150
+    # Requests overlap on the device: one is admitted while another runs.
+    rs = srv.requests
+    assert any(a.admit < b.admit < a.done for a in rs for b in rs if a is not b)
+    for r, n in zip(rs, steps):
+        assert r.steps == n
+        st1, er1 = run_alone(B, r.params, n, 2, precision)
+        for name in ("U1_", "U1", "U1q"):
+            assert np.array_equal(r.state[name], st1[name]), (r.id, name)
+        assert np.array_equal(r.er, er1[:7]), r.id
+
+
+def test_serve_stats(B):
+    reqs = poisson_requests(4, rate=20.0, seed=5, cubes=(27, 40), ico_levels=(3, 4))
+    e = B.Engine(0)
+    srv = Server(e, B.Case, precision="mixed", budget_bytes=1 << 30, max_cases=8, n_inner=2,
+                 max_steps=3, tol=1e-8)
+    for t, p in reqs:
+        srv.submit(p, t)
+    st = srv.run()
+    e.close()
+    assert st["requests"] == 4 and st["failed"] == 0
+    assert st["mcell_iter_s"] > 0 and st["tts_p99_s"] >= st["tts_p50_s"] > 0
+    for r in srv.requests:
+        assert r.arrival <= r.ready <= r.admit <= r.done
+        assert np.all(np.isfinite(r.er))

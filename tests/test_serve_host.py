@@ -1,0 +1,179 @@
+"""Host logic of the serving loop, on the CPU: admission by HBM budget in
+arrival order, continuous batching, retirement on convergence or step limit,
+and the latency statistics.  The device is a stand-in with the pool
+interface of the engine (first-fit cube ranges, case slots) whose per-case
+residual decays at a rate set by the case; the real engine is exercised by
+the GPU tests.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from bcm_cgli_amd.serve import Server, poisson_requests, residual
+
+
+class FakeCase:
+    def __init__(self, params):
+        self.ncube = params["cubes"] + 1
+        self.decay = params.get("decay", 0.5)
+        self.phys = {"P0": 1.0, "U0": 1.0, "rho0": 1.0}
+        if params.get("fail"):
+            raise RuntimeError("bad case")
+
+    def cells(self):
+        return (self.ncube - 1) * 4096
+
+
+class FakeEngine:
+    """First-fit pool of cubes with case slots; each step multiplies a case's
+    change norms by its decay factor."""
+
+    def __init__(self, step_time=0.0005):
+        self.log = []
+        self.step_time = step_time
+
+    def alloc_pool(self, cap, max_cases, precision):
+        self.cap, self.max_cases = cap, max_cases
+        self.owner = [-1] * (cap + 1)
+        self.slots = {}
+
+    def pool_info(self):
+        best = run = 0
+        for c in range(1, self.cap + 1):
+            run = run + 1 if self.owner[c] < 0 else 0
+            best = max(best, run)
+        return {"active": len(self.slots), "free_range": best, "max_cases": self.max_cases}
+
+    def add_case(self, case):
+        n = case.ncube - 1
+        slot = next(s for s in range(self.max_cases) if s not in self.slots)
+        run = 0
+        for c in range(1, self.cap + 1):
+            run = run + 1 if self.owner[c] < 0 else 0
+            if run == n:
+                for q in range(c - n + 1, c + 1):
+                    self.owner[q] = slot
+                self.slots[slot] = [case, 1.0]
+                self.log.append(("add", slot, n))
+                return slot
+        raise RuntimeError("no free range")
+
+    def remove_case(self, slot):
+        for c in range(1, self.cap + 1):
+            if self.owner[c] == slot:
+                self.owner[c] = -1
+        del self.slots[slot]
+        self.log.append(("remove", slot))
+
+    def run(self, n_steps, n_inner):
+        time.sleep(self.step_time)
+        er = np.zeros((n_inner, self.max_cases, 7))
+        for s, v in self.slots.items():
+            v[1] *= v[0].decay
+            er[:, s, :5] = v[1]
+        used = sum(1 for o in self.owner if o >= 0)
+        assert used <= self.cap
+        return er
+
+    def case_state(self, slot, name):
+        return np.full(3, slot, dtype=float)
+
+
+def make_server(eng, cap_cubes, **kw):
+    return Server(eng, FakeCase, precision="mixed", budget_bytes=cap_cubes * 1000,
+                  bytes_per_cube=1000, **kw)
+
+
+def test_all_requests_finish_and_fifo_admission():
+    eng = FakeEngine()
+    srv = make_server(eng, 100, max_cases=8, n_inner=2, max_steps=5, tol=1e-30)
+    arrivals = [0.0, 0.001, 0.002, 0.003, 0.004]
+    for i, t in enumerate(arrivals):
+        srv.submit({"cubes": 30}, t)
+    st = srv.run()
+    assert st["requests"] == 5 and st["failed"] == 0
+    assert all(r.done is not None and r.steps == 5 for r in srv.requests)
+    admits = [r.admit for r in srv.requests]
+    assert admits == sorted(admits)
+    # 100 cubes hold three 30-cube cases at a time.
+    assert max(n for _, n, _ in srv.trace) == 3
+    assert all(c <= 100 for _, _, c in srv.trace)
+    assert st["tts_p99_s"] >= st["tts_p50_s"] > 0
+
+
+def test_continuous_batching_overlaps_requests():
+    """A request arriving while another runs is admitted before that one ends."""
+    eng = FakeEngine()
+    srv = make_server(eng, 100, max_cases=4, n_inner=1, max_steps=200, tol=1e-30)
+    srv.submit({"cubes": 10}, 0.0)
+    srv.submit({"cubes": 10}, 0.01)
+    srv.run()
+    a, b = srv.requests
+    assert b.admit < a.done
+    assert b.admit >= b.arrival
+
+
+def test_retire_on_convergence_and_reuse_of_cubes():
+    eng = FakeEngine()
+    srv = make_server(eng, 50, max_cases=4, n_inner=1, max_steps=100, tol=1e-3)
+    srv.submit({"cubes": 40, "decay": 0.05}, 0.0)    # converges after 3 steps
+    srv.submit({"cubes": 40, "decay": 0.5}, 0.0)     # waits for the first one's cubes
+    st = srv.run()
+    a, b = srv.requests
+    assert a.converged and a.steps == 3
+    assert b.converged and b.steps == 10
+    assert b.admit >= a.done
+    assert st["converged"] == 2
+    assert [x[0] for x in eng.log] == ["add", "remove", "add", "remove"]
+
+
+def test_setup_failure_and_oversized_request_do_not_stop_the_loop():
+    eng = FakeEngine()
+    srv = make_server(eng, 20, max_cases=4, n_inner=1, max_steps=2, tol=1e-30)
+    srv.submit({"cubes": 5, "fail": True}, 0.0)
+    srv.submit({"cubes": 50}, 0.0)
+    srv.submit({"cubes": 5}, 0.0)
+    st = srv.run()
+    r0, r1, r2 = srv.requests
+    assert r0.error and r1.error and r2.error is None and r2.steps == 2
+    assert st["failed"] == 2
+
+
+def test_slot_limit():
+    eng = FakeEngine()
+    srv = make_server(eng, 1000, max_cases=2, n_inner=1, max_steps=3, tol=1e-30)
+    for _ in range(5):
+        srv.submit({"cubes": 1}, 0.0)
+    srv.run()
+    assert max(n for _, n, _ in srv.trace) == 2
+
+
+def test_residual_scaling():
+    er = np.array([2.0, 3.0, 1.0, 0.5, 8.0, 0.0, 0.0])
+    assert residual(er, (2.0, 1.0, 1.0, 1.0, 4.0)) == 3.0
+
+
+def test_poisson_stream_shape():
+    reqs = poisson_requests(400, rate=5.0, seed=1)
+    t = np.array([a for a, _ in reqs])
+    assert np.all(np.diff(t) > 0)
+    assert abs(len(t) / t[-1] - 5.0) < 1.0
+    cubes = np.array([np.prod(p["lattice"]) for _, p in reqs])
+    assert cubes.min() >= 20 and cubes.max() <= 1100
+    for _, p in reqs:
+        assert 0.1 <= p["mach"] <= 0.5 and 50 <= p["reynolds"] <= 1000
+        assert 3 <= p["ico_level"] <= 5
+        a, b, c = p["lattice"]
+        box = np.array([a, b, c]) * p["cube_size"]
+        ctr = np.array(p["sphere_center"])
+        assert np.all(ctr - p["sphere_diameter"] / 2 > 0) and np.all(ctr + p["sphere_diameter"] / 2 < box)
+
+
+@pytest.mark.slow
+def test_poisson_cases_build():
+    """A few drawn requests build on the host (flags, IB tables)."""
+    import bcm_cgli_amd as B
+    for _, p in poisson_requests(3, rate=1.0, seed=3, cubes=(27, 80)):
+        c = B.Case(p)
+        assert c.info["nbc_minus"] + c.info["nbc_plus"] > 0
