@@ -533,13 +533,17 @@ int engine_case_state(Engine* e, int slot, int which, double* aos, bool upload,
     if (which == 0) e->halo_full_next = true;
     HIP_TRY(hipMemcpyAsync(e->staging, aos + CUBE_STRIDE, n * cube_bytes, hipMemcpyHostToDevice,
                            e->stream));
+    double ref[5];
+    ref_state(S.tb.ph, ref);
     if (f32)
-      launch_aos_to_soa_f(e->stream, e->staging, dstf, (int)n);
+      launch_aos_to_soa_f(e->stream, e->staging, dstf, (int)n, ref);
     else
       launch_aos_to_soa(e->stream, e->staging, dst, (int)n);
   } else {
+    double ref[5];
+    ref_state(S.tb.ph, ref);
     if (f32)
-      launch_soa_to_aos_f(e->stream, dstf, e->staging, (int)n);
+      launch_soa_to_aos_f(e->stream, dstf, e->staging, (int)n, ref);
     else
       launch_soa_to_aos(e->stream, dst, e->staging, (int)n);
     HIP_TRY(hipMemcpyAsync(aos + CUBE_STRIDE, e->staging, n * cube_bytes, hipMemcpyDeviceToHost,
@@ -723,7 +727,7 @@ void engine_update(Engine* e, int RK, int er_slot) {
 
 void engine_shift(Engine* e) {
   if (e->prec == 1)
-    launch_shift_mixed(e->stream, e->Uf, e->U1, e->U1q, e->ncube, e->cube_case);
+    launch_shift_mixed(e->stream, e->Uf, e->U1, e->U1q, e->ncube, e->pt());
   else
     launch_shift(e->stream, e->U, e->U1, e->U1q, e->ncube, e->cube_case);
 }

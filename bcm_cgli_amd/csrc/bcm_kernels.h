@@ -18,6 +18,17 @@ struct DevPhys {
   double rho0, U0, Uin, V0, W0, P0, T0, mu_L;
 };
 
+// Freestream conserved state of a case: (rho0, rho0 Uin, rho0 V0, rho0 W0,
+// P0/(K-1) + rho0 |V|^2 / 2).  The mixed-precision path stores the inner
+// iterate as its FP32 deviation from this state.
+__host__ __device__ inline void ref_state(const DevPhys& ph, double (&u)[5]) {
+  u[0] = ph.rho0;
+  u[1] = ph.rho0 * ph.Uin;
+  u[2] = ph.rho0 * ph.V0;
+  u[3] = ph.rho0 * ph.W0;
+  u[4] = ph.P0 / (1.4 - 1) + 0.5 * ph.rho0 * (ph.Uin * ph.Uin + ph.V0 * ph.V0 + ph.W0 * ph.W0);
+}
+
 // Physics of every cube of a pool that holds several independent cases:
 // phys[cube_case[c]] is the case constants of cube c; cube_case[c] < 0 marks
 // a cube that belongs to no case (kernels skip it).
@@ -46,7 +57,9 @@ struct HaloSrc {
 
 // Launches cover pool cubes 1..ncube-1; "ncube" is one past the last cube in use.
 // The BC, IB fill and halo kernels exist for an FP64 inner iterate and for the
-// FP32 one of the mixed-precision path (arithmetic in FP64 either way).
+// FP32 one of the mixed-precision path, which holds the deviation from the
+// case's freestream state (ref_state); arithmetic is FP64 on full states
+// either way, copies move stored values as they are.
 void launch_bc(hipStream_t s, double* U, const int* cubes, const uint8_t* masks, int n,
                PhysTab pt);
 void launch_bc(hipStream_t s, float* U, const int* cubes, const uint8_t* masks, int n,
@@ -90,17 +103,21 @@ void launch_soa_to_aos(hipStream_t s, const double* soa, double* aos, int ncube)
 int update_blocks(int ncube);
 
 // ---- Mixed-precision path (bcm_kernels_mixed.hip) ----
-// Inner iterate U in FP32 SoA; U1, U1q (physical time levels) in FP64.
+// Inner iterate U in FP32 SoA as the deviation from ref_state of the cube's
+// case; U1, U1q (physical time levels) in FP64 as full states.
 // One workgroup per cube: flux, residual, preconditioned increment and the RK
 // update of stage RK, in place; at RK 3 one row of 8 norm partials per cube.
 void launch_flux_mixed(hipStream_t s, float* U, const double* U1, const double* U1q,
                        const int8_t* fws, const double* cube_size, int ncube, PhysTab pt,
                        double deltaT, double deltaTau, double e, int RK, double* partials);
-// U1q <- U1, U1 <- U (widened) on interior cells.
+// U1q <- U1, U1 <- ref_state + U on interior cells.
 void launch_shift_mixed(hipStream_t s, const float* U, double* U1, double* U1q, int ncube,
-                        const int* cube_case);
-// FP64 AoS <-> FP32 SoA of ncube whole cubes.
-void launch_aos_to_soa_f(hipStream_t s, const double* aos, float* soa, int ncube);
-void launch_soa_to_aos_f(hipStream_t s, const float* soa, double* aos, int ncube);
+                        PhysTab pt);
+// FP64 AoS full states <-> FP32 SoA deviations from the reference state ref
+// (5 doubles) of ncube whole cubes of one case.
+void launch_aos_to_soa_f(hipStream_t s, const double* aos, float* soa, int ncube,
+                         const double* ref5);
+void launch_soa_to_aos_f(hipStream_t s, const float* soa, double* aos, int ncube,
+                         const double* ref5);
 
 }  // namespace bcm

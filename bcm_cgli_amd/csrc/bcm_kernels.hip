@@ -29,6 +29,23 @@ __device__ __forceinline__ int fidx(int c, int i, int j, int k) {
 // max(a,b) of the reference: (((a)>(b))?(a):(b)), NaN falls to b.
 __device__ __forceinline__ double rmax(double a, double b) { return (a > b) ? a : b; }
 
+// Full-state access to an inner iterate.  FP64 storage holds the state
+// itself; FP32 storage holds its deviation from the case's reference state.
+struct NoOff {
+  __device__ explicit NoOff(const DevPhys&) {}
+  __device__ double ld(const double* U, size_t o, int) const { return U[o]; }
+  __device__ void st(double* U, size_t o, int, double x) const { U[o] = x; }
+};
+struct RefOff {
+  double r[5];
+  __device__ explicit RefOff(const DevPhys& ph) { ref_state(ph, r); }
+  __device__ double ld(const float* U, size_t o, int v) const { return (double)U[o] + r[v]; }
+  __device__ void st(float* U, size_t o, int v, double x) const { U[o] = (float)(x - r[v]); }
+};
+template <typename T> struct OffOf;
+template <> struct OffOf<double> { using type = NoOff; };
+template <> struct OffOf<float> { using type = RefOff; };
+
 // ---------------------------------------------------------------------------
 // Preconditioned Roe-type flux at one face (Flux.cpp:611-797; X :1061-1247,
 // Y :1520-1695).  D is the face-normal axis.
@@ -917,25 +934,26 @@ __global__ __launch_bounds__(WG) void k_bc(T* __restrict__ U, const int* __restr
                                            const uint8_t* __restrict__ masks, PhysTab pt) {
   const int c = cubes[blockIdx.x];
   const DevPhys ph = pt.phys[pt.cube_case[c]];
+  const typename OffOf<T>::type F(ph);
   const int m = masks[blockIdx.x];
   const int t = threadIdx.x;
   {
     const int j = (t >> 4) + 2, k = (t & 15) + 2;
     if (m & 1) {
-      const double rho = U[sidx(c, 0, 2, j, k)];
-      const double uu = U[sidx(c, 1, 2, j, k)] / rho;
-      const double vv = U[sidx(c, 2, 2, j, k)] / rho;
-      const double ww = U[sidx(c, 3, 2, j, k)] / rho;
+      const double rho = F.ld(U, sidx(c, 0, 2, j, k), 0);
+      const double uu = F.ld(U, sidx(c, 1, 2, j, k), 1) / rho;
+      const double vv = F.ld(U, sidx(c, 2, 2, j, k), 2) / rho;
+      const double ww = F.ld(U, sidx(c, 3, 2, j, k), 3) / rho;
       double VV = uu * uu + vv * vv + ww * ww;
-      const double P = (U[sidx(c, 4, 2, j, k)] - 0.5 * rho * VV) * KM1;
+      const double P = (F.ld(U, sidx(c, 4, 2, j, k), 4) - 0.5 * rho * VV) * KM1;
       VV = ph.Uin * ph.Uin + ph.V0 * ph.V0 + ph.W0 * ph.W0;
       const double E = P / KM1 + 0.5 * rho * VV;
       for (int i = 0; i < 2; ++i) {
-        U[sidx(c, 0, i, j, k)] = rho;
-        U[sidx(c, 1, i, j, k)] = rho * ph.Uin;
-        U[sidx(c, 2, i, j, k)] = rho * ph.V0;
-        U[sidx(c, 3, i, j, k)] = rho * ph.W0;
-        U[sidx(c, 4, i, j, k)] = E;
+        U[sidx(c, 0, i, j, k)] = U[sidx(c, 0, 2, j, k)];
+        F.st(U, sidx(c, 1, i, j, k), 1, rho * ph.Uin);
+        F.st(U, sidx(c, 2, i, j, k), 2, rho * ph.V0);
+        F.st(U, sidx(c, 3, i, j, k), 3, rho * ph.W0);
+        F.st(U, sidx(c, 4, i, j, k), 4, E);
       }
     }
     if (m & 2)
@@ -966,10 +984,15 @@ __global__ __launch_bounds__(WG) void k_bc(T* __restrict__ U, const int* __restr
     const int i = w / 20, j = w % 20;
     if (m & 16)
       for (int v = 0; v < 5; ++v) {
-        const double x = U[sidx(c, v, i, j, 2)];
-        const double y = v == 3 ? -x : x;
-        U[sidx(c, v, i, j, 1)] = y;
-        U[sidx(c, v, i, j, 0)] = y;
+        if (v == 3) {
+          const double y = -F.ld(U, sidx(c, v, i, j, 2), 3);
+          F.st(U, sidx(c, v, i, j, 1), 3, y);
+          F.st(U, sidx(c, v, i, j, 0), 3, y);
+        } else {
+          const T x = U[sidx(c, v, i, j, 2)];
+          U[sidx(c, v, i, j, 1)] = x;
+          U[sidx(c, v, i, j, 0)] = x;
+        }
       }
     if (m & 32)
       for (int v = 0; v < 5; ++v) {
@@ -1000,6 +1023,7 @@ __global__ __launch_bounds__(WG) void k_ghost(T* __restrict__ U,
       const GhostEntry& E = ent[n];
       const int c = E.cube;
       const DevPhys& ph = pt.phys[pt.cube_case[c]];
+      const typename OffOf<T>::type F(ph);
       const double rho0 = ph.rho0, P0 = ph.P0;
       double rc[8], uc[8], vc[8], wc[8], pc[8];
 #pragma unroll
@@ -1008,12 +1032,12 @@ __global__ __launch_bounds__(WG) void k_ghost(T* __restrict__ U,
         const int bq = (m == 2 || m == 4 || m == 6 || m == 7);
         const int cq = (m == 3 || m == 5 || m == 6 || m == 7);
         const size_t o = sidx(c, 0, E.pi + a, E.pj + bq, E.pk + cq);
-        rc[m] = U[o];
-        uc[m] = U[o + PLANE] / rc[m];
-        vc[m] = U[o + 2 * PLANE] / rc[m];
-        wc[m] = U[o + 3 * PLANE] / rc[m];
+        rc[m] = F.ld(U, o, 0);
+        uc[m] = F.ld(U, o + PLANE, 1) / rc[m];
+        vc[m] = F.ld(U, o + 2 * PLANE, 2) / rc[m];
+        wc[m] = F.ld(U, o + 3 * PLANE, 3) / rc[m];
         const double VV = uc[m] * uc[m] + vc[m] * vc[m] + wc[m] * wc[m];
-        pc[m] = (U[o + 4 * PLANE] - 0.5 * rc[m] * VV) * KM1;
+        pc[m] = (F.ld(U, o + 4 * PLANE, 4) - 0.5 * rc[m] * VV) * KM1;
       }
       double rho = E.w[0] * (rc[0] - rho0), P = E.w[0] * (pc[0] - P0);
       double uu = E.w[0] * uc[0], vv = E.w[0] * vc[0], ww = E.w[0] * wc[0];
@@ -1084,11 +1108,11 @@ __global__ __launch_bounds__(WG) void k_ghost(T* __restrict__ U,
         o4 = P / KM1 + 0.5 * rho * VV;
       }
       const size_t g = sidx(c, 0, E.gi, E.gj, E.gk);
-      U[g] = o0;
-      U[g + PLANE] = o1;
-      U[g + 2 * PLANE] = o2;
-      U[g + 3 * PLANE] = o3;
-      U[g + 4 * PLANE] = o4;
+      F.st(U, g, 0, o0);
+      F.st(U, g + PLANE, 1, o1);
+      F.st(U, g + 2 * PLANE, 2, o2);
+      F.st(U, g + 3 * PLANE, 3, o3);
+      F.st(U, g + 4 * PLANE, 4, o4);
     }
     __syncthreads();
   }

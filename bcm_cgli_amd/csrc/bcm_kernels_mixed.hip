@@ -1,7 +1,12 @@
 // Mixed-precision form of the per-iteration hot path for gfx950.
 //
 // FP32: the inner iterate U1_ (storage and arithmetic), the inviscid and
-// viscous fluxes, the residual, the preconditioner and the RK stages.
+// viscous fluxes, the residual, the preconditioner and the RK stages.  The
+// iterate is stored as its deviation d = U1_ - Uref from the case's freestream
+// state Uref, and fluxes are formed as F(U) - F(Uref) from deviations only, so
+// the FP32 rounding applies to the deviations (small away from the body)
+// rather than to the full state, whose pressure and energy carry a large
+// constant offset at these Mach numbers.
 // FP64: the physical time levels U1 and U1q, the dual-time source
 // -(3u - 4U1 + U1q)/(2 dT) (a difference of nearly equal states), the norms
 // and force sums, and the IB interpolation (bcm_kernels.hip, FP64 arithmetic on
@@ -40,36 +45,86 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float fmaxr(float a, float b) { return (a > b) ? a : b; }
 
-// Preconditioned Roe-type flux at a face with normal axis d (Flux.cpp:611-797).
-__device__ __forceinline__ void roe_f(const float (&L)[5], const float (&R)[5], int d, float e,
-                                      float (&F)[5], float& dP) {
-  const float lr = L[0], ilr = frcp(lr);
-  const float lq[3] = {L[1] * ilr, L[2] * ilr, L[3] * ilr};
-  const float lVV = lq[0] * lq[0] + lq[1] * lq[1] + lq[2] * lq[2];
-  const float lP = (L[4] - 0.5f * lr * lVV) * KM1f;
-  const float lH = 0.5f * lVV + KGf * lP * ilr * IKM1f;
-  const float rr = R[0], irr = frcp(rr);
-  const float rq[3] = {R[1] * irr, R[2] * irr, R[3] * irr};
-  const float rVV = rq[0] * rq[0] + rq[1] * rq[1] + rq[2] * rq[2];
-  const float rP = (R[4] - 0.5f * rr * rVV) * KM1f;
-  const float rH = 0.5f * rVV + KGf * rP * irr * IKM1f;
+// Reference (freestream) state of a case in FP32 and the quantities derived
+// from it that the perturbation forms below need.
+struct RefF {
+  float rho, u[3], m[3], E, P, PoR, T;   // PoR = P0/rho0, T = P0/(rho0 R)
+};
 
-  const float t5 = fsqrt(lr), t6 = fsqrt(rr), it4 = frcp(t5 + t6);
+__device__ __forceinline__ RefF make_ref(const DevPhys& ph) {
+  double r[5];
+  ref_state(ph, r);
+  RefF f;
+  f.rho = (float)r[0];
+  f.u[0] = (float)ph.Uin;
+  f.u[1] = (float)ph.V0;
+  f.u[2] = (float)ph.W0;
+  for (int k = 0; k < 3; ++k) f.m[k] = (float)r[k + 1];
+  f.E = (float)r[4];
+  f.P = (float)ph.P0;
+  f.PoR = (float)(ph.P0 / ph.rho0);
+  f.T = (float)(ph.P0 / (ph.rho0 * 287.0));
+  return f;
+}
+
+// One state d = U - Uref in perturbation form: density and its reciprocal,
+// velocity u = u0 + du, pressure P = P0 + dP, total enthalpy H.  du and dP are
+// formed from the deviations only (no difference of nearly equal numbers):
+//   du = (dm - u0 drho) / rho,
+//   dP = (K-1) (dE - (drho |u|^2 + rho0 sum du (u + u0)) / 2).
+struct Side {
+  float r, ir, du[3], u[3], dP, P, H;
+};
+
+__device__ __forceinline__ void side(const float (&d)[5], const RefF& rf, Side& s) {
+  s.r = rf.rho + d[0];
+  s.ir = frcp(s.r);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    s.du[k] = (d[k + 1] - rf.u[k] * d[0]) * s.ir;
+    s.u[k] = rf.u[k] + s.du[k];
+  }
+  const float VV = s.u[0] * s.u[0] + s.u[1] * s.u[1] + s.u[2] * s.u[2];
+  const float dke = 0.5f * (d[0] * VV + rf.rho * (s.du[0] * (s.u[0] + rf.u[0]) +
+                                                  s.du[1] * (s.u[1] + rf.u[1]) +
+                                                  s.du[2] * (s.u[2] + rf.u[2])));
+  s.dP = KM1f * (d[4] - dke);
+  s.P = rf.P + s.dP;
+  s.H = 0.5f * VV + KGf * s.P * s.ir * IKM1f;
+}
+
+// Preconditioned Roe-type flux at a face with normal axis d (Flux.cpp:611-797)
+// in perturbation form: F returns F(U) - F(Uref), which differs from the flux
+// by a constant that cancels in every face difference of the residual.
+//   mass      dm_d
+//   momentum  dm_m u_d + m0_m du_d + dP [m = d]
+//   energy    du_d (E + P) + u0_d (dE + dP)
+// The dissipation is built from differences R - L as in the reference.  dP
+// returns the deviation of the face pressure average.
+__device__ __forceinline__ void roe_f(const float (&L)[5], const float (&R)[5], int d, float e,
+                                      const RefF& rf, float (&F)[5], float& dP) {
+  Side l, r;
+  side(L, rf, l);
+  side(R, rf, r);
+  const float t5 = fsqrt(l.r), t6 = fsqrt(r.r), it4 = frcp(t5 + t6);
   const float rho = t5 * t6;
   float q[3];
 #pragma unroll
-  for (int m = 0; m < 3; ++m) q[m] = (t5 * lq[m] + t6 * rq[m]) * it4;
+  for (int m = 0; m < 3; ++m) q[m] = (t5 * l.u[m] + t6 * r.u[m]) * it4;
   const float VV = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
-  const float H = (t5 * lH + t6 * rH) * it4;
+  const float H = (t5 * l.H + t6 * r.H) * it4;
   const float C = (H - 0.5f * VV) * KM1f;
   const float beta = fmaxr(VV * frcp(C), e);
   const float un = d == 0 ? q[0] : (d == 1 ? q[1] : q[2]);
-  const float lun = d == 0 ? lq[0] : (d == 1 ? lq[1] : lq[2]);
-  const float run = d == 0 ? rq[0] : (d == 1 ? rq[1] : rq[2]);
+  const float lun = d == 0 ? l.u[0] : (d == 1 ? l.u[1] : l.u[2]);
+  const float run = d == 0 ? r.u[0] : (d == 1 ? r.u[1] : r.u[2]);
+  const float ldu = d == 0 ? l.du[0] : (d == 1 ? l.du[1] : l.du[2]);
+  const float rdu = d == 0 ? r.du[0] : (d == 1 ? r.du[1] : r.du[2]);
+  const float u0n = d == 0 ? rf.u[0] : (d == 1 ? rf.u[1] : rf.u[2]);
   const float temp = 0.5f * (1 + beta) * un;
   const float S = 0.5f * fsqrt(4 * beta * C + un * un * (1 - beta) * (1 - beta));
   const float iS = frcp(S);
-  const float dPr = rP - lP, dun = run - lun;
+  const float dPr = r.dP - l.dP, dun = rdu - ldu;
   const float temp1 = dPr * frcp(rho * beta * C);
   const float tS = temp * iS;
   const float temp3 = 0.5f * (1 - beta) * un * tS;
@@ -77,16 +132,19 @@ __device__ __forceinline__ void roe_f(const float (&L)[5], const float (&R)[5], 
   const float deltaU = (S - temp3 - beta * an) * temp1 + tS * dun;
   const float deltaP = tS * dPr + (S - an + temp3) * rho * dun;
   const float dUr = deltaU * rho;
-  F[0] = 0.5f * ((lr * lun + rr * run) - (an * (R[0] - L[0]) + dUr));
+  F[0] = 0.5f * ((L[d + 1] + R[d + 1]) - (an * (R[0] - L[0]) + dUr));
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
     const float fav = an * (R[m + 1] - L[m + 1]) + dUr * q[m] + (m == d ? deltaP : 0.f);
-    const float pf = m == d ? lP + rP : 0.f;
-    F[m + 1] = 0.5f * ((lr * lq[m] * lun + rr * rq[m] * run + pf) - fav);
+    const float cen = (L[m + 1] * lun + rf.m[m] * ldu) + (R[m + 1] * run + rf.m[m] * rdu) +
+                      (m == d ? l.dP + r.dP : 0.f);
+    F[m + 1] = 0.5f * (cen - fav);
   }
+  const float EPl = rf.E + rf.P + L[4] + l.dP, EPr = rf.E + rf.P + R[4] + r.dP;
+  const float cenE = (ldu * EPl + u0n * (L[4] + l.dP)) + (rdu * EPr + u0n * (R[4] + r.dP));
   const float fav4 = an * (R[4] - L[4]) + dUr * H + deltaP * un;
-  F[4] = 0.5f * ((lun * (3.5f * lP + 0.5f * lr * lVV) + run * (3.5f * rP + 0.5f * rr * rVV)) - fav4);
-  dP = 0.5f * (lP + rP);
+  F[4] = 0.5f * (cenE - fav4);
+  dP = 0.5f * (l.dP + r.dP);
 }
 
 // Face states of order 5, 3 or 1 (Flux.cpp:499-596) from six cells s0..s5 =
@@ -120,7 +178,8 @@ constexpr int NF = 17 * 16;       // faces of one axis in one plane
 // Flux of the face between p and p+1 along axis d (1 or 2) of the staged
 // plane: base is the plane index of cell p, s the index stride of the axis.
 __device__ __forceinline__ void plane_face(const float (*cp)[400], int base, int s, int d,
-                                           int order, float e, float (&F)[5], float& dP) {
+                                           int order, float e, const RefF& rf, float (&F)[5],
+                                           float& dP) {
   float L[5], R[5];
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
@@ -130,7 +189,7 @@ __device__ __forceinline__ void plane_face(const float (*cp)[400], int base, int
     const float s5 = order == 5 ? q[3 * s] : 0.f;
     recon(order, s0, s1, s2, s3, s4, s5, L[v], R[v]);
   }
-  roe_f(L, R, d, e, F, dP);
+  roe_f(L, R, d, e, rf, F, dP);
 }
 
 // Border cells of a 20x20 plane (j or k in {0,1,18,19}): 144 of them.
@@ -170,6 +229,9 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
   const double dxd = cube_size[c] / 16;
   const float dx = (float)dxd, inv = (float)(1.0 / dxd), vol = (float)(dxd * dxd * dxd);
   const float e = (float)e_d;
+  const RefF rf = make_ref(ph);
+  double uref[5];
+  ref_state(ph, uref);
   const float T0 = (float)ph.T0, iT0 = (float)(1.0 / ph.T0), muL = (float)ph.mu_L;
   const float sut = (float)(ph.T0 + 110.);
   const double i2dT = 1.0 / (2 * deltaT);
@@ -190,13 +252,15 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
 #pragma unroll
   for (int m = 0; m < 5; ++m) fw[m] = Fc[m * 400 + col];
 
-  auto prims = [](const float (&q)[5], float& u, float& v, float& ww, float& T) {
-    const float ir = frcp(q[0]);
-    u = q[1] * ir;
-    v = q[2] * ir;
-    ww = q[3] * ir;
-    const float P = (q[4] - 0.5f * q[0] * (u * u + v * v + ww * ww)) * KM1f;
-    T = P * ir * IRGf;
+  // Deviations of the viscous block's primitives u, v, w and T = P/(rho R)
+  // from the reference state: dT = (dP - (P0/rho0) drho) / (rho R).
+  auto prims = [&](const float (&q)[5], float& u, float& v, float& ww, float& T) {
+    Side sd;
+    side(q, rf, sd);
+    u = sd.du[0];
+    v = sd.du[1];
+    ww = sd.du[2];
+    T = (sd.dP - rf.PoR * q[0]) * sd.ir * IRGf;
   };
   // Primitive plane p into ring slot p & 3: the own cell from the column
   // window (own holds plane p), the 68 border cells from memory.
@@ -256,7 +320,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
       const int o = p >= 2 ? order_fwd(p, fb[-s], fb[s], fb[2 * s])
                            : order_bwd(2, fb[-s], fb[0], fb[2 * s]);
       float F[5], dP;
-      plane_face(cp[b], base, s, yface ? 1 : 2, o, e, F, dP);
+      plane_face(cp[b], base, s, yface ? 1 : 2, o, e, rf, F, dP);
       if (yface) {
 #pragma unroll
         for (int v = 0; v < 5; ++v) fy[v][g] = F[v];
@@ -277,7 +341,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
 #pragma unroll
       for (int v = 0; v < 5; ++v)
         recon(of, w[1][v], w[2][v], w[3][v], w[4][v], w[5][v], w[6][v], L[v], R[v]);
-      roe_f(L, R, 0, e, Ff, dPxf);
+      roe_f(L, R, 0, e, rf, Ff, dPxf);
       const int ob = order_bwd(i, fw[0], fw[1], fw[3]);
       if (i != 2 && ob == corder) {
         dPxb = cdP;
@@ -288,7 +352,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
 #pragma unroll
         for (int v = 0; v < 5; ++v)
           recon(ob, w[0][v], w[1][v], w[2][v], w[3][v], w[4][v], w[5][v], L[v], R[v]);
-        roe_f(L, R, 0, e, Fb, dPxb);
+        roe_f(L, R, 0, e, rf, Fb, dPxb);
 #pragma unroll
         for (int v = 0; v < 5; ++v) Rx[v] = Ff[v] - Fb[v];
       }
@@ -308,7 +372,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
         for (int v = 0; v < 5; ++v) Ry[v] = fy[v][ff] - fy[v][ff - 16];
       } else {
         float Fb[5], dPu;
-        plane_face(cp[b], col - 20, 20, 1, ob, e, Fb, dPu);
+        plane_face(cp[b], col - 20, 20, 1, ob, e, rf, Fb, dPu);
 #pragma unroll
         for (int v = 0; v < 5; ++v) Ry[v] = fy[v][ff] - Fb[v];
       }
@@ -325,7 +389,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
         for (int v = 0; v < 5; ++v) Rz[v] = fz[v][ff] - fz[v][ff - 1];
       } else {
         float Fb[5];
-        plane_face(cp[b], col - 1, 1, 2, ob, e, Fb, dPzb);
+        plane_face(cp[b], col - 1, 1, 2, ob, e, rf, Fb, dPzb);
 #pragma unroll
         for (int v = 0; v < 5; ++v) Rz[v] = fz[v][ff] - Fb[v];
       }
@@ -335,6 +399,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
     auto Q = [&](int a, int bb, int cq, int m) -> float {
       return pr[(i + a) & 3][m][(j + bb - 1) * 18 + (k + cq - 1)];
     };
+    // Ring values are deviations; gradients are their differences.
     const float meu = Q(0, 0, 0, 0), mev = Q(0, 0, 0, 1), mew = Q(0, 0, 0, 2), meT = Q(0, 0, 0, 3);
     float vF[5] = {0, 0, 0, 0, 0};
 #pragma unroll
@@ -345,7 +410,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
         const int o0 = d == 0 ? side : 0, o1 = d == 1 ? side : 0, o2 = d == 2 ? side : 0;
         const float fu = Q(o0, o1, o2, 0), fv = Q(o0, o1, o2, 1), fwv = Q(o0, o1, o2, 2),
                     fT = Q(o0, o1, o2, 3);
-        const float Tf = 0.5f * (meT + fT);
+        const float Tf = rf.T + 0.5f * (meT + fT);
         const float temp = Tf * iT0;
         const float mu = muL * temp * fsqrt(temp) * sut * frcp(Tf + 110.f);
         float g[3][4];
@@ -365,7 +430,8 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
                           Q(-p0 + o0, -p1 + o1, -p2 + o2, m)) * (0.25f * inv);
           }
         }
-        const float uf = 0.5f * (meu + fu), vf = 0.5f * (mev + fv), wf = 0.5f * (mew + fwv);
+        const float uf = rf.u[0] + 0.5f * (meu + fu), vf = rf.u[1] + 0.5f * (mev + fv),
+                    wf = rf.u[2] + 0.5f * (mew + fwv);
         const float div = g[0][0] + g[1][1] + g[2][2];
         float sg[3];
 #pragma unroll
@@ -382,12 +448,14 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
     }
 
     // ---- residual with the FP64 dual-time source ----
-    const float* u = w[3];
+    float u[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) u[v] = w[3][v];
     float Rk[5];
     const size_t gb = gcol + (size_t)i * 400;
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-      const double Rp = -(3.0 * (double)u[v] - 4.0 * U1[gb + (size_t)v * PLANE] +
+      const double Rp = -(3.0 * ((double)u[v] + uref[v]) - 4.0 * U1[gb + (size_t)v * PLANE] +
                           U1q[gb + (size_t)v * PLANE]) * i2dT;
       const float Rf = -(Rx[v] + Ry[v] + Rz[v]) * inv + (v == 0 ? 0.f : vF[v]);
       Rk[v] = (float)(Rp + (double)Rf);
@@ -395,10 +463,12 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
 
     // ---- preconditioned increment (Flux.cpp:2295-2375) and RK update ----
     if (IF == 1) {
-      const float U_ = meu, V_ = mev, W_ = mew;
-      const float rho = u[0], irho = frcp(rho);
+      Side me;
+      side(w[3], rf, me);
+      const float U_ = me.u[0], V_ = me.u[1], W_ = me.u[2];
+      const float irho = me.ir;
       const float VV = U_ * U_ + V_ * V_ + W_ * W_;
-      const float P = (u[4] - 0.5f * rho * VV) * KM1f;
+      const float P = me.P;
       const float C = KGf * P * irho;
       const float T = P * irho;
       const float H = 0.5f * VV + C * IKM1f;
@@ -444,12 +514,18 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
       for (int v = 0; v < 5; ++v) Uc[v * PLANE + i * 400 + col] = un[v];
       if (RK == 3) {
         // Change norms in FP64 (Flux.cpp:2530-2566).
-        const double ro = u[0], iro = 1.0 / ro;
-        const double uo = u[1] * iro, vo = u[2] * iro, wo = u[3] * iro;
-        const double Po = ((double)u[4] - 0.5 * ro * (uo * uo + vo * vo + wo * wo)) * KM1d;
-        const double r = un[0], ir = 1.0 / r;
-        const double uu = un[1] * ir, vv = un[2] * ir, ww = un[3] * ir;
-        const double Pn = ((double)un[4] - 0.5 * r * (uu * uu + vv * vv + ww * ww)) * KM1d;
+        double fo[5], fn[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          fo[v] = (double)u[v] + uref[v];
+          fn[v] = (double)un[v] + uref[v];
+        }
+        const double ro = fo[0], iro = 1.0 / ro;
+        const double uo = fo[1] * iro, vo = fo[2] * iro, wo = fo[3] * iro;
+        const double Po = (fo[4] - 0.5 * ro * (uo * uo + vo * vo + wo * wo)) * KM1d;
+        const double r = fn[0], ir = 1.0 / r;
+        const double uu = fn[1] * ir, vv = fn[2] * ir, ww = fn[3] * ir;
+        const double Pn = (fn[4] - 0.5 * r * (uu * uu + vv * vv + ww * ww)) * KM1d;
         const double dTn = Pn * ir - Po * iro;
         acc[0] += (Pn - Po) * (Pn - Po);
         acc[1] += (uu - uo) * (uu - uo);
@@ -498,10 +574,12 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
 
 __global__ __launch_bounds__(WG) void k_shift_mixed(const float* __restrict__ U,
                                                     double* __restrict__ U1,
-                                                    double* __restrict__ U1q,
-                                                    const int* __restrict__ cube_case) {
+                                                    double* __restrict__ U1q, PhysTab pt) {
   const int c = blockIdx.y + 1;
-  if (cube_case[c] < 0) return;
+  const int cc = pt.cube_case[c];
+  if (cc < 0) return;
+  double ref[5];
+  ref_state(pt.phys[cc], ref);
   const int i = blockIdx.x + 2;
   const int j = (threadIdx.x >> 4) + 2;
   const int k = (threadIdx.x & 15) + 2;
@@ -510,28 +588,36 @@ __global__ __launch_bounds__(WG) void k_shift_mixed(const float* __restrict__ U,
   for (int v = 0; v < 5; ++v) {
     const size_t o = b + (size_t)v * PLANE;
     U1q[o] = U1[o];
-    U1[o] = (double)U[o];
+    U1[o] = (double)U[o] + ref[v];
   }
 }
 
+struct Ref5 {
+  double r[5];
+};
+
 __global__ __launch_bounds__(WG) void k_aos_to_soa_f(const double* __restrict__ aos,
-                                                     float* __restrict__ soa, int ncells) {
+                                                     float* __restrict__ soa, int ncells,
+                                                     Ref5 ref) {
   const int g = blockIdx.x * WG + threadIdx.x;
   if (g >= ncells) return;
   const int c = g / PLANE, cell = g % PLANE;
 #pragma unroll
   for (int v = 0; v < 5; ++v)
-    soa[(size_t)c * CUBE_STRIDE + (size_t)v * PLANE + cell] = (float)aos[(size_t)g * 5 + v];
+    soa[(size_t)c * CUBE_STRIDE + (size_t)v * PLANE + cell] =
+        (float)(aos[(size_t)g * 5 + v] - ref.r[v]);
 }
 
 __global__ __launch_bounds__(WG) void k_soa_to_aos_f(const float* __restrict__ soa,
-                                                     double* __restrict__ aos, int ncells) {
+                                                     double* __restrict__ aos, int ncells,
+                                                     Ref5 ref) {
   const int g = blockIdx.x * WG + threadIdx.x;
   if (g >= ncells) return;
   const int c = g / PLANE, cell = g % PLANE;
 #pragma unroll
   for (int v = 0; v < 5; ++v)
-    aos[(size_t)g * 5 + v] = (double)soa[(size_t)c * CUBE_STRIDE + (size_t)v * PLANE + cell];
+    aos[(size_t)g * 5 + v] =
+        (double)soa[(size_t)c * CUBE_STRIDE + (size_t)v * PLANE + cell] + ref.r[v];
 }
 
 }  // namespace
@@ -545,19 +631,25 @@ void launch_flux_mixed(hipStream_t s, float* U, const double* U1, const double* 
 }
 
 void launch_shift_mixed(hipStream_t s, const float* U, double* U1, double* U1q, int ncube,
-                        const int* cube_case) {
+                        PhysTab pt) {
   if (ncube > 1)
-    hipLaunchKernelGGL(k_shift_mixed, dim3(16, ncube - 1), dim3(WG), 0, s, U, U1, U1q, cube_case);
+    hipLaunchKernelGGL(k_shift_mixed, dim3(16, ncube - 1), dim3(WG), 0, s, U, U1, U1q, pt);
 }
 
-void launch_aos_to_soa_f(hipStream_t s, const double* aos, float* soa, int ncube) {
+void launch_aos_to_soa_f(hipStream_t s, const double* aos, float* soa, int ncube,
+                         const double* ref5) {
   const int n = ncube * PLANE;
-  hipLaunchKernelGGL(k_aos_to_soa_f, dim3((n + WG - 1) / WG), dim3(WG), 0, s, aos, soa, n);
+  Ref5 r;
+  for (int v = 0; v < 5; ++v) r.r[v] = ref5[v];
+  hipLaunchKernelGGL(k_aos_to_soa_f, dim3((n + WG - 1) / WG), dim3(WG), 0, s, aos, soa, n, r);
 }
 
-void launch_soa_to_aos_f(hipStream_t s, const float* soa, double* aos, int ncube) {
+void launch_soa_to_aos_f(hipStream_t s, const float* soa, double* aos, int ncube,
+                         const double* ref5) {
   const int n = ncube * PLANE;
-  hipLaunchKernelGGL(k_soa_to_aos_f, dim3((n + WG - 1) / WG), dim3(WG), 0, s, soa, aos, n);
+  Ref5 r;
+  for (int v = 0; v < 5; ++v) r.r[v] = ref5[v];
+  hipLaunchKernelGGL(k_soa_to_aos_f, dim3((n + WG - 1) / WG), dim3(WG), 0, s, soa, aos, n, r);
 }
 
 }  // namespace bcm
