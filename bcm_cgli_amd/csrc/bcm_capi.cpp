@@ -50,13 +50,7 @@ struct DropIn {
     int n = 0;
   };
   std::map<std::tuple<int, const int*, int, const int*, int>, Bc> bc;
-  struct Gh {
-    GhostEntry* ent = nullptr;
-    int* clo = nullptr;
-    int* lo = nullptr;
-    int ncubes = 0;
-  };
-  std::map<std::tuple<const int*, const int*, const double*, int>, Gh> gh;
+  std::map<std::tuple<const int*, const int*, const double*, int>, GhostDevice> gh;
   std::map<std::vector<const void*>, HaloSrc*> halo;
   std::vector<void*> owned;
 };
@@ -149,16 +143,14 @@ void run_ghost(const char* where, int* NNBC, double* weight, int* GCindex, int* 
       ordered.push_back(E);
     }
     GhostSchedule gs = build_ghost_schedule(ordered);
-    DropIn::Gh g;
-    g.ncubes = (int)gs.cube_level_off.size() - 1;
-    g.ent = upload_vec(d, gs.entries);
-    g.clo = upload_vec(d, gs.cube_level_off);
-    g.lo = upload_vec(d, gs.level_off);
-    if (!g.ent || !g.clo || !g.lo) fatal(where, "device allocation failed");
+    GhostDevice g;
+    std::string err;
+    if (ghost_upload(gs, g, err)) fatal(where, err);
+    d.owned.insert(d.owned.end(), g.owned.begin(), g.owned.end());
+    g.owned.clear();
     it = d.gh.emplace(key, g).first;
   }
-  launch_ghost(h->e->stream, h->e->U, it->second.ent, it->second.clo, it->second.lo,
-               it->second.ncubes, h->e->pt());
+  launch_ghost(h->e->stream, h->e->U, it->second.t, h->e->pt());
   check(where, hipGetLastError());
 }
 }  // namespace

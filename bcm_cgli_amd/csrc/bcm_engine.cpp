@@ -57,14 +57,12 @@ void free_graph(Engine* e) {
 
 // Tables of the resident cases (rebuilt whenever a case comes or goes).
 void free_tables(Engine* e) {
-  void* ptrs[] = {e->bc_cubes, e->bc_masks,  e->ghost,      e->cube_level_off,
-                  e->level_off, e->halo,     e->halo_pairs, e->case_desc};
+  void* ptrs[] = {e->bc_cubes, e->bc_masks, e->halo, e->halo_pairs, e->case_desc};
   for (void* p : ptrs)
     if (p) hipFree(p);
+  ghost_free(e->ghost);
   e->bc_cubes = nullptr;
   e->bc_masks = nullptr;
-  e->ghost = nullptr;
-  e->cube_level_off = e->level_off = nullptr;
   e->halo = nullptr;
   e->halo_pairs = nullptr;
   e->case_desc = nullptr;
@@ -107,47 +105,140 @@ GhostSchedule build_ghost_schedule(const std::vector<GhostEntry>& ordered) {
   // Group by cube, keeping reference order inside each cube.
   std::map<int, std::vector<int>> by_cube;
   for (int n = 0; n < (int)ordered.size(); ++n) by_cube[ordered[n].cube].push_back(n);
-  std::vector<int> wlev(8000), rlev(8000);
   auto cell = [](int i, int j, int k) { return i * 400 + j * 20 + k; };
+  auto corner = [&](const GhostEntry& E, int m) {
+    return cell(E.pi + (m == 1 || m == 4 || m == 5 || m == 7),
+                E.pj + (m == 2 || m == 4 || m == 6 || m == 7),
+                E.pk + (m == 3 || m == 5 || m == 6 || m == 7));
+  };
+  std::vector<int> pos(8000), cur(8000), wlev(8000), rlev(8000);
   for (auto& kv : by_cube) {
-    std::fill(wlev.begin(), wlev.end(), 0);
-    std::fill(rlev.begin(), rlev.end(), 0);
     const std::vector<int>& idx = kv.second;
-    std::vector<int> level(idx.size());
-    int nlev = 0;
-    for (size_t a = 0; a < idx.size(); ++a) {
+    const int ne = (int)idx.size();
+    // Touched cells: written ghost cells first, then the other corners.
+    std::fill(pos.begin(), pos.end(), -1);
+    std::vector<int16_t> list;
+    for (int a = 0; a < ne; ++a) {
       const GhostEntry& E = ordered[idx[a]];
-      int lv = 1;
-      for (int m = 0; m < 8; ++m) {
-        int ci = cell(E.pi + (m == 1 || m == 4 || m == 5 || m == 7),
-                      E.pj + (m == 2 || m == 4 || m == 6 || m == 7),
-                      E.pk + (m == 3 || m == 5 || m == 6 || m == 7));
-        lv = std::max(lv, wlev[ci] + 1);   // read after an earlier write
+      const int g = cell(E.gi, E.gj, E.gk);
+      if (pos[g] < 0) {
+        pos[g] = (int)list.size();
+        list.push_back((int16_t)g);
       }
-      int g = cell(E.gi, E.gj, E.gk);
-      lv = std::max(lv, rlev[g] + 1);      // write after an earlier read
-      lv = std::max(lv, wlev[g] + 1);      // write after an earlier write
-      level[a] = lv;
-      wlev[g] = lv;
-      for (int m = 0; m < 8; ++m) {
-        int ci = cell(E.pi + (m == 1 || m == 4 || m == 5 || m == 7),
-                      E.pj + (m == 2 || m == 4 || m == 6 || m == 7),
-                      E.pk + (m == 3 || m == 5 || m == 6 || m == 7));
-        rlev[ci] = std::max(rlev[ci], lv);
-      }
-      nlev = std::max(nlev, lv);
     }
+    const int nw = (int)list.size();
+    for (int a = 0; a < ne; ++a)
+      for (int m = 0; m < 8; ++m) {
+        const int ci = corner(ordered[idx[a]], m);
+        if (pos[ci] < 0) {
+          pos[ci] = (int)list.size();
+          list.push_back((int16_t)ci);
+        }
+      }
+    const int nt = (int)list.size();
+    std::vector<GhostEntry> ents(ne);
+    std::vector<int> level(ne);
+    std::vector<int16_t> fin(nt);
+    int nlev = 0;
+    if (nt + ne <= GHOST_LDS_SLOTS) {
+      // Renamed: entry a writes slot nt + a and reads the slot holding the
+      // value the sequential fill would read; only reads of earlier writes
+      // order the levels.
+      std::vector<int> slot_level(nt + ne, 0);
+      for (int n = 0; n < nt; ++n) cur[list[n]] = n;
+      for (int a = 0; a < ne; ++a) {
+        GhostEntry E = ordered[idx[a]];
+        int lv = 1;
+        for (int m = 0; m < 8; ++m) {
+          const int sl = cur[corner(E, m)];
+          E.lc[m] = (int16_t)sl;
+          lv = std::max(lv, slot_level[sl] + 1);
+        }
+        E.lg = (int16_t)(nt + a);
+        slot_level[nt + a] = lv;
+        cur[cell(E.gi, E.gj, E.gk)] = nt + a;
+        ents[a] = E;
+        level[a] = lv;
+        nlev = std::max(nlev, lv);
+      }
+      for (int n = 0; n < nt; ++n) fin[n] = (int16_t)cur[list[n]];
+      gs.max_slots = std::max(gs.max_slots, nt + ne);
+    } else {
+      // In place: an entry also waits for earlier reads and writes of its
+      // ghost cell.
+      std::fill(wlev.begin(), wlev.end(), 0);
+      std::fill(rlev.begin(), rlev.end(), 0);
+      for (int a = 0; a < ne; ++a) {
+        GhostEntry E = ordered[idx[a]];
+        int lv = 1;
+        for (int m = 0; m < 8; ++m) lv = std::max(lv, wlev[corner(E, m)] + 1);
+        const int g = cell(E.gi, E.gj, E.gk);
+        lv = std::max(lv, rlev[g] + 1);
+        lv = std::max(lv, wlev[g] + 1);
+        wlev[g] = lv;
+        for (int m = 0; m < 8; ++m) rlev[corner(E, m)] = std::max(rlev[corner(E, m)], lv);
+        for (int m = 0; m < 8; ++m) E.lc[m] = (int16_t)pos[corner(E, m)];
+        E.lg = (int16_t)pos[g];
+        ents[a] = E;
+        level[a] = lv;
+        nlev = std::max(nlev, lv);
+      }
+      for (int n = 0; n < nt; ++n) fin[n] = (int16_t)n;
+    }
+    gs.touched_off.push_back((int)gs.touched.size());
+    gs.touched_nw.push_back(nw);
+    gs.touched.insert(gs.touched.end(), list.begin(), list.end());
+    gs.touched_final.insert(gs.touched_final.end(), fin.begin(), fin.end());
     gs.cube_level_off.push_back((int)gs.level_off.size());
     for (int lv = 1; lv <= nlev; ++lv) {
       gs.level_off.push_back((int)gs.entries.size());
-      for (size_t a = 0; a < idx.size(); ++a)
-        if (level[a] == lv) gs.entries.push_back(ordered[idx[a]]);
+      for (int a = 0; a < ne; ++a)
+        if (level[a] == lv) gs.entries.push_back(ents[a]);
     }
     gs.max_levels = std::max(gs.max_levels, nlev);
   }
   gs.cube_level_off.push_back((int)gs.level_off.size());
   gs.level_off.push_back((int)gs.entries.size());
+  gs.touched_off.push_back((int)gs.touched.size());
   return gs;
+}
+
+int ghost_upload(const GhostSchedule& gs, GhostDevice& d, std::string& err) {
+  ghost_free(d);
+  GhostEntry* ent;
+  int *clo, *lo, *toff, *tnw;
+  int16_t *tc, *tf;
+  if (dev_upload(&ent, gs.entries, err)) return 1;
+  d.owned.push_back(ent);
+  if (dev_upload(&clo, gs.cube_level_off, err)) return 1;
+  d.owned.push_back(clo);
+  if (dev_upload(&lo, gs.level_off, err)) return 1;
+  d.owned.push_back(lo);
+  if (dev_upload(&tc, gs.touched, err)) return 1;
+  d.owned.push_back(tc);
+  if (dev_upload(&tf, gs.touched_final, err)) return 1;
+  d.owned.push_back(tf);
+  if (dev_upload(&toff, gs.touched_off, err)) return 1;
+  d.owned.push_back(toff);
+  if (dev_upload(&tnw, gs.touched_nw, err)) return 1;
+  d.owned.push_back(tnw);
+  d.t.ent = ent;
+  d.t.cube_level_off = clo;
+  d.t.level_off = lo;
+  d.t.touched = tc;
+  d.t.touched_final = tf;
+  d.t.touched_off = toff;
+  d.t.touched_nw = tnw;
+  d.t.ncubes = (int)gs.cube_level_off.size() - 1;
+  d.t.max_slots = gs.max_slots;
+  return 0;
+}
+
+void ghost_free(GhostDevice& d) {
+  for (void* p : d.owned)
+    if (p) hipFree(p);
+  d.owned.clear();
+  d.t = GhostTables();
 }
 
 size_t engine_bytes_per_cube(int prec) {
@@ -362,20 +453,29 @@ int rebuild_tables(Engine* e, std::string& err) {
       bm.push_back(tb.bc_masks[a]);
     }
     const int e0 = (int)gs.entries.size(), l0 = (int)gs.level_off.size();
+    const int t0 = (int)gs.touched.size();
     for (GhostEntry E : tb.ghost.entries) {
       E.cube += off;
       gs.entries.push_back(E);
     }
     for (size_t a = 0; a + 1 < tb.ghost.level_off.size(); ++a)
       gs.level_off.push_back(tb.ghost.level_off[a] + e0);
-    for (size_t a = 0; a + 1 < tb.ghost.cube_level_off.size(); ++a)
+    for (size_t a = 0; a + 1 < tb.ghost.cube_level_off.size(); ++a) {
       gs.cube_level_off.push_back(tb.ghost.cube_level_off[a] + l0);
+      gs.touched_off.push_back(tb.ghost.touched_off[a] + t0);
+      gs.touched_nw.push_back(tb.ghost.touched_nw[a]);
+    }
+    gs.touched.insert(gs.touched.end(), tb.ghost.touched.begin(), tb.ghost.touched.end());
+    gs.touched_final.insert(gs.touched_final.end(), tb.ghost.touched_final.begin(),
+                            tb.ghost.touched_final.end());
     gs.max_levels = std::max(gs.max_levels, tb.ghost.max_levels);
+    gs.max_slots = std::max(gs.max_slots, tb.ghost.max_slots);
     for (int2 pr : tb.halo_pairs) pairs.push_back(make_int2(pr.x + off * PLANE, pr.y + off * PLANE));
     desc.push_back(CaseDesc{cs.first, tb.n, tb.mpi_ncube, s});
   }
   gs.cube_level_off.push_back((int)gs.level_off.size());
   gs.level_off.push_back((int)gs.entries.size());
+  gs.touched_off.push_back((int)gs.touched.size());
   std::vector<HaloSrc> hs(top);
   for (auto& h : hs) memset(&h, 0, sizeof h);
   for (int s : order) {
@@ -395,9 +495,7 @@ int rebuild_tables(Engine* e, std::string& err) {
   e->ghost_levels = gs.max_levels;
   e->n_halo_pairs = (int)pairs.size();
   if (dev_upload(&e->bc_cubes, bc, err) || dev_upload(&e->bc_masks, bm, err) ||
-      dev_upload(&e->ghost, gs.entries, err) ||
-      dev_upload(&e->cube_level_off, gs.cube_level_off, err) ||
-      dev_upload(&e->level_off, gs.level_off, err) || dev_upload(&e->halo, hs, err) ||
+      ghost_upload(gs, e->ghost, err) || dev_upload(&e->halo, hs, err) ||
       dev_upload(&e->halo_pairs, pairs, err) || dev_upload(&e->case_desc, desc, err))
     return 1;
   HIP_TRY(hipMemcpy(e->phys_dev, phys.data(), phys.size() * sizeof(DevPhys),
@@ -596,11 +694,9 @@ void engine_bc(Engine* e) {
 
 void engine_ghost(Engine* e) {
   if (e->prec == 1)
-    launch_ghost(e->stream, e->Uf, e->ghost, e->cube_level_off, e->level_off, e->n_ghost_cubes,
-                 e->pt());
+    launch_ghost(e->stream, e->Uf, e->ghost.t, e->pt());
   else
-    launch_ghost(e->stream, e->U, e->ghost, e->cube_level_off, e->level_off, e->n_ghost_cubes,
-                 e->pt());
+    launch_ghost(e->stream, e->U, e->ghost.t, e->pt());
 }
 
 void engine_interface(Engine* e) {

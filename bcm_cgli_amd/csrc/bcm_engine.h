@@ -19,11 +19,25 @@ struct GhostSchedule {
   std::vector<GhostEntry> entries;   // grouped per cube, level-major within the cube
   std::vector<int> cube_level_off;   // per ghost cube: first level index (+1 sentinel)
   std::vector<int> level_off;        // per level: first entry index (+1 sentinel)
+  std::vector<int16_t> touched;      // per ghost cube: written ghost cells, then other corners
+  std::vector<int16_t> touched_final;// slot of each written cell's last value
+  std::vector<int> touched_off;      // per ghost cube: first touched cell (+1 sentinel)
+  std::vector<int> touched_nw;       // per ghost cube: written cells
   int max_levels = 0;
+  int max_slots = 0;
 };
 
+// Device copy of a schedule (owns its arrays).
+struct GhostDevice {
+  GhostTables t;
+  std::vector<void*> owned;
+};
+int ghost_upload(const GhostSchedule& gs, GhostDevice& d, std::string& err);
+void ghost_free(GhostDevice& d);
+
 // Builds the level schedule of the sequential in-place IB fill for entries
-// given in reference order (minus list, then plus list).
+// given in reference order (minus list, then plus list): per cube, levels
+// over renamed LDS slots when the cube fits GHOST_LDS_SLOTS, else in place.
 GhostSchedule build_ghost_schedule(const std::vector<GhostEntry>& ordered);
 
 // Device tables of one case in its own cube numbering (cubes 1..n).
@@ -77,9 +91,7 @@ struct Engine {
   int* bc_cubes = nullptr;
   uint8_t* bc_masks = nullptr;
   int n_bc = 0;
-  GhostEntry* ghost = nullptr;
-  int* cube_level_off = nullptr;
-  int* level_off = nullptr;
+  GhostDevice ghost;
   int n_ghost_cubes = 0;
   int n_ghost_entries = 0;
   int ghost_levels = 0;

@@ -47,7 +47,31 @@ struct CaseDesc {
 struct GhostEntry {
   int cube;
   int16_t gi, gj, gk, pi, pj, pk;   // ghost cell and interpolation base corner
+  int16_t lc[8], lg;                // LDS slots read for the corners and written
   double w[8];
+};
+
+// IB fill of one cube in LDS: value slots for the cells its entries touch
+// plus one slot per entry; a cube needing more slots works on memory.
+constexpr int GHOST_LDS_SLOTS = 1536;
+
+// Device tables of the IB fill: entries grouped per cube and level, and per
+// cube the list of cells its entries touch (written ghost cells first).
+// In LDS (slots <= GHOST_LDS_SLOTS) every entry writes a slot of its own and
+// reads the slots of the values the sequential fill would read, so levels
+// follow read-after-write dependencies only; touched_final names the slot
+// holding each written cell's last value.  On memory the fill is in place
+// and levels also order writes after earlier reads and writes of a cell.
+struct GhostTables {
+  const GhostEntry* ent = nullptr;
+  const int* cube_level_off = nullptr;   // per cube: first level (+1 sentinel)
+  const int* level_off = nullptr;        // per level: first entry (+1 sentinel)
+  const int16_t* touched = nullptr;      // cells i*400 + j*20 + k
+  const int16_t* touched_final = nullptr;// slot of the last value of each written cell
+  const int* touched_off = nullptr;      // per cube: first touched cell (+1 sentinel)
+  const int* touched_nw = nullptr;       // per cube: written cells at the head of its list
+  int ncubes = 0;
+  int max_slots = 0;
 };
 
 // Source cubes for the six face halos of a cube (0 = no equal-size neighbour).
@@ -64,10 +88,8 @@ void launch_bc(hipStream_t s, double* U, const int* cubes, const uint8_t* masks,
                PhysTab pt);
 void launch_bc(hipStream_t s, float* U, const int* cubes, const uint8_t* masks, int n,
                PhysTab pt);
-void launch_ghost(hipStream_t s, double* U, const GhostEntry* e, const int* cube_level_off,
-                  const int* level_off, int ncubes_with_ghosts, PhysTab pt);
-void launch_ghost(hipStream_t s, float* U, const GhostEntry* e, const int* cube_level_off,
-                  const int* level_off, int ncubes_with_ghosts, PhysTab pt);
+void launch_ghost(hipStream_t s, double* U, const GhostTables& g, PhysTab pt);
+void launch_ghost(hipStream_t s, float* U, const GhostTables& g, PhysTab pt);
 void launch_halo_edge(hipStream_t s, double* U, const HaloSrc* src, int ncube, const int* cube_case);
 void launch_halo_edge(hipStream_t s, float* U, const HaloSrc* src, int ncube, const int* cube_case);
 // Pre-flux refresh: copy (source, destination) cell pairs (cube*8000 + cell),

@@ -35,12 +35,15 @@ struct NoOff {
   __device__ explicit NoOff(const DevPhys&) {}
   __device__ double ld(const double* U, size_t o, int) const { return U[o]; }
   __device__ void st(double* U, size_t o, int, double x) const { U[o] = x; }
+  __device__ double rt(double x, int) const { return x; }
 };
 struct RefOff {
   double r[5];
   __device__ explicit RefOff(const DevPhys& ph) { ref_state(ph, r); }
   __device__ double ld(const float* U, size_t o, int v) const { return (double)U[o] + r[v]; }
   __device__ void st(float* U, size_t o, int v, double x) const { U[o] = (float)(x - r[v]); }
+  // x as storage would return it.
+  __device__ double rt(double x, int v) const { return (double)(float)(x - r[v]) + r[v]; }
 };
 template <typename T> struct OffOf;
 template <> struct OffOf<double> { using type = NoOff; };
@@ -1010,34 +1013,61 @@ __global__ __launch_bounds__(WG) void k_bc(T* __restrict__ U, const int* __restr
 // the host (an entry's level is above every earlier entry it reads from or
 // whose corners it overwrites), so all entries of one level run in parallel
 // and levels are separated by workgroup barriers.  Interpolation corners
-// always lie in the ghost cell's own padded cube.
+// always lie in the ghost cell's own padded cube.  The cells a cube's entries
+// touch are staged in LDS as FP64 conserved values (full states), every level
+// runs out of LDS with one value slot per entry (see GhostTables), and the
+// last value of each written ghost cell goes back to memory once; a cube
+// needing more than lds_cap slots works on memory in place.  A value written
+// to LDS is rounded as storage would round it, so both ways give the same
+// results.
+
 template <typename T>
-__global__ __launch_bounds__(WG) void k_ghost(T* __restrict__ U,
-                                              const GhostEntry* __restrict__ ent,
-                                              const int* __restrict__ cube_level_off,
-                                              const int* __restrict__ level_off, PhysTab pt) {
-  const int l0 = cube_level_off[blockIdx.x], l1 = cube_level_off[blockIdx.x + 1];
+__global__ __launch_bounds__(WG) void k_ghost(T* __restrict__ U, GhostTables gt, int lds_cap,
+                                              PhysTab pt) {
+  extern __shared__ double gsm[];   // [5][lds_cap]
+  const int l0 = gt.cube_level_off[blockIdx.x], l1 = gt.cube_level_off[blockIdx.x + 1];
+  if (l0 == l1) return;
+  const int c = gt.ent[gt.level_off[l0]].cube;
+  const DevPhys ph = pt.phys[pt.cube_case[c]];
+  const typename OffOf<T>::type F(ph);
+  const double rho0 = ph.rho0, P0 = ph.P0;
+  const int t0 = gt.touched_off[blockIdx.x];
+  const int nt = gt.touched_off[blockIdx.x + 1] - t0;
+  const int ne = gt.level_off[l1] - gt.level_off[l0];
+  const bool lds = nt + ne <= lds_cap;
+  if (lds) {
+    for (int n = threadIdx.x; n < nt; n += WG) {
+      const size_t o = (size_t)c * CUBE_STRIDE + gt.touched[t0 + n];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) gsm[v * lds_cap + n] = F.ld(U, o + (size_t)v * PLANE, v);
+    }
+    __syncthreads();
+  }
   for (int lv = l0; lv < l1; ++lv) {
-    const int e0 = level_off[lv], e1 = level_off[lv + 1];
+    const int e0 = gt.level_off[lv], e1 = gt.level_off[lv + 1];
     for (int n = e0 + threadIdx.x; n < e1; n += WG) {
-      const GhostEntry& E = ent[n];
-      const int c = E.cube;
-      const DevPhys& ph = pt.phys[pt.cube_case[c]];
-      const typename OffOf<T>::type F(ph);
-      const double rho0 = ph.rho0, P0 = ph.P0;
+      const GhostEntry& E = gt.ent[n];
       double rc[8], uc[8], vc[8], wc[8], pc[8];
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         const int a = (m == 1 || m == 4 || m == 5 || m == 7);
         const int bq = (m == 2 || m == 4 || m == 6 || m == 7);
         const int cq = (m == 3 || m == 5 || m == 6 || m == 7);
-        const size_t o = sidx(c, 0, E.pi + a, E.pj + bq, E.pk + cq);
-        rc[m] = F.ld(U, o, 0);
-        uc[m] = F.ld(U, o + PLANE, 1) / rc[m];
-        vc[m] = F.ld(U, o + 2 * PLANE, 2) / rc[m];
-        wc[m] = F.ld(U, o + 3 * PLANE, 3) / rc[m];
+        double q[5];
+        if (lds) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) q[v] = gsm[v * lds_cap + E.lc[m]];
+        } else {
+          const size_t o = sidx(c, 0, E.pi + a, E.pj + bq, E.pk + cq);
+#pragma unroll
+          for (int v = 0; v < 5; ++v) q[v] = F.ld(U, o + (size_t)v * PLANE, v);
+        }
+        rc[m] = q[0];
+        uc[m] = q[1] / rc[m];
+        vc[m] = q[2] / rc[m];
+        wc[m] = q[3] / rc[m];
         const double VV = uc[m] * uc[m] + vc[m] * vc[m] + wc[m] * wc[m];
-        pc[m] = (F.ld(U, o + 4 * PLANE, 4) - 0.5 * rc[m] * VV) * KM1;
+        pc[m] = (q[4] - 0.5 * rc[m] * VV) * KM1;
       }
       double rho = E.w[0] * (rc[0] - rho0), P = E.w[0] * (pc[0] - P0);
       double uu = E.w[0] * uc[0], vv = E.w[0] * vc[0], ww = E.w[0] * wc[0];
@@ -1107,14 +1137,26 @@ __global__ __launch_bounds__(WG) void k_ghost(T* __restrict__ U,
         o3 = rho * ww;
         o4 = P / KM1 + 0.5 * rho * VV;
       }
-      const size_t g = sidx(c, 0, E.gi, E.gj, E.gk);
-      F.st(U, g, 0, o0);
-      F.st(U, g + PLANE, 1, o1);
-      F.st(U, g + 2 * PLANE, 2, o2);
-      F.st(U, g + 3 * PLANE, 3, o3);
-      F.st(U, g + 4 * PLANE, 4, o4);
+      const double out[5] = {o0, o1, o2, o3, o4};
+      if (lds) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) gsm[v * lds_cap + E.lg] = F.rt(out[v], v);
+      } else {
+        const size_t g = sidx(c, 0, E.gi, E.gj, E.gk);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) F.st(U, g + (size_t)v * PLANE, v, out[v]);
+      }
     }
     __syncthreads();
+  }
+  if (lds) {
+    const int nw = gt.touched_nw[blockIdx.x];
+    for (int n = threadIdx.x; n < nw; n += WG) {
+      const size_t o = (size_t)c * CUBE_STRIDE + gt.touched[t0 + n];
+      const int sl = gt.touched_final[t0 + n];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) F.st(U, o + (size_t)v * PLANE, v, gsm[v * lds_cap + sl]);
+    }
   }
 }
 
@@ -1279,19 +1321,17 @@ void launch_bc(hipStream_t s, float* U, const int* cubes, const uint8_t* masks, 
 }
 
 template <typename T>
-void launch_ghost_t(hipStream_t s, T* U, const GhostEntry* e, const int* cube_level_off,
-                    const int* level_off, int ncubes_with_ghosts, PhysTab pt) {
-  if (ncubes_with_ghosts > 0)
-    hipLaunchKernelGGL(k_ghost<T>, dim3(ncubes_with_ghosts), dim3(WG), 0, s, U, e,
-                       cube_level_off, level_off, pt);
+void launch_ghost_t(hipStream_t s, T* U, const GhostTables& g, PhysTab pt) {
+  if (g.ncubes <= 0) return;
+  const int cap = g.max_slots < GHOST_LDS_SLOTS ? g.max_slots : GHOST_LDS_SLOTS;
+  const size_t shmem = (size_t)(cap > 0 ? cap : 1) * 5 * sizeof(double);
+  hipLaunchKernelGGL(k_ghost<T>, dim3(g.ncubes), dim3(WG), shmem, s, U, g, cap, pt);
 }
-void launch_ghost(hipStream_t s, double* U, const GhostEntry* e, const int* cube_level_off,
-                  const int* level_off, int ncubes_with_ghosts, PhysTab pt) {
-  launch_ghost_t(s, U, e, cube_level_off, level_off, ncubes_with_ghosts, pt);
+void launch_ghost(hipStream_t s, double* U, const GhostTables& g, PhysTab pt) {
+  launch_ghost_t(s, U, g, pt);
 }
-void launch_ghost(hipStream_t s, float* U, const GhostEntry* e, const int* cube_level_off,
-                  const int* level_off, int ncubes_with_ghosts, PhysTab pt) {
-  launch_ghost_t(s, U, e, cube_level_off, level_off, ncubes_with_ghosts, pt);
+void launch_ghost(hipStream_t s, float* U, const GhostTables& g, PhysTab pt) {
+  launch_ghost_t(s, U, g, pt);
 }
 
 template <typename T>
