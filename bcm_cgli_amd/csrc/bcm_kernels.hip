@@ -427,8 +427,9 @@ __device__ __forceinline__ int order_bwd(int q, int fm2, int fm1, int fp1) {
 }
 
 // Flux through the face between p and p+1 along axis D.  b indexes cell p
-// (variable 0); only the cells the order needs are loaded.
-template <int D>
+// (variable 0) in an array whose variables are VS apart (PLANE in memory, one
+// 20x20 plane in LDS); only the cells the order needs are loaded.
+template <int D, int VS = PLANE>
 __device__ __forceinline__ void face_flux(const double* __restrict__ U, size_t b, int order,
                                           double e, double (&F)[5], double& dP) {
   constexpr int st = D == 0 ? 400 : (D == 1 ? 20 : 1);
@@ -437,7 +438,7 @@ __device__ __forceinline__ void face_flux(const double* __restrict__ U, size_t b
   if (order == 5) {
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-      const double* q = U + b + (size_t)v * PLANE;
+      const double* q = U + b + (size_t)v * VS;
       const double s0 = q[-2 * st], s1 = q[-st], s2 = q[0], s3 = q[st], s4 = q[2 * st], s5 = q[3 * st];
       L[v] = c60 * (2 * s0 - 13 * s1 + 47 * s2 + 27 * s3 - 3 * s4);
       R[v] = c60 * (-3 * s1 + 27 * s2 + 47 * s3 - 13 * s4 + 2 * s5);
@@ -445,7 +446,7 @@ __device__ __forceinline__ void face_flux(const double* __restrict__ U, size_t b
   } else if (order == 3) {
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-      const double* q = U + b + (size_t)v * PLANE;
+      const double* q = U + b + (size_t)v * VS;
       const double s1 = q[-st], s2 = q[0], s3 = q[st], s4 = q[2 * st];
       L[v] = c6 * (-s1 + 5 * s2 + 2 * s3);
       R[v] = c6 * (2 * s2 + 5 * s3 - s4);
@@ -453,7 +454,7 @@ __device__ __forceinline__ void face_flux(const double* __restrict__ U, size_t b
   } else {
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-      const double* q = U + b + (size_t)v * PLANE;
+      const double* q = U + b + (size_t)v * VS;
       L[v] = q[0];
       R[v] = q[st];
     }
@@ -483,6 +484,8 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
   __shared__ double pr[4][3 * PR_PLANE];   // u, v, w, T = P/(rho R) of 3 planes
   __shared__ double fy[5][NFACE];
   __shared__ double fz[6][NFACE];          // flux + face pressure average
+  __shared__ double cpl[5 * 400];          // conserved values of plane i
+  __shared__ int8_t fpl[400];              // flags of plane i
   const int seg = blockIdx.x % nseg;
   const int c = blockIdx.x / nseg + 1;
   const int cc = pt.cube_case[c];
@@ -575,26 +578,36 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
     // update safe now.
     if (fuse && i - 4 >= i0) update_plane(i - 4);
     fill_prims(i + 1);
-    // y-faces of plane i: face f joins cells j = p and p+1 at k.
-    for (int f = t; f < NFACE; f += WG) {
-      const int p = f / 16 + 1, kk = f % 16 + 2;
-      const int8_t* fb = fws + fidx(c, i, p, kk);
-      const int o = p >= 2 ? order_fwd(p, fb[-20], fb[20], fb[40]) : order_bwd(2, fb[-20], fb[0], fb[40]);
-      double F[5], dP;
-      face_flux<1>(U, sidx(c, 0, i, p, kk), o, e, F, dP);
+    // Plane i (conserved values and flags) into LDS for the y- and z-faces.
+    for (int n = t; n < 400; n += WG) {
+      const size_t o = sidx(c, 0, i, 0, 0) + n;
 #pragma unroll
-      for (int v = 0; v < 5; ++v) fy[v][f] = F[v];
+      for (int v = 0; v < 5; ++v) cpl[v * 400 + n] = U[o + (size_t)v * PLANE];
+      fpl[n] = fws[fidx(c, i, 0, 0) + n];
     }
-    // z-faces of plane i: face f joins cells k = p and p+1 at j.
-    for (int f = t; f < NFACE; f += WG) {
-      const int jj = f / 17 + 2, p = f % 17 + 1;
-      const int8_t* fb = fws + fidx(c, i, jj, p);
-      const int o = p >= 2 ? order_fwd(p, fb[-1], fb[1], fb[2]) : order_bwd(2, fb[-1], fb[0], fb[2]);
-      double F[5], dP;
-      face_flux<2>(U, sidx(c, 0, i, jj, p), o, e, F, dP);
+    __syncthreads();
+    // y-faces (f < 272: joins j = p and p+1 at k) and z-faces (joins k = p
+    // and p+1 at j) of plane i in one pass.
+    for (int f = t; f < 2 * NFACE; f += WG) {
+      if (f < NFACE) {
+        const int p = f / 16 + 1, kk = f % 16 + 2;
+        const int8_t* fb = fpl + p * 20 + kk;
+        const int o = p >= 2 ? order_fwd(p, fb[-20], fb[20], fb[40]) : order_bwd(2, fb[-20], fb[0], fb[40]);
+        double F[5], dP;
+        face_flux<1, 400>(cpl, p * 20 + kk, o, e, F, dP);
 #pragma unroll
-      for (int v = 0; v < 5; ++v) fz[v][f] = F[v];
-      fz[5][f] = dP;
+        for (int v = 0; v < 5; ++v) fy[v][f] = F[v];
+      } else {
+        const int g = f - NFACE;
+        const int jj = g / 17 + 2, p = g % 17 + 1;
+        const int8_t* fb = fpl + jj * 20 + p;
+        const int o = p >= 2 ? order_fwd(p, fb[-1], fb[1], fb[2]) : order_bwd(2, fb[-1], fb[0], fb[2]);
+        double F[5], dP;
+        face_flux<2, 400>(cpl, jj * 20 + p, o, e, F, dP);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) fz[v][g] = F[v];
+        fz[5][g] = dP;
+      }
     }
     __syncthreads();
 
@@ -626,26 +639,28 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
     }
     // ---- inviscid, y ----
     {
-      const int fm2 = fc[-40], fm1 = fc[-20], fp1 = fc[20];
+      const int8_t* fl = fpl + j * 20 + k;
+      const int fm2 = fl[-40], fm1 = fl[-20], fp1 = fl[20];
       const int ff = (j - 1) * 16 + (k - 2), fbk = ff - 16;
       const int ob = order_bwd(j, fm2, fm1, fp1);
-      const bool share = j == 2 || ob == order_fwd(j - 1, fm2, fc[0], fp1);
+      const bool share = j == 2 || ob == order_fwd(j - 1, fm2, fl[0], fp1);
       if (share) {
 #pragma unroll
         for (int v = 0; v < 5; ++v) Ry[v] = fy[v][ff] - fy[v][fbk];
       } else {
         double Fb[5], dPu;
-        face_flux<1>(U, b - 20, ob, e, Fb, dPu);
+        face_flux<1, 400>(cpl, j * 20 + k - 20, ob, e, Fb, dPu);
 #pragma unroll
         for (int v = 0; v < 5; ++v) Ry[v] = fy[v][ff] - Fb[v];
       }
     }
     // ---- inviscid, z ----
     {
-      const int fm2 = fc[-2], fm1 = fc[-1], fp1 = fc[1];
+      const int8_t* fl = fpl + j * 20 + k;
+      const int fm2 = fl[-2], fm1 = fl[-1], fp1 = fl[1];
       const int ff = (j - 2) * 17 + (k - 1), fbk = ff - 1;
       const int ob = order_bwd(k, fm2, fm1, fp1);
-      const bool share = k == 2 || ob == order_fwd(k - 1, fm2, fc[0], fp1);
+      const bool share = k == 2 || ob == order_fwd(k - 1, fm2, fl[0], fp1);
       dPzf = fz[5][ff];
       if (share) {
         dPzb = fz[5][fbk];
@@ -653,7 +668,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
         for (int v = 0; v < 5; ++v) Rz[v] = fz[v][ff] - fz[v][fbk];
       } else {
         double Fb[5];
-        face_flux<2>(U, b - 1, ob, e, Fb, dPzb);
+        face_flux<2, 400>(cpl, j * 20 + k - 1, ob, e, Fb, dPzb);
 #pragma unroll
         for (int v = 0; v < 5; ++v) Rz[v] = fz[v][ff] - Fb[v];
       }
