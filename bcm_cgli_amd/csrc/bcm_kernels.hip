@@ -1224,32 +1224,66 @@ __device__ __forceinline__ void edge_fill(T* __restrict__ U, int c) {
 // cube's edge-ghost averaging in the reference's three sequential passes
 // (:5640-5732).  Sources are interior layers, which nothing in this kernel
 // writes, so cubes proceed independently.
+template <typename T> struct Vec2;
+template <> struct Vec2<double> { using type = double2; };
+template <> struct Vec2<float> { using type = float2; };
+
 template <typename T>
 __global__ __launch_bounds__(WG) void k_halo_edge(T* __restrict__ U,
                                                   const HaloSrc* __restrict__ srcs,
                                                   const int* __restrict__ cube_case) {
+  using T2 = typename Vec2<T>::type;
   const int c = blockIdx.x + 1;
   if (cube_case[c] < 0) return;
   const HaloSrc hs = srcs[c];
   const int t = threadIdx.x;
   const int p = (t >> 4) + 2, r = (t & 15) + 2;
+  // All loads first, then all stores: one memory round trip per lane.  The
+  // two layers of a z-face are adjacent in k and move as one 2-element vector
+  // (rows start 16-byte aligned for FP64 and 8-byte aligned for FP32).
+  // Faces without a neighbour read their own cube (discarded) so that every
+  // load is unconditional.
+  const int sx0 = hs.src[0], sx1 = hs.src[1], sy0 = hs.src[2], sy1 = hs.src[3];
+  const int sz0 = hs.src[4], sz1 = hs.src[5];
+  const int ax0 = sx0 > 0 ? sx0 : c, ax1 = sx1 > 0 ? sx1 : c, ay0 = sy0 > 0 ? sy0 : c,
+            ay1 = sy1 > 0 ? sy1 : c, az0 = sz0 > 0 ? sz0 : c, az1 = sz1 > 0 ? sz1 : c;
+  T x0[2][5], x1[2][5], y0[2][5], y1[2][5], z0[2][5], z1[2][5];
 #pragma unroll
-  for (int f = 0; f < 6; ++f) {
-    const int sc = hs.src[f];
-    if (sc <= 0) continue;
-    const int ax = f >> 1;
-    const bool plus = f & 1;
+  for (int l = 0; l < 2; ++l)
 #pragma unroll
-    for (int l = 0; l < 2; ++l) {
-      const int dl = plus ? 18 + l : l;      // destination layer
-      const int sl = plus ? 2 + l : 16 + l;  // source layer
-      const int di = ax == 0 ? dl : p, dj = ax == 1 ? dl : (ax == 0 ? p : r),
-                dk = ax == 2 ? dl : r;
-      const int si = ax == 0 ? sl : p, sj = ax == 1 ? sl : (ax == 0 ? p : r),
-                sk = ax == 2 ? sl : r;
-#pragma unroll
-      for (int v = 0; v < 5; ++v) U[sidx(c, v, di, dj, dk)] = U[sidx(sc, v, si, sj, sk)];
+    for (int v = 0; v < 5; ++v) {
+      x0[l][v] = U[sidx(ax0, v, 16 + l, p, r)];
+      x1[l][v] = U[sidx(ax1, v, 2 + l, p, r)];
+      y0[l][v] = U[sidx(ay0, v, p, 16 + l, r)];
+      y1[l][v] = U[sidx(ay1, v, p, 2 + l, r)];
     }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const T2 a = *reinterpret_cast<const T2*>(U + sidx(az0, v, p, r, 16));
+    const T2 b = *reinterpret_cast<const T2*>(U + sidx(az1, v, p, r, 2));
+    z0[0][v] = a.x;
+    z0[1][v] = a.y;
+    z1[0][v] = b.x;
+    z1[1][v] = b.y;
+  }
+#pragma unroll
+  for (int l = 0; l < 2; ++l)
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      if (sx0 > 0) U[sidx(c, v, l, p, r)] = x0[l][v];
+      if (sx1 > 0) U[sidx(c, v, 18 + l, p, r)] = x1[l][v];
+      if (sy0 > 0) U[sidx(c, v, p, l, r)] = y0[l][v];
+      if (sy1 > 0) U[sidx(c, v, p, 18 + l, r)] = y1[l][v];
+    }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    T2 a, b;
+    a.x = z0[0][v];
+    a.y = z0[1][v];
+    b.x = z1[0][v];
+    b.y = z1[1][v];
+    if (sz0 > 0) *reinterpret_cast<T2*>(U + sidx(c, v, p, r, 0)) = a;
+    if (sz1 > 0) *reinterpret_cast<T2*>(U + sidx(c, v, p, r, 18)) = b;
   }
   __syncthreads();
   edge_fill(U, c);

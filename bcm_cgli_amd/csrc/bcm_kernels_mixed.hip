@@ -192,6 +192,52 @@ __device__ __forceinline__ void plane_face(const float (*cp)[400], int base, int
   roe_f(L, R, d, e, rf, F, dP);
 }
 
+// Viscous flux through the face between cell P = (i+a0, j0, k0) and P + e_D
+// (Flux.cpp:1946-2272, forward-face formulas of P): one evaluation serves
+// both adjacent cells.  ring holds deviations of u, v, w, T of planes i-1..i+1
+// (slot = plane & 3, j,k in [1,18]).  L receives the x, y, z stress and the
+// energy flux.
+template <int D>
+__device__ __forceinline__ void visc_face(const float (*ring)[4][18 * 18], int i, int a0, int j0,
+                                          int k0, const RefF& rf, float inv, float iT0,
+                                          float muL, float sut, float (&L)[4]) {
+  constexpr int da = D == 0, dj = D == 1, dk = D == 2;
+  auto P = [&](int a, int jj, int kk, int m) -> float {
+    return ring[(i + a) & 3][m][(jj - 1) * 18 + (kk - 1)];
+  };
+  float me[4], fv[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    me[m] = P(a0, j0, k0, m);
+    fv[m] = P(a0 + da, j0 + dj, k0 + dk, m);
+  }
+  const float Tf = rf.T + 0.5f * (me[3] + fv[3]);
+  const float temp = Tf * iT0;
+  const float mu = muL * temp * fsqrt(temp) * sut * frcp(Tf + 110.f);
+  float g[3][4];
+#pragma unroll
+  for (int tt = 0; tt < 3; ++tt) {
+    if (tt == D) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m) g[tt][m] = (fv[m] - me[m]) * inv;
+    } else {
+      const int ta = tt == 0, tj = tt == 1, tk = tt == 2;
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+        g[tt][m] = (P(a0 + ta, j0 + tj, k0 + tk, m) + P(a0 + ta + da, j0 + tj + dj, k0 + tk + dk, m) -
+                    P(a0 - ta, j0 - tj, k0 - tk, m) - P(a0 - ta + da, j0 - tj + dj, k0 - tk + dk, m)) *
+                   (0.25f * inv);
+    }
+  }
+  const float uf = rf.u[0] + 0.5f * (me[0] + fv[0]), vf = rf.u[1] + 0.5f * (me[1] + fv[1]),
+              wf = rf.u[2] + 0.5f * (me[2] + fv[2]);
+  const float div = g[0][0] + g[1][1] + g[2][2];
+#pragma unroll
+  for (int m = 0; m < 3; ++m)
+    L[m] = (m == D) ? mu * (2 * g[D][D] - (2.0f / 3.0f) * div) : mu * (g[m][D] + g[D][m]);
+  L[3] = uf * L[0] + vf * L[1] + wf * L[2] + mu * HEATf * g[D][3];
+}
+
 // Border cells of a 20x20 plane (j or k in {0,1,18,19}): 144 of them.
 __device__ __forceinline__ int border20(int t) {
   const int edge[4] = {0, 1, 18, 19};
@@ -217,6 +263,8 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
   __shared__ float pr[4][4][PR];      // u, v, w, T of planes (ring of 4)
   __shared__ float fy[5][NF];
   __shared__ float fz[6][NF];         // flux + face pressure average
+  __shared__ float vy[4][NF];         // viscous fluxes of the y- and z-faces
+  __shared__ float vz[4][NF];
   __shared__ double red[(WG / 64) * 7];
 
   const int c = blockIdx.x + 1;
@@ -263,8 +311,9 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
     T = (sd.dP - rf.PoR * q[0]) * sd.ir * IRGf;
   };
   // Primitive plane p into ring slot p & 3: the own cell from the column
-  // window (own holds plane p), the 68 border cells from memory.
-  auto fill_prims = [&](int p, const float (&own)[5]) {
+  // window (own holds plane p), the 68 border cells (lanes t < 68) from bq,
+  // their conserved values loaded beforehand.
+  auto fill_prims = [&](int p, const float (&own)[5], const float (&bq)[5]) {
     float* dst = &pr[p & 3][0][0];
     float u, v, ww, T;
     prims(own, u, v, ww, T);
@@ -275,10 +324,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
     dst[3 * PR + o] = T;
     if (t < 68) {
       const int cell = border18(t);
-      float q[5];
-#pragma unroll
-      for (int vv = 0; vv < 5; ++vv) q[vv] = Uc[vv * PLANE + p * 400 + cell];
-      prims(q, u, v, ww, T);
+      prims(bq, u, v, ww, T);
       const int ob = (cell / 20 - 1) * 18 + (cell % 20 - 1);
       dst[ob] = u;
       dst[PR + ob] = v;
@@ -286,10 +332,35 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
       dst[3 * PR + ob] = T;
     }
   };
-  fill_prims(1, w[2]);
-  fill_prims(2, w[3]);
+  // Border values are loaded one iteration ahead of their use: bcp/bfl the
+  // 144 border cells of the next plane to stage, bpr the 68 primitive-ring
+  // border cells of the plane after it.
+  auto load_b18 = [&](int p, float (&q)[5]) {
+    if (t < 68 && p <= 19) {
+      const int cell = border18(t);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) q[v] = Uc[v * PLANE + p * 400 + cell];
+    }
+  };
+  auto load_b20 = [&](int p, float (&q)[5], int& f) {
+    if (t < 144 && p <= 17) {
+      const int cell = border20(t);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) q[v] = Uc[v * PLANE + p * 400 + cell];
+      f = Fc[p * 400 + cell];
+    }
+  };
+  float bcp[5] = {0, 0, 0, 0, 0}, bpr[5] = {1, 0, 0, 0, 0};
+  int bfl = 0;
+  load_b18(1, bpr);
+  fill_prims(1, w[2], bpr);
+  load_b18(2, bpr);
+  fill_prims(2, w[3], bpr);
+  load_b20(2, bcp, bfl);
+  load_b18(3, bpr);
 
   float cF[5] = {0, 0, 0, 0, 0}, cdP = 0.f;
+  float cL[4] = {0, 0, 0, 0};   // viscous x-face flux carried from the previous plane
   int corder = -1;
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
 
@@ -303,10 +374,21 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
     if (t < 144) {
       const int cell = border20(t);
 #pragma unroll
-      for (int v = 0; v < 5; ++v) cp[b][v][cell] = Uc[v * PLANE + i * 400 + cell];
-      fl[b][cell] = Fc[i * 400 + cell];
+      for (int v = 0; v < 5; ++v) cp[b][v][cell] = bcp[v];
+      fl[b][cell] = (int8_t)bfl;
     }
-    fill_prims(i + 1, w[4]);
+    fill_prims(i + 1, w[4], bpr);
+    // Loads for later: the next iteration's borders, this cell's time levels
+    // folded into the FP64 source constant Bv = 3 Uref - 4 U1 + U1q.
+    load_b20(i + 1, bcp, bfl);
+    load_b18(i + 2, bpr);
+    double Bv[5];
+    {
+      const size_t gb = gcol + (size_t)i * 400;
+#pragma unroll
+      for (int v = 0; v < 5; ++v)
+        Bv[v] = 3.0 * uref[v] - 4.0 * U1[gb + (size_t)v * PLANE] + U1q[gb + (size_t)v * PLANE];
+    }
     __syncthreads();
 
     // ---- y- and z-faces of plane i: 272 + 272 faces over 256 lanes ----
@@ -319,15 +401,21 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
       const int8_t* fb = fl[b] + base;
       const int o = p >= 2 ? order_fwd(p, fb[-s], fb[s], fb[2 * s])
                            : order_bwd(2, fb[-s], fb[0], fb[2 * s]);
-      float F[5], dP;
+      float F[5], dP, Lv[4];
       plane_face(cp[b], base, s, yface ? 1 : 2, o, e, rf, F, dP);
       if (yface) {
+        visc_face<1>(pr, i, 0, p, g % 16 + 2, rf, inv, iT0, muL, sut, Lv);
 #pragma unroll
         for (int v = 0; v < 5; ++v) fy[v][g] = F[v];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) vy[v][g] = Lv[v];
       } else {
+        visc_face<2>(pr, i, 0, g / 17 + 2, p, rf, inv, iT0, muL, sut, Lv);
 #pragma unroll
         for (int v = 0; v < 5; ++v) fz[v][g] = F[v];
         fz[5][g] = dP;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) vz[v][g] = Lv[v];
       }
     }
     __syncthreads();
@@ -395,56 +483,20 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
       }
     }
 
-    // ---- viscous (Flux.cpp:1722-2279) from the primitive ring ----
-    auto Q = [&](int a, int bb, int cq, int m) -> float {
-      return pr[(i + a) & 3][m][(j + bb - 1) * 18 + (k + cq - 1)];
-    };
-    // Ring values are deviations; gradients are their differences.
-    const float meu = Q(0, 0, 0, 0), mev = Q(0, 0, 0, 1), mew = Q(0, 0, 0, 2), meT = Q(0, 0, 0, 3);
-    float vF[5] = {0, 0, 0, 0, 0};
+    // ---- viscous (Flux.cpp:1722-2279): x-face carried, y/z faces shared ----
+    float vF[5];
+    {
+      float Lf[4];
+      visc_face<0>(pr, i, 0, j, k, rf, inv, iT0, muL, sut, Lf);
+      if (i == 2) visc_face<0>(pr, i, -1, j, k, rf, inv, iT0, muL, sut, cL);
+      const int fyf = (j - 1) * 16 + (k - 2), fzf = (j - 2) * 17 + (k - 1);
+      vF[0] = 0.f;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      float Lb[5], Lf[5];
+      for (int v = 1; v < 5; ++v)
+        vF[v] = ((Lf[v - 1] - cL[v - 1]) + (vy[v - 1][fyf] - vy[v - 1][fyf - 16]) +
+                 (vz[v - 1][fzf] - vz[v - 1][fzf - 1])) * inv;
 #pragma unroll
-      for (int side = -1; side <= 1; side += 2) {
-        const int o0 = d == 0 ? side : 0, o1 = d == 1 ? side : 0, o2 = d == 2 ? side : 0;
-        const float fu = Q(o0, o1, o2, 0), fv = Q(o0, o1, o2, 1), fwv = Q(o0, o1, o2, 2),
-                    fT = Q(o0, o1, o2, 3);
-        const float Tf = rf.T + 0.5f * (meT + fT);
-        const float temp = Tf * iT0;
-        const float mu = muL * temp * fsqrt(temp) * sut * frcp(Tf + 110.f);
-        float g[3][4];
-#pragma unroll
-        for (int tt = 0; tt < 3; ++tt) {
-          if (tt == d) {
-            const float sg = side < 0 ? inv : -inv;
-            g[tt][0] = (meu - fu) * sg;
-            g[tt][1] = (mev - fv) * sg;
-            g[tt][2] = (mew - fwv) * sg;
-            g[tt][3] = (meT - fT) * sg;
-          } else {
-            const int p0 = tt == 0, p1 = tt == 1, p2 = tt == 2;
-#pragma unroll
-            for (int m = 0; m < 4; ++m)
-              g[tt][m] = (Q(p0, p1, p2, m) + Q(p0 + o0, p1 + o1, p2 + o2, m) - Q(-p0, -p1, -p2, m) -
-                          Q(-p0 + o0, -p1 + o1, -p2 + o2, m)) * (0.25f * inv);
-          }
-        }
-        const float uf = rf.u[0] + 0.5f * (meu + fu), vf = rf.u[1] + 0.5f * (mev + fv),
-                    wf = rf.u[2] + 0.5f * (mew + fwv);
-        const float div = g[0][0] + g[1][1] + g[2][2];
-        float sg[3];
-#pragma unroll
-        for (int m = 0; m < 3; ++m)
-          sg[m] = (m == d) ? mu * (2 * g[d][d] - (2.0f / 3.0f) * div) : mu * (g[m][d] + g[d][m]);
-        float* L = side < 0 ? Lb : Lf;
-        L[1] = sg[0];
-        L[2] = sg[1];
-        L[3] = sg[2];
-        L[4] = uf * sg[0] + vf * sg[1] + wf * sg[2] + mu * HEATf * g[d][3];
-      }
-#pragma unroll
-      for (int v = 1; v < 5; ++v) vF[v] += (Lf[v] - Lb[v]) * inv;
+      for (int v = 0; v < 4; ++v) cL[v] = Lf[v];
     }
 
     // ---- residual with the FP64 dual-time source ----
@@ -452,11 +504,9 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
 #pragma unroll
     for (int v = 0; v < 5; ++v) u[v] = w[3][v];
     float Rk[5];
-    const size_t gb = gcol + (size_t)i * 400;
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-      const double Rp = -(3.0 * ((double)u[v] + uref[v]) - 4.0 * U1[gb + (size_t)v * PLANE] +
-                          U1q[gb + (size_t)v * PLANE]) * i2dT;
+      const double Rp = -(3.0 * (double)u[v] + Bv[v]) * i2dT;
       const float Rf = -(Rx[v] + Ry[v] + Rz[v]) * inv + (v == 0 ? 0.f : vF[v]);
       Rk[v] = (float)(Rp + (double)Rf);
     }
