@@ -57,12 +57,13 @@ void free_graph(Engine* e) {
 
 // Tables of the resident cases (rebuilt whenever a case comes or goes).
 void free_tables(Engine* e) {
-  void* ptrs[] = {e->bc_cubes, e->bc_masks, e->halo, e->halo_pairs, e->case_desc};
+  void* ptrs[] = {e->bc_cubes, e->bc_masks, e->halo, e->halo_pairs, e->case_desc, e->bc_mask_cube};
   for (void* p : ptrs)
     if (p) hipFree(p);
   ghost_free(e->ghost);
   e->bc_cubes = nullptr;
   e->bc_masks = nullptr;
+  e->bc_mask_cube = nullptr;
   e->halo = nullptr;
   e->halo_pairs = nullptr;
   e->case_desc = nullptr;
@@ -78,7 +79,8 @@ void free_case(Engine* e) {
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (e->Uf) hipFree(e->Uf);
-  e->Uf = nullptr;
+  if (e->Uf2) hipFree(e->Uf2);
+  e->Uf = e->Uf2 = nullptr;
   if (e->er_host) hipHostFree(e->er_host);
   e->U = e->U1 = e->U1q = e->U1p1 = e->U1p2 = e->Rku = e->Res = e->staging = nullptr;
   e->fws = nullptr;
@@ -304,8 +306,9 @@ int engine_alloc(Engine* e, int cap, int max_cases, std::string& err, int prec) 
   const size_t ncells = (size_t)cap * PLANE;
   const size_t bytes = ncells * 5 * sizeof(double);
   if (prec == 1) {
-    if (dev_alloc(&e->Uf, ncells * 5, err)) return 1;
+    if (dev_alloc(&e->Uf, ncells * 5, err) || dev_alloc(&e->Uf2, ncells * 5, err)) return 1;
     HIP_TRY(hipMemsetAsync(e->Uf, 0, ncells * 5 * sizeof(float), e->stream));
+    HIP_TRY(hipMemsetAsync(e->Uf2, 0, ncells * 5 * sizeof(float), e->stream));
     for (double** p : {&e->U1, &e->U1q, &e->staging}) {
       if (dev_alloc(p, ncells * 5, err)) return 1;
       HIP_TRY(hipMemsetAsync(*p, 0, bytes, e->stream));
@@ -476,6 +479,8 @@ int rebuild_tables(Engine* e, std::string& err) {
   gs.cube_level_off.push_back((int)gs.level_off.size());
   gs.level_off.push_back((int)gs.entries.size());
   gs.touched_off.push_back((int)gs.touched.size());
+  std::vector<uint8_t> cube_mask(e->cap, 0);
+  for (size_t a = 0; a < bc.size(); ++a) cube_mask[bc[a]] = bm[a];
   std::vector<HaloSrc> hs(top);
   for (auto& h : hs) memset(&h, 0, sizeof h);
   for (int s : order) {
@@ -496,7 +501,8 @@ int rebuild_tables(Engine* e, std::string& err) {
   e->n_halo_pairs = (int)pairs.size();
   if (dev_upload(&e->bc_cubes, bc, err) || dev_upload(&e->bc_masks, bm, err) ||
       ghost_upload(gs, e->ghost, err) || dev_upload(&e->halo, hs, err) ||
-      dev_upload(&e->halo_pairs, pairs, err) || dev_upload(&e->case_desc, desc, err))
+      dev_upload(&e->halo_pairs, pairs, err) || dev_upload(&e->case_desc, desc, err) ||
+      dev_upload(&e->bc_mask_cube, cube_mask, err))
     return 1;
   HIP_TRY(hipMemcpy(e->phys_dev, phys.data(), phys.size() * sizeof(DevPhys),
                     hipMemcpyHostToDevice));
@@ -733,8 +739,9 @@ bool engine_fused(const Engine* e) {
 
 void engine_flux_update(Engine* e, int RK, int er_slot) {
   if (e->prec == 1) {
-    launch_flux_mixed(e->stream, e->Uf, e->U1, e->U1q, e->fws, e->cube_size, e->ncube, e->pt(),
-                      e->deltaT, e->deltaTau, e->e, RK, e->partials);
+    launch_flux_mixed(e->stream, e->Uf, e->Uf2, e->halo, e->U1, e->U1q, e->fws, e->cube_size,
+                      e->ncube, e->pt(), e->deltaT, e->deltaTau, e->e, RK, e->partials);
+    std::swap(e->Uf, e->Uf2);
     if (RK == 3)
       launch_reduce(e->stream, e->partials, 1, e->case_desc, e->n_active, e->phys_dev,
                     er_row(e, er_slot));
@@ -829,6 +836,19 @@ void engine_shift(Engine* e) {
 }
 
 void engine_stage(Engine* e, int RK, int er_slot, bool full_pre_halo) {
+  if (e->prec == 1) {
+    // The previous flux kernel wrote the face halos; the edge passes of that
+    // exchange and this stage's BCs open the stage (the edge passes are
+    // idempotent when no face changed, as at the start of a run).
+    launch_edge_bc(e->stream, e->Uf, e->bc_mask_cube, e->ncube, e->pt());
+    engine_ghost(e);
+    if (full_pre_halo)
+      engine_interface(e);
+    else
+      engine_halo_refresh(e);
+    engine_flux_update(e, RK, er_slot);
+    return;
+  }
   engine_bc(e);
   engine_ghost(e);
   if (full_pre_halo)
@@ -857,9 +877,16 @@ int ensure_er(Engine* e, int slots, std::string& err) {
 namespace {
 
 void enqueue_step(Engine* e, int n_inner, bool full_first) {
+  float* start = e->Uf;
   for (int it = 0; it < n_inner; ++it)
     for (int RK = 1; RK <= 3; ++RK) engine_stage(e, RK, it, full_first && it == 0 && RK == 1);
   engine_shift(e);
+  if (e->prec == 1 && e->Uf != start) {
+    // An odd number of flux kernels: bring the iterate back to the copy the
+    // step started from, so that every replay of a captured step is alike.
+    launch_copy_f(e->stream, e->Uf, start, (size_t)e->ncube * CUBE_STRIDE);
+    std::swap(e->Uf, e->Uf2);
+  }
 }
 
 }  // namespace
@@ -908,6 +935,9 @@ int engine_run(Engine* e, int n_steps, int n_inner, double* er_hist, bool use_gr
                 e->er_host[it * row + c * 8 + 1 + m];
     }
   }
+  // The mixed path's last flux kernel wrote the face halos; the edge passes
+  // complete that exchange (as the reference's closing BCM_Interface does).
+  if (e->prec == 1 && n_steps > 0) launch_edge(e->stream, e->Uf, e->ncube, e->cube_case);
   HIP_TRY(hipGetLastError());
   return 0;
 }

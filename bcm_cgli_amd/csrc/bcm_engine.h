@@ -79,7 +79,11 @@ struct Engine {
   // Arithmetic of the pool: 0 plain FP64 (bitwise the reference), 1 mixed
   // (FP32 inner iterate and stage arithmetic, FP64 time levels, source and norms).
   int prec = 0;
-  float* Uf = nullptr;          // inner iterate of the mixed path
+  // Inner iterate of the mixed path: the current copy and the one the next
+  // flux kernel writes (the two are swapped after every flux kernel).
+  float* Uf = nullptr;
+  float* Uf2 = nullptr;
+  uint8_t* bc_mask_cube = nullptr;   // [cap] BC side bits per pool cube
 
   // Device arrays (SoA per cube, cap cubes each).  The mixed path keeps U1,
   // U1q, staging, flags and sizes; U, U1p1, U1p2, Rku and Res stay null.

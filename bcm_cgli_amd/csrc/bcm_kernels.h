@@ -98,6 +98,14 @@ void launch_halo_sparse(hipStream_t s, double* U, const int2* pairs, int npairs,
                         const int* cube_case);
 void launch_halo_sparse(hipStream_t s, float* U, const int2* pairs, int npairs, int ncube,
                         const int* cube_case);
+// Edge passes of every cube, then the domain BCs of boundary cubes
+// (cube_mask[c]: the k_bc side bits of pool cube c, 0 for none).  Used after a
+// flux kernel that wrote the face halos itself.
+void launch_edge_bc(hipStream_t s, double* U, const uint8_t* cube_mask, int ncube, PhysTab pt);
+void launch_edge_bc(hipStream_t s, float* U, const uint8_t* cube_mask, int ncube, PhysTab pt);
+// Edge passes of every cube.
+void launch_edge(hipStream_t s, double* U, int ncube, const int* cube_case);
+void launch_edge(hipStream_t s, float* U, int ncube, const int* cube_case);
 void launch_flux(hipStream_t s, const double* U, const double* U1, const double* U1q,
                  const int8_t* fws, const double* cube_size, double* Rku, double* Res,
                  int ncube, PhysTab pt, double deltaT, double deltaTau, double e);
@@ -128,10 +136,17 @@ int update_blocks(int ncube);
 // Inner iterate U in FP32 SoA as the deviation from ref_state of the cube's
 // case; U1, U1q (physical time levels) in FP64 as full states.
 // One workgroup per cube: flux, residual, preconditioned increment and the RK
-// update of stage RK, in place; at RK 3 one row of 8 norm partials per cube.
-void launch_flux_mixed(hipStream_t s, float* U, const double* U1, const double* U1q,
-                       const int8_t* fws, const double* cube_size, int ncube, PhysTab pt,
-                       double deltaT, double deltaTau, double e, int RK, double* partials);
+// update of stage RK, reading Uin and writing every interior cell to Uout
+// (updated fluid cells, other cells unchanged).  Cells of the two layers
+// next to a face with an equal-size neighbour (halo[c]) are also written into
+// that neighbour's ghost layers in Uout: the face part of the post-update
+// exchange.  At RK 3 one row of 8 norm partials per cube.
+void launch_flux_mixed(hipStream_t s, const float* Uin, float* Uout, const HaloSrc* halo,
+                       const double* U1, const double* U1q, const int8_t* fws,
+                       const double* cube_size, int ncube, PhysTab pt, double deltaT,
+                       double deltaTau, double e, int RK, double* partials);
+// dst[0..n) <- src[0..n).
+void launch_copy_f(hipStream_t s, const float* src, float* dst, size_t n);
 // U1q <- U1, U1 <- ref_state + U on interior cells.
 void launch_shift_mixed(hipStream_t s, const float* U, double* U1, double* U1q, int ncube,
                         PhysTab pt);

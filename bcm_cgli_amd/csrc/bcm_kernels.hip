@@ -948,12 +948,9 @@ __global__ __launch_bounds__(WG) void k_reduce(const double* __restrict__ partia
 // (BCM_Z_boundary_condition.cpp:42-92).  mask bits: 1 x-min, 2 x-max, 4 y-min,
 // 8 y-max, 16 z-min, 32 z-max.
 template <typename T>
-__global__ __launch_bounds__(WG) void k_bc(T* __restrict__ U, const int* __restrict__ cubes,
-                                           const uint8_t* __restrict__ masks, PhysTab pt) {
-  const int c = cubes[blockIdx.x];
+__device__ __forceinline__ void bc_cube(T* __restrict__ U, int c, int m, PhysTab pt) {
   const DevPhys ph = pt.phys[pt.cube_case[c]];
   const typename OffOf<T>::type F(ph);
-  const int m = masks[blockIdx.x];
   const int t = threadIdx.x;
   {
     const int j = (t >> 4) + 2, k = (t & 15) + 2;
@@ -1019,6 +1016,12 @@ __global__ __launch_bounds__(WG) void k_bc(T* __restrict__ U, const int* __restr
         U[sidx(c, v, i, j, 19)] = x;
       }
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(WG) void k_bc(T* __restrict__ U, const int* __restrict__ cubes,
+                                           const uint8_t* __restrict__ masks, PhysTab pt) {
+  bc_cube(U, cubes[blockIdx.x], masks[blockIdx.x], pt);
 }
 
 // ---------------------------------------------------------------------------
@@ -1314,6 +1317,23 @@ __global__ __launch_bounds__(WG) void k_edge(T* __restrict__ U,
   edge_fill(U, blockIdx.x + 1);
 }
 
+// Stage start after a flux kernel that wrote its face halos into the
+// neighbours (flux_push): the edge passes of the post-update exchange
+// (Interface.cpp:5640-5732), then the domain BCs of the next stage, per cube.
+template <typename T>
+__global__ __launch_bounds__(WG) void k_edge_bc(T* __restrict__ U,
+                                                const uint8_t* __restrict__ cube_mask,
+                                                PhysTab pt) {
+  const int c = blockIdx.x + 1;
+  if (pt.cube_case[c] < 0) return;
+  edge_fill(U, c);
+  const int m = cube_mask[c];
+  if (m) {
+    __syncthreads();
+    bc_cube(U, c, m, pt);
+  }
+}
+
 // Physical-time shift of interior cells (main.cpp:1292-1315).
 __global__ __launch_bounds__(WG) void k_shift(const double* __restrict__ U,
                                               double* __restrict__ U1,
@@ -1381,6 +1401,27 @@ void launch_ghost(hipStream_t s, double* U, const GhostTables& g, PhysTab pt) {
 }
 void launch_ghost(hipStream_t s, float* U, const GhostTables& g, PhysTab pt) {
   launch_ghost_t(s, U, g, pt);
+}
+
+template <typename T>
+void launch_edge_bc_t(hipStream_t s, T* U, const uint8_t* cube_mask, int ncube, PhysTab pt) {
+  if (ncube > 1) hipLaunchKernelGGL(k_edge_bc<T>, dim3(ncube - 1), dim3(WG), 0, s, U, cube_mask, pt);
+}
+void launch_edge_bc(hipStream_t s, double* U, const uint8_t* cube_mask, int ncube, PhysTab pt) {
+  launch_edge_bc_t(s, U, cube_mask, ncube, pt);
+}
+void launch_edge_bc(hipStream_t s, float* U, const uint8_t* cube_mask, int ncube, PhysTab pt) {
+  launch_edge_bc_t(s, U, cube_mask, ncube, pt);
+}
+template <typename T>
+void launch_edge_t(hipStream_t s, T* U, int ncube, const int* cube_case) {
+  if (ncube > 1) hipLaunchKernelGGL(k_edge<T>, dim3(ncube - 1), dim3(WG), 0, s, U, cube_case);
+}
+void launch_edge(hipStream_t s, double* U, int ncube, const int* cube_case) {
+  launch_edge_t(s, U, ncube, cube_case);
+}
+void launch_edge(hipStream_t s, float* U, int ncube, const int* cube_case) {
+  launch_edge_t(s, U, ncube, cube_case);
 }
 
 template <typename T>

@@ -19,10 +19,12 @@
 // along i.  Each lane keeps its own i-column of conserved values (planes
 // i-3..i+3) and flags (i-2..i+2) in registers for the x-stencils; the plane
 // being processed is staged in LDS for the y- and z-stencils; primitives of
-// three planes sit in an LDS ring for the viscous terms.  After a plane has
-// been staged nothing reads its interior cells from memory again, so each
-// cell's RK update is written in place as soon as it is computed: no stage
-// increment array, no RK scratch arrays.  The reference's RK2/RK3 read U1p1
+// three planes sit in an LDS ring for the viscous terms.  The kernel reads
+// one copy of the iterate and writes the other, so each cell's RK update is
+// stored as soon as it is computed: no stage increment array, no RK scratch
+// arrays; the two layers next to a face with a neighbour are written into
+// the neighbour's ghost layers as well, which replaces the face part of the
+// post-update halo exchange.  The reference's RK2/RK3 read U1p1
 // and U1p2, which equal U1_ at every fluid cell when those stages run (only
 // ghost and non-fluid cells change in between), so the stages use U1_.
 #include <hip/hip_runtime.h>
@@ -255,7 +257,8 @@ __device__ __forceinline__ int border18(int t) {
 }
 
 __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
-    float* __restrict__ U, const double* __restrict__ U1, const double* __restrict__ U1q,
+    const float* __restrict__ U, float* __restrict__ Uout, const HaloSrc* __restrict__ halo,
+    const double* __restrict__ U1, const double* __restrict__ U1q,
     const int8_t* __restrict__ fws, const double* __restrict__ cube_size, PhysTab pt,
     double deltaT, double deltaTau, double e_d, int RK, double* __restrict__ partials) {
   __shared__ float cp[2][5][400];     // conserved values of plane i (double-buffered)
@@ -284,7 +287,9 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
   const float sut = (float)(ph.T0 + 110.);
   const double i2dT = 1.0 / (2 * deltaT);
   const float dT = (float)deltaT, dtau = (float)deltaTau;
-  float* __restrict__ Uc = U + (size_t)c * CUBE_STRIDE;
+  const float* __restrict__ Uc = U + (size_t)c * CUBE_STRIDE;
+  float* __restrict__ Oc = Uout + (size_t)c * CUBE_STRIDE;
+  const HaloSrc hs = halo[c];
   const int8_t* __restrict__ Fc = fws + (size_t)c * PLANE;
   const size_t gcol = (size_t)c * CUBE_STRIDE + col;
 
@@ -512,6 +517,9 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
     }
 
     // ---- preconditioned increment (Flux.cpp:2295-2375) and RK update ----
+    float nv[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) nv[v] = u[v];
     if (IF == 1) {
       Side me;
       side(w[3], rf, me);
@@ -561,7 +569,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
           un[v] = (1.0f / 3.0f) * u[v] + (2.0f / 3.0f) * u[v] + (2.0f / 3.0f) * out[v];
       }
 #pragma unroll
-      for (int v = 0; v < 5; ++v) Uc[v * PLANE + i * 400 + col] = un[v];
+      for (int v = 0; v < 5; ++v) nv[v] = un[v];
       if (RK == 3) {
         // Change norms in FP64 (Flux.cpp:2530-2566).
         double fo[5], fn[5];
@@ -587,6 +595,24 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
       // Force density of non-fluid cells (Flux.cpp:2395-2398), summed at RK 3.
       acc[5] += (double)(-(-(dPxf - dPxb) * inv + vF[1]) * vol);
       acc[6] += (double)(-(-(dPzf - dPzb) * inv + vF[3]) * vol);
+    }
+    // Every interior cell goes to Uout; cells next to a face with a
+    // neighbour also go to that neighbour's ghost layers.
+    {
+      const int o = i * 400 + col;
+      const int nx = i <= 3 ? hs.src[0] : (i >= 16 ? hs.src[1] : 0);
+      const int ny = j <= 3 ? hs.src[2] : (j >= 16 ? hs.src[3] : 0);
+      const int nz = k <= 3 ? hs.src[4] : (k >= 16 ? hs.src[5] : 0);
+      const int ox = o + (i <= 3 ? 16 * 400 : -16 * 400);
+      const int oy = o + (j <= 3 ? 16 * 20 : -16 * 20);
+      const int oz = o + (k <= 3 ? 16 : -16);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        Oc[v * PLANE + o] = nv[v];
+        if (nx > 0) Uout[(size_t)nx * CUBE_STRIDE + v * PLANE + ox] = nv[v];
+        if (ny > 0) Uout[(size_t)ny * CUBE_STRIDE + v * PLANE + oy] = nv[v];
+        if (nz > 0) Uout[(size_t)nz * CUBE_STRIDE + v * PLANE + oz] = nv[v];
+      }
     }
 
     // ---- advance the column window ----
@@ -646,6 +672,12 @@ struct Ref5 {
   double r[5];
 };
 
+__global__ __launch_bounds__(WG) void k_copy_f(const float* __restrict__ src,
+                                               float* __restrict__ dst, size_t n) {
+  for (size_t g = (size_t)blockIdx.x * WG + threadIdx.x; g < n; g += (size_t)gridDim.x * WG)
+    dst[g] = src[g];
+}
+
 __global__ __launch_bounds__(WG) void k_aos_to_soa_f(const double* __restrict__ aos,
                                                      float* __restrict__ soa, int ncells,
                                                      Ref5 ref) {
@@ -672,12 +704,20 @@ __global__ __launch_bounds__(WG) void k_soa_to_aos_f(const float* __restrict__ s
 
 }  // namespace
 
-void launch_flux_mixed(hipStream_t s, float* U, const double* U1, const double* U1q,
-                       const int8_t* fws, const double* cube_size, int ncube, PhysTab pt,
-                       double deltaT, double deltaTau, double e, int RK, double* partials) {
+void launch_flux_mixed(hipStream_t s, const float* Uin, float* Uout, const HaloSrc* halo,
+                       const double* U1, const double* U1q, const int8_t* fws,
+                       const double* cube_size, int ncube, PhysTab pt, double deltaT,
+                       double deltaTau, double e, int RK, double* partials) {
   if (ncube > 1)
-    hipLaunchKernelGGL(k_flux_mixed, dim3(ncube - 1), dim3(WG), 0, s, U, U1, U1q, fws, cube_size,
-                       pt, deltaT, deltaTau, e, RK, partials);
+    hipLaunchKernelGGL(k_flux_mixed, dim3(ncube - 1), dim3(WG), 0, s, Uin, Uout, halo, U1, U1q,
+                       fws, cube_size, pt, deltaT, deltaTau, e, RK, partials);
+}
+
+void launch_copy_f(hipStream_t s, const float* src, float* dst, size_t n) {
+  if (n == 0) return;
+  const size_t blocks = (n + WG - 1) / WG;
+  hipLaunchKernelGGL(k_copy_f, dim3(blocks < 65536 ? (unsigned)blocks : 65536u), dim3(WG), 0, s,
+                     src, dst, n);
 }
 
 void launch_shift_mixed(hipStream_t s, const float* U, double* U1, double* U1q, int ncube,
