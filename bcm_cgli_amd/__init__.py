@@ -272,13 +272,15 @@ class Engine:
         _check(_lib.bcm_engine_time_part(self._h, PARTS[name], int(RK), int(reps), ctypes.byref(ms)))
         return ms.value
 
-    def set_flux(self, variant, nseg=1, fuse=True):
-        _check(_lib.bcm_engine_set_flux(self._h, int(variant), int(nseg), int(bool(fuse))))
+    def set_flux(self, variant, nseg=1, fuse=1):
+        """fuse: 0 separate update, 1 update fused in place, 2 update and face
+        halos written to a second copy of the iterate (engine stages)."""
+        _check(_lib.bcm_engine_set_flux(self._h, int(variant), int(nseg), int(fuse)))
 
     def autotune(self):
         out = (ctypes.c_int * 3)()
         _check(_lib.bcm_engine_autotune(self._h, out))
-        return {"variant": out[0], "nseg": out[1], "fuse": bool(out[2])}
+        return {"variant": out[0], "nseg": out[1], "fuse": int(out[2])}
 
     def ghost_stats(self):
         s = (ctypes.c_int * 3)()

@@ -474,10 +474,13 @@ int bcm_engine_time_part(bcm_engine* h, int part, int RK, int reps, double* ms) 
 int bcm_engine_set_flux(bcm_engine* h, int variant, int nseg, int fuse) {
   if (!h) return fail("null engine");
   if (h->e->prec == 1) return fail("the mixed path has a single flux kernel");
-  h->e->flux_fuse = fuse != 0;
   if (variant != 0 && variant != 1) return fail("flux variant must be 0 or 1");
   if (nseg != 1 && nseg != 2 && nseg != 4 && nseg != 8 && nseg != 16)
     return fail("nseg must divide 16");
+  if (fuse < 0 || fuse > 2) return fail("fuse must be 0, 1 or 2");
+  if (fuse == 2 && (variant != 1 || nseg != 1)) return fail("fuse 2 needs variant 1, nseg 1");
+  h->e->flux_fuse = fuse != 0;
+  h->e->flux_push = fuse == 2;
   h->e->flux_variant = variant;
   h->e->flux_nseg = nseg;
   h->e->flux_tuned = true;
@@ -493,7 +496,7 @@ int bcm_engine_autotune(bcm_engine* h, int* out3) {
   if (out3) {
     out3[0] = v;
     out3[1] = n;
-    out3[2] = h->e->flux_fuse ? 1 : 0;
+    out3[2] = h->e->flux_push ? 2 : (h->e->flux_fuse ? 1 : 0);
   }
   return 0;
 }

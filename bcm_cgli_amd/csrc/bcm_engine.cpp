@@ -75,14 +75,14 @@ void free_case(Engine* e) {
   free_tables(e);
   void* ptrs[] = {e->U,   e->U1,        e->U1q,      e->U1p1,     e->U1p2,   e->Rku,
                   e->Res, e->staging,   e->fws,      e->cube_size, e->partials, e->er_dev,
-                  e->phys_dev, e->cube_case};
+                  e->phys_dev, e->cube_case, e->U2};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (e->Uf) hipFree(e->Uf);
   if (e->Uf2) hipFree(e->Uf2);
   e->Uf = e->Uf2 = nullptr;
   if (e->er_host) hipHostFree(e->er_host);
-  e->U = e->U1 = e->U1q = e->U1p1 = e->U1p2 = e->Rku = e->Res = e->staging = nullptr;
+  e->U = e->U1 = e->U1q = e->U1p1 = e->U1p2 = e->Rku = e->Res = e->staging = e->U2 = nullptr;
   e->fws = nullptr;
   e->cube_size = nullptr;
   e->partials = e->er_dev = e->er_host = nullptr;
@@ -244,8 +244,8 @@ void ghost_free(GhostDevice& d) {
 }
 
 size_t engine_bytes_per_cube(int prec) {
-  const size_t state = prec == 1 ? (size_t)CUBE_STRIDE * (sizeof(float) + 3 * sizeof(double))
-                                 : (size_t)8 * CUBE_STRIDE * sizeof(double);
+  const size_t state = prec == 1 ? (size_t)CUBE_STRIDE * (2 * sizeof(float) + 3 * sizeof(double))
+                                 : (size_t)9 * CUBE_STRIDE * sizeof(double);
   return state + PLANE + sizeof(double) + sizeof(int) + (size_t)16 * 8 * sizeof(double);
 }
 
@@ -314,7 +314,8 @@ int engine_alloc(Engine* e, int cap, int max_cases, std::string& err, int prec) 
       HIP_TRY(hipMemsetAsync(*p, 0, bytes, e->stream));
     }
   } else {
-    for (double** p : {&e->U, &e->U1, &e->U1q, &e->U1p1, &e->U1p2, &e->Rku, &e->Res, &e->staging}) {
+    for (double** p : {&e->U, &e->U1, &e->U1q, &e->U1p1, &e->U1p2, &e->Rku, &e->Res, &e->staging,
+                       &e->U2}) {
       if (dev_alloc(p, ncells * 5, err)) return 1;
       HIP_TRY(hipMemsetAsync(*p, 0, bytes, e->stream));
     }
@@ -643,6 +644,12 @@ int engine_case_state(Engine* e, int slot, int which, double* aos, bool upload,
       launch_aos_to_soa_f(e->stream, e->staging, dstf, (int)n, ref);
     else
       launch_aos_to_soa(e->stream, e->staging, dst, (int)n);
+    // The second copy of the iterate starts equal (cells no kernel rewrites
+    // keep their values in both).
+    if (which == 0 && f32)
+      launch_copy_f(e->stream, dstf, e->Uf2 + (size_t)S.first * CUBE_STRIDE, n * CUBE_STRIDE);
+    if (which == 0 && !f32 && e->U2)
+      launch_copy_d(e->stream, dst, e->U2 + (size_t)S.first * CUBE_STRIDE, n * CUBE_STRIDE);
   } else {
     double ref[5];
     ref_state(S.tb.ph, ref);
@@ -670,6 +677,7 @@ int engine_upload_state(Engine* e, int which, const double* aos) {
   if (hipMemcpyAsync(e->staging, aos, pool_bytes(e), hipMemcpyHostToDevice, e->stream) != hipSuccess)
     return 1;
   launch_aos_to_soa(e->stream, e->staging, d, e->ncube);
+  if (which == 0 && e->U2) launch_copy_d(e->stream, d, e->U2, (size_t)e->ncube * CUBE_STRIDE);
   return hipStreamSynchronize(e->stream) == hipSuccess ? 0 : 1;
 }
 
@@ -739,8 +747,9 @@ bool engine_fused(const Engine* e) {
 
 void engine_flux_update(Engine* e, int RK, int er_slot) {
   if (e->prec == 1) {
-    launch_flux_mixed(e->stream, e->Uf, e->Uf2, e->halo, e->U1, e->U1q, e->fws, e->cube_size,
-                      e->ncube, e->pt(), e->deltaT, e->deltaTau, e->e, RK, e->partials);
+    launch_flux_mixed(e->stream, e->Uf, e->Uf2, e->halo, e->bc_mask_cube, e->U1, e->U1q, e->fws,
+                      e->cube_size, e->ncube, e->pt(), e->deltaT, e->deltaTau, e->e, RK,
+                      e->partials);
     std::swap(e->Uf, e->Uf2);
     if (RK == 3)
       launch_reduce(e->stream, e->partials, 1, e->case_desc, e->n_active, e->phys_dev,
@@ -774,42 +783,50 @@ int engine_autotune(Engine* e, int* variant, int* nseg, std::string& err) {
     return 0;
   }
   // Candidates (variant, nseg, fuse): the plane kernel, the marching kernel
-  // with 1..8 segments, and the marching kernel with the RK update fused.
-  // Each is timed as flux + update of one RK stage on the resident case; the
-  // arrays the update writes are saved first and restored afterwards.
-  const int cand[6][3] = {{0, 1, 0}, {1, 1, 1}, {1, 1, 0}, {1, 2, 0}, {1, 4, 0}, {1, 8, 0}};
-  double* saved[3] = {nullptr, nullptr, nullptr};
-  double* live[3] = {e->U, e->U1p1, e->U1p2};
-  for (int a = 0; a < 3; ++a) {
+  // with 1..8 segments, the marching kernel with the RK update fused in place,
+  // and the marching kernel writing the update and face halos into the other
+  // copy of the iterate (fuse 2).  Each is timed as one whole RK stage on the
+  // resident cases; every array a stage writes is saved first and restored
+  // afterwards.
+  const int cand[7][3] = {{0, 1, 0}, {1, 1, 1}, {1, 1, 2}, {1, 1, 0},
+                          {1, 2, 0}, {1, 4, 0}, {1, 8, 0}};
+  double* const live[6] = {e->U, e->U2, e->U1p1, e->U1p2, e->Rku, e->Res};
+  double* saved[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  for (int a = 0; a < 6; ++a) {
     HIP_TRY(hipMalloc((void**)&saved[a], pool_bytes(e)));
     HIP_TRY(hipMemcpyAsync(saved[a], live[a], pool_bytes(e), hipMemcpyDeviceToDevice, e->stream));
   }
+  const bool full = e->halo_full_next;
   float best = 1e30f;
   int bv = e->flux_variant, bs = e->flux_nseg;
-  bool bf = e->flux_fuse;
+  bool bf = e->flux_fuse, bp = e->flux_push;
   for (auto& cv : cand) {
     e->flux_variant = cv[0];
     e->flux_nseg = cv[1];
     e->flux_fuse = cv[2] != 0;
-    engine_flux_update(e, 2, 0);  // warm up
+    e->flux_push = cv[2] == 2;
+    engine_stage(e, 2, 0, full);  // warm up
     HIP_TRY(hipEventRecord(e->ev0, e->stream));
-    for (int r = 0; r < 3; ++r) engine_flux_update(e, 2, 0);
+    for (int r = 0; r < 3; ++r) engine_stage(e, 2, 0, full);
     HIP_TRY(hipEventRecord(e->ev1, e->stream));
     HIP_TRY(hipEventSynchronize(e->ev1));
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    e->U = live[0];
+    e->U2 = live[1];
+    for (int a = 0; a < 6; ++a)
+      HIP_TRY(hipMemcpyAsync(live[a], saved[a], pool_bytes(e), hipMemcpyDeviceToDevice, e->stream));
     if (ms < best) {
       best = ms;
       bv = cv[0];
       bs = cv[1];
       bf = cv[2] != 0;
+      bp = cv[2] == 2;
     }
   }
-  for (int a = 0; a < 3; ++a) {
-    HIP_TRY(hipMemcpyAsync(live[a], saved[a], pool_bytes(e), hipMemcpyDeviceToDevice, e->stream));
-  }
   HIP_TRY(hipStreamSynchronize(e->stream));
-  for (int a = 0; a < 3; ++a) hipFree(saved[a]);
+  for (int a = 0; a < 6; ++a) hipFree(saved[a]);
+  e->flux_push = bp;
   e->flux_variant = bv;
   e->flux_nseg = bs;
   e->flux_fuse = bf;
@@ -849,6 +866,22 @@ void engine_stage(Engine* e, int RK, int er_slot, bool full_pre_halo) {
     engine_flux_update(e, RK, er_slot);
     return;
   }
+  if (e->flux_push) {
+    launch_edge_bc(e->stream, e->U, e->bc_mask_cube, e->ncube, e->pt());
+    engine_ghost(e);
+    if (full_pre_halo)
+      engine_interface(e);
+    else
+      engine_halo_refresh(e);
+    launch_flux_push(e->stream, e->U, e->U2, e->halo, e->bc_mask_cube, e->U1, e->U1q, e->fws,
+                     e->cube_size, e->Rku, e->Res, e->ncube, e->pt(), e->deltaT, e->deltaTau,
+                     e->e, RK, e->U1p1, e->U1p2, e->partials);
+    std::swap(e->U, e->U2);
+    if (RK == 3)
+      launch_reduce(e->stream, e->partials, 1, e->case_desc, e->n_active, e->phys_dev,
+                    er_row(e, er_slot));
+    return;
+  }
   engine_bc(e);
   engine_ghost(e);
   if (full_pre_halo)
@@ -877,15 +910,21 @@ int ensure_er(Engine* e, int slots, std::string& err) {
 namespace {
 
 void enqueue_step(Engine* e, int n_inner, bool full_first) {
-  float* start = e->Uf;
+  float* startf = e->Uf;
+  double* startd = e->U;
   for (int it = 0; it < n_inner; ++it)
     for (int RK = 1; RK <= 3; ++RK) engine_stage(e, RK, it, full_first && it == 0 && RK == 1);
   engine_shift(e);
-  if (e->prec == 1 && e->Uf != start) {
-    // An odd number of flux kernels: bring the iterate back to the copy the
-    // step started from, so that every replay of a captured step is alike.
-    launch_copy_f(e->stream, e->Uf, start, (size_t)e->ncube * CUBE_STRIDE);
+  // An odd number of flux kernels that swap copies: bring the iterate back to
+  // the copy the step started from, so that every replay of a captured step
+  // is alike.
+  if (e->prec == 1 && e->Uf != startf) {
+    launch_copy_f(e->stream, e->Uf, startf, (size_t)e->ncube * CUBE_STRIDE);
     std::swap(e->Uf, e->Uf2);
+  }
+  if (e->prec == 0 && e->U != startd) {
+    launch_copy_d(e->stream, e->U, startd, (size_t)e->ncube * CUBE_STRIDE);
+    std::swap(e->U, e->U2);
   }
 }
 
@@ -938,6 +977,8 @@ int engine_run(Engine* e, int n_steps, int n_inner, double* er_hist, bool use_gr
   // The mixed path's last flux kernel wrote the face halos; the edge passes
   // complete that exchange (as the reference's closing BCM_Interface does).
   if (e->prec == 1 && n_steps > 0) launch_edge(e->stream, e->Uf, e->ncube, e->cube_case);
+  if (e->prec == 0 && e->flux_push && n_steps > 0)
+    launch_edge(e->stream, e->U, e->ncube, e->cube_case);
   HIP_TRY(hipGetLastError());
   return 0;
 }

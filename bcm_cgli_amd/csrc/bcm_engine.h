@@ -89,6 +89,7 @@ struct Engine {
   // U1q, staging, flags and sizes; U, U1p1, U1p2, Rku and Res stay null.
   double *U = nullptr, *U1 = nullptr, *U1q = nullptr, *U1p1 = nullptr, *U1p2 = nullptr;
   double *Rku = nullptr, *Res = nullptr, *staging = nullptr;
+  double* U2 = nullptr;         // second copy of the FP64 iterate (flux with halo push)
   int8_t* fws = nullptr;
   double* cube_size = nullptr;
   // Tables of all resident cases in pool numbering.
@@ -117,6 +118,10 @@ struct Engine {
   int flux_variant = 1;
   int flux_nseg = 1;
   bool flux_fuse = true;     // marching kernel with nseg 1 also applies the RK update
+  // Stages run the marching kernel that writes the update and the face halos
+  // into the other copy of the iterate (engine stages only; the stage pieces
+  // and drop-ins keep the in-place kernels).
+  bool flux_push = true;
   bool flux_tuned = false;
 
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -172,8 +177,9 @@ void engine_stage(Engine* e, int RK, int er_slot, bool full_pre_halo = false);
 int engine_run(Engine* e, int n_steps, int n_inner, double* er_hist, bool use_graph,
                std::string& err);
 int engine_sync(Engine* e, std::string& err);
-// Times every flux variant on the resident cases and keeps the fastest.
-// Returns the chosen (variant, nseg) through the pointers.
+// Times one RK stage for every flux choice on the resident cases (state
+// restored afterwards) and keeps the fastest.  Returns the chosen
+// (variant, nseg) through the pointers.
 int engine_autotune(Engine* e, int* variant, int* nseg, std::string& err);
 
 }  // namespace bcm

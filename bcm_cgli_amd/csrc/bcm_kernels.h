@@ -79,6 +79,33 @@ struct HaloSrc {
   int src[6];   // -X, +X, -Y, +Y, -Z, +Z
 };
 
+// Copies, with the threads of one workgroup, the ghost regions the domain BCs
+// of cube c write (mask bits 1 x-min, 2 x-max, 4 y-min, 8 y-max, 16 z-min,
+// 32 z-max) from Uin to Uout: x faces over j,k in [2,17], y faces over all i
+// and k in [2,17], z faces over all i,j.  The regions are disjoint and no
+// neighbour writes them.
+template <typename T>
+__device__ inline void copy_bc_regions(const T* __restrict__ Uin, T* __restrict__ Uout, int c,
+                                       int m) {
+  const size_t base = (size_t)c * CUBE_STRIDE;
+  auto cp = [&](int cell) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Uout[base + (size_t)v * PLANE + cell] = Uin[base + (size_t)v * PLANE + cell];
+  };
+  for (int side = 0; side < 2; ++side) {
+    const int off = side ? 18 : 0;
+    if (m & (1 << side))
+      for (int n = threadIdx.x; n < 512; n += blockDim.x)
+        cp((off + n / 256) * 400 + ((n % 256) / 16 + 2) * 20 + (n % 16 + 2));
+    if (m & (4 << side))
+      for (int n = threadIdx.x; n < 640; n += blockDim.x)
+        cp(((n % 320) / 16) * 400 + (off + n / 320) * 20 + (n % 16 + 2));
+    if (m & (16 << side))
+      for (int n = threadIdx.x; n < 800; n += blockDim.x)
+        cp(((n % 400) / 20) * 400 + ((n % 400) % 20) * 20 + off + n / 400);
+  }
+}
+
 // Launches cover pool cubes 1..ncube-1; "ncube" is one past the last cube in use.
 // The BC, IB fill and halo kernels exist for an FP64 inner iterate and for the
 // FP32 one of the mixed-precision path, which holds the deviation from the
@@ -117,6 +144,18 @@ void launch_flux_march(hipStream_t s, double* U, const double* U1, const double*
                        const int8_t* fws, const double* cube_size, double* Rku, double* Res,
                        int ncube, PhysTab pt, double deltaT, double deltaTau, double e, int nseg,
                        int RK, int fuse, double* U1p1, double* U1p2, double* partials);
+// Marching kernel with the update written to Uout (one workgroup per cube):
+// every interior cell, and the two layers next to each face with an
+// equal-size neighbour (halo[c]) into that neighbour's ghost layers in Uout.
+// Writes Rku, Res, U1p1 (RK1), U1p2 (RK2) and, at RK 3, one row of norm
+// partials per cube, bitwise as the in-place fused kernel does.
+// Ghost regions the domain BCs wrote (cube_mask) are copied from Uin to Uout.
+void launch_flux_push(hipStream_t s, const double* Uin, double* Uout, const HaloSrc* halo,
+                      const uint8_t* cube_mask, const double* U1, const double* U1q,
+                      const int8_t* fws, const double* cube_size, double* Rku, double* Res,
+                      int ncube, PhysTab pt, double deltaT, double deltaTau, double e, int RK,
+                      double* U1p1, double* U1p2, double* partials);
+void launch_copy_d(hipStream_t s, const double* src, double* dst, size_t n);
 void launch_update(hipStream_t s, int RK, double* U, double* U1p1, double* U1p2,
                    const double* Rku, const double* Res, const int8_t* fws, int ncube,
                    const int* cube_case, double* partials);
@@ -142,7 +181,8 @@ int update_blocks(int ncube);
 // that neighbour's ghost layers in Uout: the face part of the post-update
 // exchange.  At RK 3 one row of 8 norm partials per cube.
 void launch_flux_mixed(hipStream_t s, const float* Uin, float* Uout, const HaloSrc* halo,
-                       const double* U1, const double* U1q, const int8_t* fws,
+                       const uint8_t* cube_mask, const double* U1, const double* U1q,
+                       const int8_t* fws,
                        const double* cube_size, int ncube, PhysTab pt, double deltaT,
                        double deltaTau, double e, int RK, double* partials);
 // dst[0..n) <- src[0..n).

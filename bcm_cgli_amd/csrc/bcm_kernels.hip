@@ -475,12 +475,22 @@ constexpr int NFACE = 17 * 16;      // faces of one axis in one i-plane
 // neighbour's forward-face order evaluates its own backward face.  All
 // arithmetic is the reference's, in its order: results are bitwise those of
 // k_flux.
+//
+// PUSH: the update is not done in place but written to Uout, with every
+// interior cell (non-fluid cells unchanged) and, for the two layers next to a
+// face with an equal-size neighbour, into that neighbour's ghost layers: the
+// face part of the post-update exchange.  It uses U for U1p1 (RK2) and U1p2
+// (RK3): those equal U1_ bitwise at fluid cells during a stage sequence,
+// since only ghost and non-fluid cells change between stages; both are still
+// written as the reference writes them.
+template <bool PUSH>
 __global__ __launch_bounds__(WG, 2) void k_flux_march(
     double* __restrict__ U, const double* __restrict__ U1, const double* __restrict__ U1q,
     const int8_t* __restrict__ fws, const double* __restrict__ cube_size,
     double* __restrict__ Rku, double* __restrict__ Res, PhysTab pt, double deltaT,
     double deltaTau, double e, int nseg, int RK, int fuse, double* __restrict__ U1p1,
-    double* __restrict__ U1p2, double* __restrict__ partials) {
+    double* __restrict__ U1p2, double* __restrict__ partials, double* __restrict__ Uout,
+    const HaloSrc* __restrict__ halo, const uint8_t* __restrict__ cube_mask) {
   __shared__ double pr[4][3 * PR_PLANE];   // u, v, w, T = P/(rho R) of 3 planes
   __shared__ double fy[5][NFACE];
   __shared__ double fz[6][NFACE];          // flux + face pressure average
@@ -576,7 +586,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
     // Plane i-4 was last read during iteration i-1 (backward-face stencil of
     // plane i-1 reaches i-4); the barrier that ended that iteration makes its
     // update safe now.
-    if (fuse && i - 4 >= i0) update_plane(i - 4);
+    if (!PUSH && fuse && i - 4 >= i0) update_plane(i - 4);
     fill_prims(i + 1);
     // Plane i (conserved values and flags) into LDS for the y- and z-faces.
     for (int n = t; n < 400; n += WG) {
@@ -817,15 +827,76 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
       const double r1 = -(pz + vF[3]) * dx * dy * dz;
       Res[b] = r0;
       Res[b + PLANE] = r1;
-      if (fuse && RK == 3) {
+      if ((fuse || PUSH) && RK == 3) {
         acc[5] += r0;
         acc[6] += r1;
       }
     }
+    if constexpr (PUSH) {
+      // The RK update of update_plane with U1p1 = U1p2 = U (see above).
+      double nv[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) nv[v] = u[v];
+      if (IF == 1) {
+        if (RK == 1) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) {
+            const double x = u[v] + out[v];
+            nv[v] = x;
+            U1p1[b + (size_t)v * PLANE] = x;
+          }
+        } else if (RK == 2) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) {
+            const double x = 0.75 * u[v] + 0.25 * u[v] + 0.25 * out[v];
+            nv[v] = x;
+            U1p2[b + (size_t)v * PLANE] = x;
+          }
+        } else {
+          const double ro = u[0];
+          const double uo1 = u[1] / ro, vo1 = u[2] / ro, wo1 = u[3] / ro;
+          const double VVo = uo1 * uo1 + vo1 * vo1 + wo1 * wo1;
+          const double Po = (u[4] - 0.5 * ro * VVo) * KM1;
+          const double To = Po / ro;
+#pragma unroll
+          for (int v = 0; v < 5; ++v) nv[v] = 1.0 / 3.0 * u[v] + 2.0 / 3.0 * u[v] + 2.0 / 3.0 * out[v];
+          const double r = nv[0];
+          const double un = nv[1] / r, vn = nv[2] / r, wn = nv[3] / r;
+          const double VV = un * un + vn * vn + wn * wn;
+          const double P = (nv[4] - 0.5 * r * VV) * KM1;
+          const double T = P / r;
+          acc[0] += (P - Po) * (P - Po);
+          acc[1] += (un - uo1) * (un - uo1);
+          acc[2] += (vn - vo1) * (vn - vo1);
+          acc[3] += (wn - wo1) * (wn - wo1);
+          acc[4] += (T - To) * (T - To);
+        }
+      }
+      const HaloSrc hs = halo[c];
+      const int o = i * 400 + j * 20 + k;
+      const int nx = i <= 3 ? hs.src[0] : (i >= 16 ? hs.src[1] : 0);
+      const int ny = j <= 3 ? hs.src[2] : (j >= 16 ? hs.src[3] : 0);
+      const int nz = k <= 3 ? hs.src[4] : (k >= 16 ? hs.src[5] : 0);
+      const int ox = o + (i <= 3 ? 16 * 400 : -16 * 400);
+      const int oy = o + (j <= 3 ? 16 * 20 : -16 * 20);
+      const int oz = o + (k <= 3 ? 16 : -16);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        Uout[(size_t)c * CUBE_STRIDE + (size_t)v * PLANE + o] = nv[v];
+        if (nx > 0) Uout[(size_t)nx * CUBE_STRIDE + (size_t)v * PLANE + ox] = nv[v];
+        if (ny > 0) Uout[(size_t)ny * CUBE_STRIDE + (size_t)v * PLANE + oy] = nv[v];
+        if (nz > 0) Uout[(size_t)nz * CUBE_STRIDE + (size_t)v * PLANE + oz] = nv[v];
+      }
+    }
     __syncthreads();
   }
-  if (fuse) {
-    for (int ip = i1 - 3 < i0 ? i0 : i1 - 3; ip <= i1; ++ip) update_plane(ip);
+  if constexpr (PUSH) {
+    const int bm = cube_mask[c];
+    if (bm) copy_bc_regions(U, Uout, c, bm);
+  }
+  if (fuse || PUSH) {
+    if (!PUSH)
+      for (int ip = i1 - 3 < i0 ? i0 : i1 - 3; ip <= i1; ++ip) update_plane(ip);
     if (RK == 3) {
       __shared__ double sh[(WG / 64) * 7];
       block_sum<7>(acc, sh);
@@ -1468,9 +1539,33 @@ void launch_flux_march(hipStream_t s, double* U, const double* U1, const double*
                        int ncube, PhysTab pt, double deltaT, double deltaTau, double e, int nseg,
                        int RK, int fuse, double* U1p1, double* U1p2, double* partials) {
   if (ncube > 1)
-    hipLaunchKernelGGL(k_flux_march, dim3((ncube - 1) * nseg), dim3(WG), 0, s, U, U1, U1q, fws,
-                       cube_size, Rku, Res, pt, deltaT, deltaTau, e, nseg, RK, fuse && nseg == 1,
-                       U1p1, U1p2, partials);
+    hipLaunchKernelGGL(k_flux_march<false>, dim3((ncube - 1) * nseg), dim3(WG), 0, s, U, U1, U1q,
+                       fws, cube_size, Rku, Res, pt, deltaT, deltaTau, e, nseg, RK,
+                       fuse && nseg == 1, U1p1, U1p2, partials, nullptr, nullptr, nullptr);
+}
+
+void launch_flux_push(hipStream_t s, const double* Uin, double* Uout, const HaloSrc* halo,
+                      const uint8_t* cube_mask, const double* U1, const double* U1q,
+                      const int8_t* fws, const double* cube_size, double* Rku, double* Res,
+                      int ncube, PhysTab pt, double deltaT, double deltaTau, double e, int RK,
+                      double* U1p1, double* U1p2, double* partials) {
+  if (ncube > 1)
+    hipLaunchKernelGGL(k_flux_march<true>, dim3(ncube - 1), dim3(WG), 0, s,
+                       const_cast<double*>(Uin), U1, U1q, fws, cube_size, Rku, Res, pt, deltaT,
+                       deltaTau, e, 1, RK, 0, U1p1, U1p2, partials, Uout, halo, cube_mask);
+}
+
+__global__ __launch_bounds__(WG) void k_copy_d(const double* __restrict__ src,
+                                               double* __restrict__ dst, size_t n) {
+  for (size_t g = (size_t)blockIdx.x * WG + threadIdx.x; g < n; g += (size_t)gridDim.x * WG)
+    dst[g] = src[g];
+}
+
+void launch_copy_d(hipStream_t s, const double* src, double* dst, size_t n) {
+  if (n == 0) return;
+  const size_t blocks = (n + WG - 1) / WG;
+  hipLaunchKernelGGL(k_copy_d, dim3(blocks < 65536 ? (unsigned)blocks : 65536u), dim3(WG), 0, s,
+                     src, dst, n);
 }
 
 void launch_update(hipStream_t s, int RK, double* U, double* U1p1, double* U1p2,

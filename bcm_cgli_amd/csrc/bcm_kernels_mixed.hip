@@ -258,7 +258,8 @@ __device__ __forceinline__ int border18(int t) {
 
 __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
     const float* __restrict__ U, float* __restrict__ Uout, const HaloSrc* __restrict__ halo,
-    const double* __restrict__ U1, const double* __restrict__ U1q,
+    const uint8_t* __restrict__ cube_mask, const double* __restrict__ U1,
+    const double* __restrict__ U1q,
     const int8_t* __restrict__ fws, const double* __restrict__ cube_size, PhysTab pt,
     double deltaT, double deltaTau, double e_d, int RK, double* __restrict__ partials) {
   __shared__ float cp[2][5][400];     // conserved values of plane i (double-buffered)
@@ -629,6 +630,12 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
     if (i + 3 <= 19) fw[4] = Fc[(i + 3) * 400 + col];
   }
 
+  // Ghost regions the domain BCs wrote this stage go to Uout as they are (no
+  // neighbour writes them), so Uout holds them as an in-place update would.
+  {
+    const int bm = cube_mask[c];
+    if (bm) copy_bc_regions(U, Uout, c, bm);
+  }
   if (RK == 3) {
 #pragma unroll
     for (int m = 0; m < 7; ++m)
@@ -705,12 +712,12 @@ __global__ __launch_bounds__(WG) void k_soa_to_aos_f(const float* __restrict__ s
 }  // namespace
 
 void launch_flux_mixed(hipStream_t s, const float* Uin, float* Uout, const HaloSrc* halo,
-                       const double* U1, const double* U1q, const int8_t* fws,
-                       const double* cube_size, int ncube, PhysTab pt, double deltaT,
-                       double deltaTau, double e, int RK, double* partials) {
+                       const uint8_t* cube_mask, const double* U1, const double* U1q,
+                       const int8_t* fws, const double* cube_size, int ncube, PhysTab pt,
+                       double deltaT, double deltaTau, double e, int RK, double* partials) {
   if (ncube > 1)
-    hipLaunchKernelGGL(k_flux_mixed, dim3(ncube - 1), dim3(WG), 0, s, Uin, Uout, halo, U1, U1q,
-                       fws, cube_size, pt, deltaT, deltaTau, e, RK, partials);
+    hipLaunchKernelGGL(k_flux_mixed, dim3(ncube - 1), dim3(WG), 0, s, Uin, Uout, halo, cube_mask,
+                       U1, U1q, fws, cube_size, pt, deltaT, deltaTau, e, RK, partials);
 }
 
 void launch_copy_f(hipStream_t s, const float* src, float* dst, size_t n) {

@@ -173,12 +173,14 @@ int bcm_engine_sync(bcm_engine* e);
  * engine stream). */
 int bcm_engine_time_part(bcm_engine* e, int part, int RK, int reps, double* ms);
 /* Flux kernel choice: variant 0 (one workgroup per cell plane) or 1 (one
- * workgroup per cube segment marching along i, nseg in {1,2,4,8,16}); fuse
- * non-zero lets variant 1 with nseg 1 apply the RK update in the same kernel.
- * All choices give bitwise the same results.  Setting a choice disables
- * tuning. */
+ * workgroup per cube segment marching along i, nseg in {1,2,4,8,16}); fuse 1
+ * lets variant 1 with nseg 1 apply the RK update in the same kernel, in
+ * place; fuse 2 (variant 1, nseg 1) writes the update and the face halos into
+ * a second copy of the iterate in engine stages, which then open with the
+ * edge passes and BCs instead of closing with the halo exchange.  All choices
+ * give bitwise the same results.  Setting a choice disables tuning. */
 int bcm_engine_set_flux(bcm_engine* e, int variant, int nseg, int fuse);
-/* Times the flux choices (flux + RK update of one stage) on the loaded case,
+/* Times the flux choices (one whole RK stage each) on the resident cases,
  * restoring the state afterwards, and keeps the fastest; runs automatically
  * before the first bcm_engine_run otherwise.  out3 (may be null) receives
  * {variant, nseg, fuse}. */
