@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(_HERE, "libbcm_cgli.so")
 
 STATE_NAMES = ("U1_", "U1", "U1q", "U1p1", "U1p2", "Rku1", "Residual1")
 PARTS = {"bc": 0, "ghost": 1, "interface": 2, "flux": 3, "update": 4, "shift": 5, "stage": 6,
-         "flux_update": 7, "halo_refresh": 8}
+         "flux_update": 7, "halo_refresh": 8, "flux_push": 9}
 
 
 class BCMError(RuntimeError):

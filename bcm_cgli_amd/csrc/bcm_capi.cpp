@@ -403,6 +403,7 @@ int bcm_engine_part(bcm_engine* h, int part, int RK) {
     case 6: engine_stage(e, RK, 0, e->halo_full_next); e->halo_full_next = false; break;
     case 7: engine_flux_update(e, RK, 0); break;
     case 8: engine_halo_refresh(e); break;
+    case 9: engine_flux_push(e, RK, 0); break;
     default: return fail("unknown part");
   }
   std::string err;
@@ -458,6 +459,7 @@ int bcm_engine_time_part(bcm_engine* h, int part, int RK, int reps, double* ms) 
       case 6: engine_stage(e, RK, 0); break;
       case 7: engine_flux_update(e, RK, 0); break;
       case 8: engine_halo_refresh(e); break;
+      case 9: engine_flux_push(e, RK, 0); break;
       default: return fail("unknown part");
     }
   }

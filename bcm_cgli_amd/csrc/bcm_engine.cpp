@@ -852,6 +852,20 @@ void engine_shift(Engine* e) {
     launch_shift(e->stream, e->U, e->U1, e->U1q, e->ncube, e->cube_case);
 }
 
+void engine_flux_push(Engine* e, int RK, int er_slot) {
+  if (e->prec == 1) {
+    engine_flux_update(e, RK, er_slot);
+    return;
+  }
+  launch_flux_push(e->stream, e->U, e->U2, e->halo, e->bc_mask_cube, e->U1, e->U1q, e->fws,
+                   e->cube_size, e->Rku, e->Res, e->ncube, e->pt(), e->deltaT, e->deltaTau, e->e,
+                   RK, e->U1p1, e->U1p2, e->partials);
+  std::swap(e->U, e->U2);
+  if (RK == 3)
+    launch_reduce(e->stream, e->partials, 1, e->case_desc, e->n_active, e->phys_dev,
+                  er_row(e, er_slot));
+}
+
 void engine_stage(Engine* e, int RK, int er_slot, bool full_pre_halo) {
   if (e->prec == 1) {
     // The previous flux kernel wrote the face halos; the edge passes of that
@@ -873,13 +887,7 @@ void engine_stage(Engine* e, int RK, int er_slot, bool full_pre_halo) {
       engine_interface(e);
     else
       engine_halo_refresh(e);
-    launch_flux_push(e->stream, e->U, e->U2, e->halo, e->bc_mask_cube, e->U1, e->U1q, e->fws,
-                     e->cube_size, e->Rku, e->Res, e->ncube, e->pt(), e->deltaT, e->deltaTau,
-                     e->e, RK, e->U1p1, e->U1p2, e->partials);
-    std::swap(e->U, e->U2);
-    if (RK == 3)
-      launch_reduce(e->stream, e->partials, 1, e->case_desc, e->n_active, e->phys_dev,
-                    er_row(e, er_slot));
+    engine_flux_push(e, RK, er_slot);
     return;
   }
   engine_bc(e);

@@ -170,6 +170,9 @@ void engine_update(Engine* e, int RK, int er_slot);
 // choice allows it (marching kernel, one workgroup per cube).
 void engine_flux_update(Engine* e, int RK, int er_slot);
 bool engine_fused(const Engine* e);
+// Flux and update of one stage written to the other copy of the iterate with
+// the face halos (the form stages use when flux_push is set); swaps copies.
+void engine_flux_push(Engine* e, int RK, int er_slot);
 void engine_shift(Engine* e);
 void engine_stage(Engine* e, int RK, int er_slot, bool full_pre_halo = false);
 // n_steps physical steps of n_inner inner iterations for every resident case.

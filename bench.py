@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # read the inner iterate (FP64 40 B, FP32 20 B), U1 and U1q (40 B each) and the
 # flag (4 B), write the stage result (40 B or 20 B).
 ALGO_BYTES = {"fp64": 164, "mixed": 124}
-FLUX_KERNEL = {"fp64": "k_flux_march", "mixed": "k_flux_mixed"}
+FLUX_KERNEL = {"fp64": "k_flux_march", "mixed": "k_flux_mixed"}   # traffic file stems
 
 
 def parse_args():
@@ -106,7 +106,10 @@ def run_pool(B, cases, precision, steps, warmup, n_inner, flux_reps, keep_state)
         for s, c in zip(slots, cases):
             fluid = c.array("FWS")[1:, 2:18, 2:18, 2:18] == 1
             res["rho"].append(eng.case_state(s, "U1")[1:, 2:18, 2:18, 2:18, 0][fluid])
-    res["flux_ms"] = eng.time_part("flux_update", RK=2, reps=flux_reps)
+    # The flux kernel the stages run: with the halo push (fuse 2) or not.
+    flux_part = "flux_push" if choice.get("fuse") == 2 else "flux_update"
+    res["flux_ms"] = eng.time_part(flux_part, RK=2, reps=flux_reps)
+    res["flux_part"] = flux_part
     res["stage_ms"] = eng.time_part("stage", RK=1, reps=max(3, flux_reps // 3))
     res["parts_ms"] = {p: round(eng.time_part(p, RK=1, reps=flux_reps), 4)
                        for p in ("bc", "ghost", "halo_refresh", "interface")}
@@ -126,13 +129,14 @@ def errors(mixed, ref):
     return out
 
 
-def roofline(precision, cells, flux_ms, config):
+def roofline(precision, cells, flux_ms, config, push=True):
     algo = ALGO_BYTES[precision] * cells
     achieved = algo / (flux_ms * 1e-3) / 1e9
+    form = "writing the update and face halos to the second copy" if push else "in place"
     return {
         "bound": "hbm",
         "kernel": f"{FLUX_KERNEL[precision]} (flux + residual + preconditioned increment + RK "
-                  "update of one stage, all cubes)",
+                  f"update of one stage, all cubes, {form})",
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -206,7 +210,8 @@ def main():
             "parallelism": "single GPU, hipGraph per physical step",
             "setup_s": round(setup_s, 3),
         },
-        "roofline": roofline(main_prec, cells, r["flux_ms"], args.config),
+        "roofline": roofline(main_prec, cells, r["flux_ms"], args.config,
+                             r["flux_part"] == "flux_push" or main_prec == "mixed"),
         "flux_choice": r["choice"],
         "kernels_ms": {"flux_update": round(r["flux_ms"], 4), "stage": round(r["stage_ms"], 4),
                        **r["parts_ms"]},
@@ -250,7 +255,8 @@ def main():
         }
         f["batch_fp64"] = {
             "value": round(b64["value"], 3), "unit": "Mcell-iter/s", "config": batch_cfg,
-            "roofline": roofline("fp64", b64["cells"], b64["flux_ms"], "batch"),
+            "roofline": roofline("fp64", b64["cells"], b64["flux_ms"], "batch",
+                                 b64["flux_part"] == "flux_push"),
             "flux_choice": b64["choice"],
         }
         f["batch_mixed"] = {

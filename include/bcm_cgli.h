@@ -161,7 +161,9 @@ int bcm_engine_run(bcm_engine* e, int n_steps, int n_inner, double* er_hist, int
 /* One piece of a stage: 0 domain BCs, 1 IB ghost fill, 2 halo + edges,
  * 3 flux, 4 RK update (with norms at RK 3), 5 time shift, 6 whole stage,
  * 7 flux and RK update as a stage runs them (fused when the choice allows),
- * 8 the pre-flux halo refresh a stage uses (IB ghost copies + edge passes). */
+ * 8 the pre-flux halo refresh a stage uses (IB ghost copies + edge passes),
+ * 9 flux and RK update written to the other copy of the iterate together
+ * with the face halos (the stage form of flux choice fuse 2). */
 int bcm_engine_part(bcm_engine* e, int part, int RK);
 /* which: 0 U1_, 1 U1, 2 U1q, 3 U1p1, 4 U1p2, 5 Rku1, 6 Residual1. */
 int bcm_engine_get_state(bcm_engine* e, int which, double* aos);
