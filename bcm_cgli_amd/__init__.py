@@ -84,6 +84,17 @@ def load_library(path=LIB_PATH):
     lib.bcm_engine_case_state.argtypes = [c_void_p, c_int, c_int, c_void_p, c_int]
     lib.bcm_engine_case_er.argtypes = [c_void_p, c_int, c_int, c_void_p]
     lib.bcm_engine_pool_info.argtypes = [c_void_p, c_void_p]
+    lib.bcm_case_read.argtypes = [c_char_p, P(CaseParamsC), P(c_void_p)]
+    lib.bcm_case_write_ibm.argtypes = [c_void_p, c_char_p]
+    lib.bcm_case_write_plot3d.argtypes = [c_void_p, c_void_p, c_int, c_char_p]
+    LL = ctypes.c_longlong
+    lib.bcm_write_restart.argtypes = [c_char_p, c_int, LL, c_void_p, c_void_p, c_void_p, c_void_p]
+    lib.bcm_read_restart.argtypes = [c_char_p, c_int, P(LL), c_void_p, c_void_p, c_void_p,
+                                     c_void_p]
+    lib.bcm_engine_output.argtypes = [c_void_p, c_int, c_int, c_char_p]
+    lib.bcm_engine_output_wait.argtypes = [c_void_p]
+    lib.bcm_engine_save_restart.argtypes = [c_void_p, c_int, c_char_p, LL, c_void_p]
+    lib.bcm_engine_load_restart.argtypes = [c_void_p, c_int, c_char_p, P(LL), c_void_p]
     _lib = lib
     return lib
 
@@ -96,23 +107,26 @@ def _check(rc):
 class Case:
     """A flow case built on the host in the reference's array layouts."""
 
-    def __init__(self, params):
+    def __init__(self, params, _from_dir=None):
         lib = load_library()
         self.params = dict(params)
         p = CaseParamsC()
         for d in range(3):
-            p.lattice[d] = int(params["lattice"][d])
-            p.origin[d] = float(params["origin"][d])
-            p.sphere_center[d] = float(params["sphere_center"][d])
-        p.cube_size = float(params["cube_size"])
+            p.lattice[d] = int(params.get("lattice", (0, 0, 0))[d])
+            p.origin[d] = float(params.get("origin", (0.0, 0.0, 0.0))[d])
+            p.sphere_center[d] = float(params.get("sphere_center", (0.0, 0.0, 0.0))[d])
+        p.cube_size = float(params.get("cube_size", 0.0))
         p.sphere_diameter = float(params["sphere_diameter"])
-        p.ico_level = int(params["ico_level"])
+        p.ico_level = int(params.get("ico_level", 0))
         p.mach = float(params["mach"])
         p.reynolds = float(params["reynolds"])
-        p.seed = int(params["seed"])
-        p.noise_amp = float(params["noise_amp"])
+        p.seed = int(params.get("seed", 0))
+        p.noise_amp = float(params.get("noise_amp", 0.0))
         h = ctypes.c_void_p()
-        _check(lib.bcm_case_build(ctypes.byref(p), ctypes.byref(h)))
+        if _from_dir is None:
+            _check(lib.bcm_case_build(ctypes.byref(p), ctypes.byref(h)))
+        else:
+            _check(lib.bcm_case_read(os.fsencode(_from_dir), ctypes.byref(p), ctypes.byref(h)))
         self._h = h
         info = (ctypes.c_int * 16)()
         phys = (ctypes.c_double * 8)()
@@ -124,6 +138,14 @@ class Case:
         self.phys = dict(zip(("rho0", "U0", "Uin", "V0", "W0", "P0", "T0", "mu_L"), list(phys)))
         self.ncube = self.info["ncube"]
 
+    @classmethod
+    def from_files(cls, directory, params):
+        """A case from the reference's input files BCMgrid00000.dat and
+        BCM_STL_fine.stl in ``directory``.  ``params`` needs ``mach``,
+        ``reynolds`` and ``sphere_diameter`` (the reference length of the
+        viscosity) and may give ``seed`` and ``noise_amp``."""
+        return cls(params, _from_dir=directory)
+
     def array(self, name):
         """Numpy view of a case array (owned by the case)."""
         n = ctypes.c_longlong()
@@ -131,20 +153,38 @@ class Case:
         if not ptr:
             raise BCMError(_lib.bcm_last_error().decode())
         dbl = name in ("U1_", "U1", "U1q", "cube_size", "Xcube", "Ycube", "Zcube",
-                       "ib_minus_w", "ib_plus_w")
+                       "Xcnt", "Ycnt", "Zcnt", "ib_minus_w", "ib_plus_w")
         ct = ctypes.c_double if dbl else ctypes.c_int
         buf = (ct * n.value).from_address(ptr)
         a = np.ctypeslib.as_array(buf)
         if name in ("U1_", "U1", "U1q"):
             a = a.reshape(self.ncube, 20, 20, 20, 5)
-        elif name == "FWS":
+        elif name in ("FWS", "FWS_grid"):
             a = a.reshape(self.ncube, 20, 20, 20)
         elif name == "adj_number":
             a = a.reshape(self.ncube, 5, 7)
+        elif name in ("Xcnt", "Ycnt", "Zcnt"):
+            a = a.reshape(self.ncube, 20)
         return a
 
     def write_inputs(self, directory):
-        _check(_lib.bcm_case_write_inputs(self._h, directory.encode()))
+        """BCMgrid00000.dat and BCM_STL_fine.stl, the reference's inputs."""
+        _check(_lib.bcm_case_write_inputs(self._h, os.fsencode(directory)))
+
+    def write_ibm(self, directory):
+        """IBM_minus00000.dat and IBM_plus00000.dat in the reference's format."""
+        _check(_lib.bcm_case_write_ibm(self._h, os.fsencode(directory)))
+
+    def write_plot3d(self, directory, step, state=None):
+        """PLOT3D BCMdata00001.x and qBCMdata<step>.q of ``state`` (an AoS
+        state of this case; default its U1_)."""
+        ptr = None
+        if state is not None:
+            a = np.ascontiguousarray(state, dtype=np.float64)
+            if a.size != self.ncube * 40000:
+                raise BCMError("state does not match the case's cube count")
+            ptr = a.ctypes.data
+        _check(_lib.bcm_case_write_plot3d(self._h, ptr, int(step), os.fsencode(directory)))
 
     def cells(self):
         return (self.ncube - 1) * 4096
@@ -162,6 +202,33 @@ class Case:
 
 
 PRECISIONS = {"fp64": 0, "mixed": 1}
+
+
+def write_restart(path, step, U1_, U1, U1q, E=None):
+    """Restart snapshot of the three time levels (AoS states of one case),
+    the step and the norm scales E1..E5."""
+    lib = load_library()
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (U1_, U1, U1q)]
+    ncube = arrs[0].size // 40000
+    if any(a.size != ncube * 40000 for a in arrs):
+        raise BCMError("the three states differ in size")
+    e = None if E is None else np.ascontiguousarray(E, dtype=np.float64)
+    if e is not None and e.size != 5:
+        raise BCMError("E holds five norm scales")
+    _check(lib.bcm_write_restart(os.fsencode(path), ncube, int(step),
+                                 None if e is None else e.ctypes.data,
+                                 *(a.ctypes.data for a in arrs)))
+
+
+def read_restart(path, ncube):
+    """(step, E[5], U1_, U1, U1q) of a restart file holding ``ncube`` cubes."""
+    lib = load_library()
+    step = ctypes.c_longlong()
+    e = np.zeros(5)
+    arrs = [np.empty((ncube, 20, 20, 20, 5)) for _ in range(3)]
+    _check(lib.bcm_read_restart(os.fsencode(path), int(ncube), ctypes.byref(step), e.ctypes.data,
+                                *(a.ctypes.data for a in arrs)))
+    return (step.value, e, *arrs)
 
 
 def bytes_per_cube(precision="fp64"):
@@ -223,6 +290,28 @@ class Engine:
         out = np.zeros(8)
         _check(_lib.bcm_engine_case_er(self._h, int(slot), int(inner), out.ctypes.data))
         return out
+
+    def output(self, slot, step, directory):
+        """Queues the PLOT3D output of a resident case's U1_ (asynchronous;
+        output_wait reports write errors)."""
+        _check(_lib.bcm_engine_output(self._h, int(slot), int(step), os.fsencode(directory)))
+
+    def output_wait(self):
+        _check(_lib.bcm_engine_output_wait(self._h))
+
+    def save_restart(self, slot, path, step, E=None):
+        e = None if E is None else np.ascontiguousarray(E, dtype=np.float64)
+        _check(_lib.bcm_engine_save_restart(self._h, int(slot), os.fsencode(path), int(step),
+                                            None if e is None else e.ctypes.data))
+
+    def load_restart(self, slot, path):
+        """Replaces a resident case's time levels by a restart file's; returns
+        (step, E[5])."""
+        step = ctypes.c_longlong()
+        e = np.zeros(5)
+        _check(_lib.bcm_engine_load_restart(self._h, int(slot), os.fsencode(path),
+                                            ctypes.byref(step), e.ctypes.data))
+        return step.value, e
 
     def pool_info(self):
         v = (ctypes.c_int * 6)()

@@ -34,7 +34,10 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-SOURCES = ["bcm_kernels.hip", "bcm_kernels_mixed.hip", "bcm_engine.cpp", "bcm_capi.cpp", "bcm_setup.cpp"]
+SOURCES = ["bcm_kernels.hip", "bcm_kernels_mixed.hip", "bcm_engine.cpp", "bcm_output.cpp",
+           "bcm_capi.cpp", "bcm_setup.cpp", "bcm_io.cpp"]
+# Sources that include the HIP runtime; the others are plain host C++.
+HIP_HOST = ("bcm_engine.cpp", "bcm_output.cpp", "bcm_capi.cpp")
 
 # The mixed-precision kernels carry no bitwise contract with the reference:
 # they let the compiler fuse multiply-adds.
@@ -66,7 +69,7 @@ def build_product(verbose=False, extra=()):
     procs = []
     for src in SOURCES:
         obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
-        lang = ["-x", "hip"] if src.endswith(".hip") or src in ("bcm_engine.cpp", "bcm_capi.cpp") else []
+        lang = ["-x", "hip"] if src.endswith(".hip") or src in HIP_HOST else []
         cmd = [hipcc, *HIP_FLAGS, *PER_FILE_FLAGS.get(src, []), *extra, *lang, "-c",
                os.path.join(CSRC, src), "-o", obj]
         if verbose:

@@ -161,9 +161,8 @@ const char* bcm_last_error(void) { return g_err.c_str(); }
 
 // ---------------------------------------------------------------------------
 // Host cases.
-int bcm_case_build(const bcm_case_params* p, bcm_case** out) {
-  if (!p || !out) return fail("null argument");
-  *out = nullptr;
+namespace {
+CaseParams to_params(const bcm_case_params* p) {
   CaseParams cp;
   for (int d = 0; d < 3; ++d) {
     cp.lattice[d] = p->lattice[d];
@@ -177,17 +176,37 @@ int bcm_case_build(const bcm_case_params* p, bcm_case** out) {
   cp.reynolds = p->reynolds;
   cp.seed = p->seed;
   cp.noise_amp = p->noise_amp;
-  bcm_case* c = new bcm_case();
-  std::string err;
-  if (build_case(cp, c->cs, err)) {
-    delete c;
-    return fail(err);
-  }
-  for (auto pr : {std::make_pair(&c->cs.ib_minus, &c->ib_minus_gc), std::make_pair(&c->cs.ib_plus, &c->ib_plus_gc)})
-    *pr.second = pr.first->gcindex;
+  return cp;
+}
+
+// IB tables also exposed with their 1-based index arrays.
+void finish_handle(bcm_case* c) {
+  c->ib_minus_gc = c->cs.ib_minus.gcindex;
+  c->ib_plus_gc = c->cs.ib_plus.gcindex;
   c->ib_minus_ip = c->cs.ib_minus.ipsur;
   c->ib_plus_ip = c->cs.ib_plus.ipsur;
-  *out = c;
+}
+}  // namespace
+
+int bcm_case_build(const bcm_case_params* p, bcm_case** out) {
+  if (!p || !out) return fail("null argument");
+  *out = nullptr;
+  std::unique_ptr<bcm_case> c(new bcm_case());
+  std::string err;
+  if (build_case(to_params(p), c->cs, err)) return fail(err);
+  finish_handle(c.get());
+  *out = c.release();
+  return 0;
+}
+
+int bcm_case_read(const char* dir, const bcm_case_params* p, bcm_case** out) {
+  if (!dir || !p || !out) return fail("null argument");
+  *out = nullptr;
+  std::unique_ptr<bcm_case> c(new bcm_case());
+  std::string err;
+  if (read_case(dir, to_params(p), c->cs, err)) return fail(err);
+  finish_handle(c.get());
+  *out = c.release();
   return 0;
 }
 
@@ -235,6 +254,9 @@ void* bcm_case_array(bcm_case* c, const char* name, long long* count) {
   };
   std::string n(name);
   if (n == "U1_") return ret(cs.U1_);
+  if (n == "Xcnt") return ret(cs.Xcnt);
+  if (n == "Ycnt") return ret(cs.Ycnt);
+  if (n == "Zcnt") return ret(cs.Zcnt);
   if (n == "U1") return ret(cs.U1);
   if (n == "U1q") return ret(cs.U1q);
   if (n == "cube_size") return ret(cs.cube_size);
@@ -242,6 +264,7 @@ void* bcm_case_array(bcm_case* c, const char* name, long long* count) {
   if (n == "Ycube") return ret(cs.Ycube);
   if (n == "Zcube") return ret(cs.Zcube);
   if (n == "FWS") return ret(cs.fws);
+  if (n == "FWS_grid") return ret(cs.fws_grid);
   if (n == "csl") return ret(cs.csl);
   if (n == "adj_number") return ret(cs.adj_number);
   if (n == "ib_minus_gc") return ret(c->ib_minus_gc);
@@ -265,6 +288,40 @@ int bcm_case_write_inputs(const bcm_case* c, const char* dir) {
   std::string d(dir);
   if (write_bcmgrid(c->cs, d + "/BCMgrid00000.dat")) return fail("cannot write grid file");
   if (write_stl(c->cs, d + "/BCM_STL_fine.stl")) return fail("cannot write STL file");
+  return 0;
+}
+
+int bcm_case_write_ibm(const bcm_case* c, const char* dir) {
+  if (!c || !dir) return fail("null argument");
+  if (write_ibm(c->cs, dir)) return fail(std::string("cannot write the IB tables in ") + dir);
+  return 0;
+}
+
+int bcm_case_write_plot3d(const bcm_case* c, const double* aos, int step, const char* dir) {
+  if (!c || !dir) return fail("null argument");
+  if (step < 0 || step > 99999) return fail("output step must lie in 0..99999");
+  const Case& cs = c->cs;
+  std::string err;
+  if (write_plot3d(cs.ncube, cs.Xcnt.data(), cs.Ycnt.data(), cs.Zcnt.data(), cs.ncube - 1,
+                   aos ? aos : cs.U1_.data(), step, dir, err))
+    return fail(err);
+  return 0;
+}
+
+int bcm_write_restart(const char* path, int ncube, long long step, const double* E5,
+                      const double* U1_, const double* U1, const double* U1q) {
+  if (!path || !U1_ || !U1 || !U1q) return fail("null argument");
+  if (ncube < 2) return fail("a restart needs at least one cube");
+  std::string err;
+  if (write_restart(path, ncube, step, E5, U1_, U1, U1q, err)) return fail(err);
+  return 0;
+}
+
+int bcm_read_restart(const char* path, int ncube, long long* step, double* E5, double* U1_,
+                     double* U1, double* U1q) {
+  if (!path || !U1_ || !U1 || !U1q) return fail("null argument");
+  std::string err;
+  if (read_restart(path, ncube, step, E5, U1_, U1, U1q, err)) return fail(err);
   return 0;
 }
 
@@ -332,6 +389,53 @@ int bcm_engine_case_state(bcm_engine* h, int slot, int which, double* aos, int u
   if (which < 0 || which > 6) return fail("unknown state array");
   std::string err;
   if (engine_case_state(h->e, slot, which, aos, upload != 0, err)) return fail(err);
+  return 0;
+}
+
+int bcm_engine_output(bcm_engine* h, int slot, int step, const char* dir) {
+  if (!h || !dir) return fail("null argument");
+  std::string err;
+  if (engine_output(h->e, slot, step, dir, err)) return fail(err);
+  return 0;
+}
+
+int bcm_engine_output_wait(bcm_engine* h) {
+  if (!h) return fail("null engine");
+  std::string err;
+  if (engine_output_wait(h->e, err)) return fail(err);
+  return 0;
+}
+
+int bcm_engine_save_restart(bcm_engine* h, int slot, const char* path, long long step,
+                            const double* E5) {
+  if (!h || !path) return fail("null argument");
+  Engine* e = h->e;
+  if (slot < 0 || slot >= e->max_cases || !e->cases[slot].active) return fail("no such case slot");
+  const int ncube = e->cases[slot].tb.n + 1;
+  std::vector<double> st[3];
+  std::string err;
+  for (int w = 0; w < 3; ++w) {
+    st[w].resize((size_t)ncube * CUBE_STRIDE);
+    if (engine_case_state(e, slot, w, st[w].data(), false, err)) return fail(err);
+  }
+  if (write_restart(path, ncube, step, E5, st[0].data(), st[1].data(), st[2].data(), err))
+    return fail(err);
+  return 0;
+}
+
+int bcm_engine_load_restart(bcm_engine* h, int slot, const char* path, long long* step,
+                            double* E5) {
+  if (!h || !path) return fail("null argument");
+  Engine* e = h->e;
+  if (slot < 0 || slot >= e->max_cases || !e->cases[slot].active) return fail("no such case slot");
+  const int ncube = e->cases[slot].tb.n + 1;
+  std::vector<double> st[3];
+  for (auto& v : st) v.resize((size_t)ncube * CUBE_STRIDE);
+  std::string err;
+  if (read_restart(path, ncube, step, E5, st[0].data(), st[1].data(), st[2].data(), err))
+    return fail(err);
+  for (int w = 0; w < 3; ++w)
+    if (engine_case_state(e, slot, w, st[w].data(), true, err)) return fail(err);
   return 0;
 }
 
@@ -714,6 +818,71 @@ void BCM_Flux_XYZ_Viscous_Runge_kutta(int myid, int ncube, int RK, double deltaT
     check(where, hipStreamSynchronize(en->stream));
     for (int m = 1; m <= 7; ++m) er[m] = tmp[m];
   }
+}
+
+void BCM_Grid(int myid, int ncube, int n_wallcube, double* cube_size, int* csl, double* Xcube,
+              double* Ycube, double* Zcube, double* Xcnt, double* Ycnt, double* Zcnt, int* FWS,
+              int* adj_number, int* wallcube) {
+  const char* where = "BCM_Grid";
+  char name[64];
+  snprintf(name, sizeof name, "BCMgrid%05d.dat", myid);
+  Case cs;
+  std::string err;
+  if (read_bcmgrid(name, cs, err)) fatal(where, err);
+  if (cs.ncube != ncube || cs.n_wallcube != n_wallcube)
+    fatal(where, "cube or wall-cube count differs from the grid file");
+  grid_geometry(cs);
+  // Entries the reference's reader sets: cubes 1..ncube-1, adjacency slots
+  // 1..4 of directions 1..6, wall cubes 1..n_wallcube-1.
+  for (int c = 1; c < ncube; ++c) {
+    cube_size[c] = cs.cube_size[c];
+    Xcube[c] = cs.Xcube[c];
+    Ycube[c] = cs.Ycube[c];
+    Zcube[c] = cs.Zcube[c];
+    csl[c] = cs.csl[c];
+    memcpy(Xcnt + (size_t)c * NP, &cs.Xcnt[(size_t)c * NP], NP * sizeof(double));
+    memcpy(Ycnt + (size_t)c * NP, &cs.Ycnt[(size_t)c * NP], NP * sizeof(double));
+    memcpy(Zcnt + (size_t)c * NP, &cs.Zcnt[(size_t)c * NP], NP * sizeof(double));
+    memcpy(FWS + (size_t)c * CELLS_PADDED, &cs.fws[(size_t)c * CELLS_PADDED],
+           CELLS_PADDED * sizeof(int));
+    for (int slot = 1; slot <= 4; ++slot)
+      for (int d = 1; d <= 6; ++d)
+        adj_number[adj_index(c, slot, d)] = cs.adj_number[adj_index(c, slot, d)];
+  }
+  for (int w = 1; w < n_wallcube; ++w) wallcube[w] = cs.wallcube[w];
+}
+
+void BCM_Reading_IBM(int myid, int* NNBC, int mp_switch, double* weight, int* GCindex,
+                     int* IPsur) {
+  const char* where = "BCM_Reading_IBM";
+  if (!NNBC || mp_switch != 1 && mp_switch != -1) fatal(where, "mp_switch must be 1 or -1");
+  char name[64];
+  snprintf(name, sizeof name, "IBM_%s%05d.dat", mp_switch == 1 ? "plus" : "minus", myid);
+  IBTable tb;
+  std::string err;
+  if (read_ibm(name, tb, err)) fatal(where, err);
+  const int n = *NNBC;
+  if (n < 0 || n > tb.n) fatal(where, "the file holds fewer entries than requested");
+  memcpy(GCindex + 1, tb.gcindex.data() + 1, (size_t)4 * n * sizeof(int));
+  memcpy(IPsur + 1, tb.ipsur.data() + 1, (size_t)4 * n * sizeof(int));
+  memcpy(weight + 1, tb.weight.data() + 1, (size_t)8 * n * sizeof(double));
+}
+
+void BCM_Output(int myid, int ncube, int step, int* rank_map, double* U1, double* U1q,
+                double* cube_size, double* Xcnt, double* Ycnt, double* Zcnt) {
+  const char* where = "BCM_Output";
+  (void)myid;
+  (void)rank_map;
+  (void)U1q;
+  (void)cube_size;
+  if (step < 0 || step > 99999) fatal(where, "step must lie in 0..99999");
+  // The state of a bound engine lives on the device: bring U1_ home first.
+  if (g_current && g_current->di && U1 == g_current->di->U1_) {
+    if (g_current->di->ncube != ncube) fatal(where, "ncube differs from the bound arrays");
+    if (engine_download_state(g_current->e, 0, U1)) fatal(where, "state download failed");
+  }
+  std::string err;
+  if (write_plot3d(ncube, Xcnt, Ycnt, Zcnt, ncube - 1, U1, step, ".", err)) fatal(where, err);
 }
 
 void BCM_Physical_time_shift(double* U1_, double* U1, double* U1q) {

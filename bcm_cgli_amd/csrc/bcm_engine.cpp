@@ -274,6 +274,7 @@ void engine_destroy(Engine* e) {
   if (!e) return;
   hipSetDevice(e->device);
   if (e->stream) hipStreamSynchronize(e->stream);
+  engine_output_free(e);
   free_case(e);
   if (e->ev0) hipEventDestroy(e->ev0);
   if (e->ev1) hipEventDestroy(e->ev1);
@@ -283,6 +284,7 @@ void engine_destroy(Engine* e) {
 
 int engine_alloc(Engine* e, int cap, int max_cases, std::string& err, int prec) {
   HIP_TRY(hipSetDevice(e->device));
+  if (engine_output_wait(e, err)) return 1;
   free_case(e);
   if (cap < 2) {
     err = "a pool needs at least one cube";
@@ -560,6 +562,13 @@ int engine_add_case(Engine* e, const Case& cs, int* slot_out, std::string& err) 
   S.tb = std::move(tb);
   S.active = true;
   const int n = S.tb.n;
+  S.centres.assign((size_t)3 * (n + 1) * NP, 0.0);
+  for (int a = 0; a < 3; ++a) {
+    const std::vector<double>& src = a == 0 ? cs.Xcnt : a == 1 ? cs.Ycnt : cs.Zcnt;
+    if (src.size() >= (size_t)(n + 1) * NP)
+      std::copy(src.begin(), src.begin() + (size_t)(n + 1) * NP,
+                S.centres.begin() + (size_t)a * (n + 1) * NP);
+  }
   // Flags, cube sizes and state of the range; RK scratch, increments and
   // force densities start at zero as in the reference's fresh arrays.
   {

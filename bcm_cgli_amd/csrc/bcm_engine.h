@@ -57,7 +57,10 @@ struct CaseSlot {
   bool active = false;
   int first = 0;        // pool cube of the case's cube 1
   CaseTables tb;
+  std::vector<double> centres;   // Xcnt, Ycnt, Zcnt of cubes 0..n, [3][n + 1][20]
 };
+
+struct OutputQueue;
 
 struct Engine {
   int device = 0;
@@ -125,6 +128,7 @@ struct Engine {
   bool flux_tuned = false;
 
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  OutputQueue* out = nullptr;   // asynchronous output (created on first use)
   bool graph_valid = false;
   hipGraphExec_t graph_exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -184,5 +188,14 @@ int engine_sync(Engine* e, std::string& err);
 // restored afterwards) and keeps the fastest.  Returns the chosen
 // (variant, nseg) through the pointers.
 int engine_autotune(Engine* e, int* variant, int* nseg, std::string& err);
+
+// Asynchronous PLOT3D output (bcm_output.cpp): queues the current inner
+// iterate U1_ of a case for writing as BCMdata00001.x and qBCMdata<step>.q in
+// dir.  The solver stream only pays a layout conversion; the copy to the host
+// and the file writes overlap later work.  Errors of the writes are reported
+// by engine_output_wait, which also waits for every queued output.
+int engine_output(Engine* e, int slot, int step, const std::string& dir, std::string& err);
+int engine_output_wait(Engine* e, std::string& err);
+void engine_output_free(Engine* e);
 
 }  // namespace bcm

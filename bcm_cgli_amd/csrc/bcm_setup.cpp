@@ -644,20 +644,7 @@ int build_case(const CaseParams& p, Case& cs, std::string& err) {
         a[1 * 7 + 5] = iz > 0 ? cid(ix, iy, iz - 1) : 0;
         a[1 * 7 + 6] = iz < nzc - 1 ? cid(ix, iy, iz + 1) : 0;
       }
-  // Cell-centre coordinates, BCM_Grid.cpp:123-150.
-  for (int c = 1; c < ncube; ++c) {
-    double dx = cs.cube_size[c] / NC;
-    for (int i = 0; i < NP; ++i) {
-      cs.Xcnt[c * NP + i] = cs.Xcube[c] + (i - 1.5) * dx;
-      cs.Ycnt[c * NP + i] = cs.Ycube[c] + (i - 1.5) * dx;
-      cs.Zcnt[c * NP + i] = cs.Zcube[c] + (i - 1.5) * dx;
-    }
-  }
-  // Cube size level, BCM_Grid.cpp:155-168.
-  double size_min = BIGNUM;
-  for (int c = 1; c < ncube; ++c) size_min = dmin(size_min, cs.cube_size[c]);
-  for (int c = 1; c < ncube; ++c)
-    cs.csl[c] = static_cast<int>(0.5 + log(cs.cube_size[c] / size_min) / log(2.0));
+  grid_geometry(cs);
 
   // Grid flags: all cells fluid, interior cells whose centre lies inside the
   // sphere solid.  This is the content of the grid file the reference reads
@@ -683,6 +670,37 @@ int build_case(const CaseParams& p, Case& cs, std::string& err) {
   }
   cs.n_wallcube = (int)cs.wallcube.size();
   cs.fws_grid = cs.fws;
+  make_icosphere(p.ico_level, p.sphere_center, r, cs.stl);
+  return finish_setup(cs, err);
+}
+
+// Grid geometry derived from cube sizes and corners: cell centres
+// (BCM_Grid.cpp:123-150) and cube size levels (:155-168).
+void grid_geometry(Case& cs) {
+  const int ncube = cs.ncube;
+  cs.Xcnt.assign((size_t)ncube * NP, 0.0);
+  cs.Ycnt.assign((size_t)ncube * NP, 0.0);
+  cs.Zcnt.assign((size_t)ncube * NP, 0.0);
+  for (int c = 1; c < ncube; ++c) {
+    double dx = cs.cube_size[c] / NC;
+    for (int i = 0; i < NP; ++i) {
+      cs.Xcnt[c * NP + i] = cs.Xcube[c] + (i - 1.5) * dx;
+      cs.Ycnt[c * NP + i] = cs.Ycube[c] + (i - 1.5) * dx;
+      cs.Zcnt[c * NP + i] = cs.Zcube[c] + (i - 1.5) * dx;
+    }
+  }
+  cs.csl.assign(ncube, 0);
+  double size_min = BIGNUM;
+  for (int c = 1; c < ncube; ++c) size_min = dmin(size_min, cs.cube_size[c]);
+  for (int c = 1; c < ncube; ++c)
+    cs.csl[c] = static_cast<int>(0.5 + log(cs.cube_size[c] / size_min) / log(2.0));
+}
+
+// Everything after the grid (cube geometry, adjacency, grid flags, wall
+// cubes) and the surface triangles are known: main.cpp:108-1139.
+int finish_setup(Case& cs, std::string& err) {
+  const CaseParams& p = cs.p;
+  const int ncube = cs.ncube;
 
   // Domain-boundary cube lists, main.cpp:108-189.
   double Xmax = -BIGNUM, Xmin = BIGNUM, Ymax = -BIGNUM, Ymin = BIGNUM, Zmax = -BIGNUM, Zmin = BIGNUM;
@@ -770,8 +788,7 @@ int build_case(const CaseParams& p, Case& cs, std::string& err) {
     }
   }
 
-  // Surface and IB tables (main.cpp:811).
-  make_icosphere(p.ico_level, p.sphere_center, r, cs.stl);
+  // IB tables (main.cpp:811).
   immersed_boundary(cs);
 
   int* lp[L_COUNT];

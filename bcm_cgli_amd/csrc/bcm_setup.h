@@ -71,6 +71,12 @@ struct Case {
 // Builds the whole case.  Returns 0 on success, non-zero with a message in err.
 int build_case(const CaseParams& p, Case& cs, std::string& err);
 
+// Cell centres and size levels from cube sizes and corners.
+void grid_geometry(Case& cs);
+// The setup passes that follow the grid and the surface: boundary lists,
+// interface tables, flags, initial state, ghost detection, IB tables.
+int finish_setup(Case& cs, std::string& err);
+
 // Individual passes, exposed for tests.
 void compute_physics(const CaseParams& p, Physics& ph);
 void make_icosphere(int level, const double center[3], double radius,
@@ -84,5 +90,27 @@ void host_interface_copy(int ncube, const int* const* lists, const int* nadj,
 // Writers for the on-disk inputs the reference reads.
 int write_bcmgrid(const Case& cs, const std::string& path);
 int write_stl(const Case& cs, const std::string& path);
+
+// Case I/O (bcm_io.cpp).  Readers return 0 or non-zero with a message in err.
+int read_bcmgrid(const std::string& path, Case& cs, std::string& err);
+int read_stl(const std::string& path, std::vector<Triangle>& tris, std::string& err);
+int read_ibm(const std::string& path, IBTable& tb, std::string& err);
+// IBM_minus00000.dat and IBM_plus00000.dat in dir.
+int write_ibm(const Case& cs, const std::string& dir);
+// A case from BCMgrid00000.dat and BCM_STL_fine.stl in dir; p supplies the
+// flow parameters (Mach, Reynolds number, reference length, noise).
+int read_case(const std::string& dir, const CaseParams& p, Case& cs, std::string& err);
+// PLOT3D grid BCMdata00001.x and solution qBCMdata<step>.q in dir for cubes
+// 1..ncube-1 with cell centres Xcnt/Ycnt/Zcnt [ncube][20] and the AoS state U
+// [ncube][20][20][20][5]; the headers announce mpi_ncube blocks.
+int write_plot3d(int ncube, const double* Xcnt, const double* Ycnt, const double* Zcnt,
+                 int mpi_ncube, const double* U, int step, const std::string& dir,
+                 std::string& err);
+// Restart snapshot: step, norm scales E1..E5 and U1_, U1, U1q (AoS, ncube
+// slots each).  E5 may be null on write (zeros) and on read (ignored).
+int write_restart(const std::string& path, int ncube, long long step, const double* E5,
+                  const double* U1_, const double* U1, const double* U1q, std::string& err);
+int read_restart(const std::string& path, int ncube, long long* step, double* E5, double* U1_,
+                 double* U1, double* U1q, std::string& err);
 
 }  // namespace bcm

@@ -109,6 +109,25 @@ void BCM_Flux_XYZ_Viscous_Runge_kutta(int myid, int ncube, int RK, double deltaT
 /* Physical-time shift U1q <- U1, U1 <- U1_ on interior cells, which the
  * reference does inline in its main loop (main.cpp:1292-1315). */
 void BCM_Physical_time_shift(double* U1_, double* U1, double* U1q);
+/* Replaces BCM_Grid.cpp:18-347 (main.h:25-43): reads BCMgrid<myid>.dat from
+ * the working directory into the caller's arrays (cubes 1..ncube-1, slots
+ * 1..4 of adjacency directions 1..6, wall cubes 1..n_wallcube-1); cell
+ * centres and size levels follow from the cube sizes and corners. */
+void BCM_Grid(int myid, int ncube, int n_wallcube, double* cube_size, int* csl, double* Xcube,
+              double* Ycube, double* Zcube, double* Xcnt, double* Ycnt, double* Zcnt, int* FWS,
+              int* adj_number, int* wallcube);
+/* Replaces BCM_Reading_IBM.cpp:15-77 (main.h:359-373): reads *NNBC entries
+ * of IBM_plus<myid>.dat (mp_switch 1) or IBM_minus<myid>.dat (mp_switch -1)
+ * into the 1-based GCindex[4n+1], IPsur[4n+1] and weight[8n+1]. */
+void BCM_Reading_IBM(int myid, int* NNBC, int mp_switch, double* weight, int* GCindex,
+                     int* IPsur);
+/* Replaces BCM_Output.cpp:17-303 (main.h:515-533) for one rank: writes
+ * BCMdata00001.x and qBCMdata<step>.q of U1 (the caller passes U1_, as
+ * main.cpp:1348 does) into the working directory.  When U1 is the bound
+ * engine's U1_, the device state is copied into it first.  rank_map, U1q
+ * and cube_size are accepted and unused, as in the reference. */
+void BCM_Output(int myid, int ncube, int step, int* rank_map, double* U1, double* U1q,
+                double* cube_size, double* Xcnt, double* Ycnt, double* Zcnt);
 
 /* ------------------------------------------------------------------------ */
 /* 2. Host case construction.                                                 */
@@ -134,13 +153,41 @@ void bcm_case_free(bcm_case* c);
  * n_ghost_candidates, 0.  phys[8]: rho0, U0, Uin, V0, W0, P0, T0, mu_L. */
 int bcm_case_info(const bcm_case* c, int* info, double* phys);
 /* Pointer to a case array and its element count.  Names: "U1_", "U1", "U1q",
- * "cube_size", "Xcube", "Ycube", "Zcube" (double); "FWS", "csl",
+ * "cube_size", "Xcube", "Ycube", "Zcube", "Xcnt", "Ycnt", "Zcnt" (double); "FWS",
+ * "FWS_grid" (the flags as the grid file holds them), "csl",
  * "adj_number", "ib_minus_gc", "ib_minus_ip", "ib_plus_gc", "ib_plus_ip",
  * "Xbc_l", "Xbc_u", "Ybc_l", "Ybc_u", "Zbc_l", "Zbc_u", "adjX_eq", "adjY_eq",
  * "adjZ_eq" (int); "ib_minus_w", "ib_plus_w" (double). */
 void* bcm_case_array(bcm_case* c, const char* name, long long* count);
 /* Writes BCMgrid00000.dat and BCM_STL_fine.stl, the reference's input files. */
 int bcm_case_write_inputs(const bcm_case* c, const char* dir);
+/* Builds a case from the reference's input files in dir: the grid file
+ * BCMgrid00000.dat as BCM_Grid.cpp:63-261 reads it (cube sizes and corners,
+ * adjacency, wall cubes, run-length coded flags) and the ASCII STL surface
+ * BCM_STL_fine.stl as BCM_Immersed_boundary.cpp:198-275 reads it; then the
+ * setup of main.cpp:108-1139.  p supplies the flow parameters the reference
+ * keeps in prm.h: mach, reynolds, sphere_diameter (the reference length of
+ * mu_L), seed and noise_amp; its lattice, cube_size, origin, sphere_center
+ * and ico_level are not used. */
+int bcm_case_read(const char* dir, const bcm_case_params* p, bcm_case** out);
+/* Writes the IB tables as IBM_minus00000.dat and IBM_plus00000.dat in the
+ * byte format of BCM_Immersed_boundary.cpp:1009-1065 (read back by
+ * BCM_Reading_IBM.cpp:48-75). */
+int bcm_case_write_ibm(const bcm_case* c, const char* dir);
+/* Writes the PLOT3D grid BCMdata00001.x and solution qBCMdata<step>.q of the
+ * state aos ((ncube) cubes of 20^3 x 5 doubles; null: the case's U1_) in the
+ * byte layout BCM_Output.cpp:86-301 writes on one rank (replaces BCM_Output,
+ * main.h, called at main.cpp:1346-1351).  step in 0..99999. */
+int bcm_case_write_plot3d(const bcm_case* c, const double* aos, int step, const char* dir);
+/* Restart snapshot (the reference has none: its reader,
+ * BCM_Initial_condition.cpp:91-227, is disabled, and its output omits U1q):
+ * step, the norm scales E1..E5 (E5 may be null: zeros on write, ignored on
+ * read) and U1_, U1, U1q, each ncube cubes of 20^3 x 5 doubles.  A read
+ * fails unless the file holds exactly ncube cubes. */
+int bcm_write_restart(const char* path, int ncube, long long step, const double* E5,
+                      const double* U1_, const double* U1, const double* U1q);
+int bcm_read_restart(const char* path, int ncube, long long* step, double* E5, double* U1_,
+                     double* U1, double* U1q);
 
 /* ------------------------------------------------------------------------ */
 /* 3. Device engine.                                                          */
@@ -221,6 +268,28 @@ int bcm_engine_case_er(bcm_engine* e, int slot, int inner, double* er8);
 /* info6: pool cubes, cubes spanned, active cases, case slots, largest free
  * range, cubes in use. */
 int bcm_engine_pool_info(bcm_engine* e, int* info6);
+
+/* ------------------------------------------------------------------------ */
+/* 5. Output and restart of resident cases.                                   */
+
+/* Queues the PLOT3D output of a resident case's inner iterate U1_ (the array
+ * main.cpp:1348 hands to BCM_Output) as BCMdata00001.x and qBCMdata<step>.q in
+ * dir, in the bytes bcm_case_write_plot3d writes.  The engine stream pays one
+ * layout conversion; the copy to the host runs on a second stream and the
+ * files are written by a host thread, both overlapping later engine work.
+ * At most two outputs are in flight; a third waits for the oldest. */
+int bcm_engine_output(bcm_engine* e, int slot, int step, const char* dir);
+/* Waits for every queued output; returns the first write error since the
+ * last wait. */
+int bcm_engine_output_wait(bcm_engine* e);
+/* Restart snapshot of a resident case (U1_, U1, U1q, with the caller's step
+ * and norm scales) in the bcm_write_restart format, and its inverse.  A
+ * mixed pool saves U1_ as the FP64 sum of the freestream state and the FP32
+ * deviation; loading re-rounds the deviation to FP32. */
+int bcm_engine_save_restart(bcm_engine* e, int slot, const char* path, long long step,
+                            const double* E5);
+int bcm_engine_load_restart(bcm_engine* e, int slot, const char* path, long long* step,
+                            double* E5);
 
 #ifdef __cplusplus
 }

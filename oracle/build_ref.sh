@@ -27,7 +27,7 @@ CXXFLAGS="-O2 -ffp-contract=off -fno-fast-math -fopenmp -w -I$OUT/cfg -I- -I$REF
 TUS="BCM_Flux_XYZ_Viscous_Runge_kutta BCM_Ghostcell_minus BCM_Ghostcell_plus
      BCM_X_boundary_condition BCM_Y_boundary_condition BCM_Z_boundary_condition
      BCM_Interface BCM_Interface_table BCM_Grid BCM_FWS_Interface BCM_Initial_condition
-     BCM_Immersed_boundary BCM_Reading_IBM"
+     BCM_Immersed_boundary BCM_Reading_IBM BCM_Output"
 pids=()
 for tu in $TUS; do
   g++ $CXXFLAGS -c "$REF/$tu.cpp" -o "$OUT/obj/$tu.o" 2> "$OUT/obj/$tu.log" &

@@ -7,7 +7,9 @@
 // and physical-step counts are arguments with the convergence break disabled,
 // and fluid densities get the seeded noise the cases prescribe between the
 // initial condition and the U1 -> U1_/U1q copy.  Arrays are dumped as raw
-// little-endian binaries in Array.h layout for the fixture writer.
+// little-endian binaries in Array.h layout for the fixture writer; the
+// reference's own writers leave IBM_minus/plus00000.dat and the PLOT3D files
+// BCMdata00001.x and qBCMdata<n_steps>.q in the case directory.
 //
 // Usage (cwd = case directory holding BCMgrid00000.dat and BCM_STL_fine.stl):
 //   ref_driver <n_steps> <n_inner> <seed> <noise_amp> <dump_dir> [dump_stages]
@@ -54,6 +56,8 @@ void BCM_Ghostcell_minus(int, int*, double*, int*, int*, Flags, State);
 void BCM_Ghostcell_plus(int, int*, double*, int*, int*, Flags, State);
 void BCM_Flux_XYZ_Viscous_Runge_kutta(int, int, int, double, double, double, Flags, int*, double*,
                                       State, State, State, State, State, State, State, double*);
+void BCM_Output(int, int, int, int (*)[MPI_Ncube], State, State, double*, double (*)[X_size],
+                double (*)[Y_size], double (*)[Z_size]);
 
 // Globals the reference declares extern for its default arguments.
 int Ncube, N_wallcube, NBC, NBC_plus, NBC_minus, MPI_Nadj;
@@ -327,6 +331,9 @@ int main(int argc, char** argv) {
             }
   }
   fclose(fer);
+  // PLOT3D files of the last step, written as main.cpp:1346-1351 does after
+  // the time-level shift (into the case directory).
+  BCM_Output(myid, ncube, n_steps, rank_map, U1_, U1q, cube_size, Xcnt, Ycnt, Zcnt);
   dump("final_U1_.bin", U1_, sbytes);
   dump("final_U1.bin", U1, sbytes);
   dump("final_U1q.bin", U1q, sbytes);

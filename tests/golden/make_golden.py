@@ -13,6 +13,9 @@ with a fixed inner-iteration count, and stores:
   state, state after each of the first three RK stages, final state), the
   per-inner-iteration norm history er[1..7] as exact hex floats, and the
   run parameters;
+* ``file_sha256`` in the same JSON -- SHA-256 of the files the reference's
+  own writers leave in the case directory: the IB tables IBM_minus/plus00000.dat
+  and the PLOT3D grid and solution files BCMdata00001.x, qBCMdata<n_steps>.q;
 * ``<case>_ib.npz`` -- the IB tables the reference wrote (IBM_minus/plus),
   parsed into integer and float arrays.
 """
@@ -87,6 +90,8 @@ def make(name):
         "nbc_minus": case.info["nbc_minus"],
         "nbc_plus": case.info["nbc_plus"],
         "sha256": hashes,
+        "file_sha256": {f: sha(os.path.join(run_dir, f)) for f in (
+            "IBM_minus00000.dat", "IBM_plus00000.dat", "BCMdata00001.x", f"qBCMdata{n_steps:05d}.q")},
         "er_hex": er,
         "source": "reference sources compiled by oracle/build_ref.sh, driven by oracle/ref_driver.cpp",
     }

@@ -19,7 +19,7 @@ def test_header_declares_reference_dropins():
     names = declared_functions()
     for ref in ("BCM_X_boundary_condition", "BCM_Y_boundary_condition", "BCM_Z_boundary_condition",
                 "BCM_Ghostcell_minus", "BCM_Ghostcell_plus", "BCM_Interface_table", "BCM_Interface",
-                "BCM_Flux_XYZ_Viscous_Runge_kutta"):
+                "BCM_Flux_XYZ_Viscous_Runge_kutta", "BCM_Grid", "BCM_Reading_IBM", "BCM_Output"):
         assert ref in names
 
 

@@ -4,6 +4,8 @@ er[1..7] are block-reduced on the device in a different summation order than
 the reference's sequential sums, so they are compared at 1e-12 relative.
 """
 import ctypes
+import hashlib
+import os
 
 import numpy as np
 import pytest
@@ -199,9 +201,11 @@ def test_cfg2_one_inner_iteration_bitwise(B, O, engine):
     assert sha(engine.get_state("U1_")) == sha(orc.state["U1_"])
 
 
-def _dropin_run(B, case, n_steps, n_inner):
+def _dropin_run(B, case, n_steps, n_inner, output_dir=None):
     """Runs the reference main-loop call sequence (main.cpp:1146-1357) through
-    the drop-in BCM_* entry points on host arrays bound to an engine."""
+    the drop-in BCM_* entry points on host arrays bound to an engine; with
+    output_dir, BCM_Output writes the last step there as main.cpp:1346-1351
+    does (from the bound device state)."""
     lib = B.load_library()
     c_int, c_double = ctypes.c_int, ctypes.c_double
     P = ctypes.c_void_p
@@ -272,15 +276,30 @@ def _dropin_run(B, case, n_steps, n_inner):
                 interface()
             hist.append(er[1:8].copy())
         lib.BCM_Physical_time_shift(U, arrays["U1"].ctypes.data, arrays["U1q"].ctypes.data)
+    if output_dir is not None:
+        xyz = [np.ascontiguousarray(case.array(n)) for n in ("Xcnt", "Ycnt", "Zcnt")]
+        lib.BCM_Output.argtypes = [c_int, c_int, c_int] + [P] * 7
+        lib.BCM_Output.restype = None
+        cwd = os.getcwd()
+        os.chdir(output_dir)
+        try:
+            lib.BCM_Output(0, nc, n_steps, di, U, arrays["U1q"].ctypes.data, cs.ctypes.data,
+                           *(a.ctypes.data for a in xyz))
+        finally:
+            os.chdir(cwd)
     assert lib.bcm_sync_to_host(h) == 0
     lib.bcm_engine_destroy(h)
     return arrays, np.array(hist)
 
 
-def test_dropin_main_loop_matches_reference_fixture(B):
+def test_dropin_main_loop_matches_reference_fixture(B, tmp_path):
     g = golden("tiny")
     case = B.Case(B.case_params("tiny"))
-    arrays, hist = _dropin_run(B, case, *STEPPING["tiny"])
+    n_steps, n_inner = STEPPING["tiny"]
+    arrays, hist = _dropin_run(B, case, n_steps, n_inner, output_dir=str(tmp_path))
     for n in STATE:
         assert sha(arrays[n]) == g["sha256"][f"final_{n}.bin"], n
     assert er_close(hist, g["er"])
+    for f in ("BCMdata00001.x", f"qBCMdata{n_steps:05d}.q"):
+        with open(tmp_path / f, "rb") as fh:
+            assert hashlib.sha256(fh.read()).hexdigest() == g["file_sha256"][f], f
