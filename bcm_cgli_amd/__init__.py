@@ -85,6 +85,8 @@ def load_library(path=LIB_PATH):
     lib.bcm_engine_case_er.argtypes = [c_void_p, c_int, c_int, c_void_p]
     lib.bcm_engine_pool_info.argtypes = [c_void_p, c_void_p]
     lib.bcm_case_read.argtypes = [c_char_p, P(CaseParamsC), P(c_void_p)]
+    lib.bcm_case_build_on.argtypes = [P(CaseParamsC), c_int, P(c_void_p)]
+    lib.bcm_case_read_on.argtypes = [c_char_p, P(CaseParamsC), c_int, P(c_void_p)]
     lib.bcm_case_write_ibm.argtypes = [c_void_p, c_char_p]
     lib.bcm_case_write_plot3d.argtypes = [c_void_p, c_void_p, c_int, c_char_p]
     LL = ctypes.c_longlong
@@ -107,7 +109,10 @@ def _check(rc):
 class Case:
     """A flow case built on the host in the reference's array layouts."""
 
-    def __init__(self, params, _from_dir=None):
+    def __init__(self, params, device=None, _from_dir=None):
+        """Builds the case on the host; with ``device`` (a GPU index) the
+        quadratic passes of the IB setup run on that GPU, with bitwise the
+        same result."""
         lib = load_library()
         self.params = dict(params)
         p = CaseParamsC()
@@ -123,10 +128,15 @@ class Case:
         p.seed = int(params.get("seed", 0))
         p.noise_amp = float(params.get("noise_amp", 0.0))
         h = ctypes.c_void_p()
-        if _from_dir is None:
+        if _from_dir is None and device is None:
             _check(lib.bcm_case_build(ctypes.byref(p), ctypes.byref(h)))
-        else:
+        elif _from_dir is None:
+            _check(lib.bcm_case_build_on(ctypes.byref(p), int(device), ctypes.byref(h)))
+        elif device is None:
             _check(lib.bcm_case_read(os.fsencode(_from_dir), ctypes.byref(p), ctypes.byref(h)))
+        else:
+            _check(lib.bcm_case_read_on(os.fsencode(_from_dir), ctypes.byref(p), int(device),
+                                        ctypes.byref(h)))
         self._h = h
         info = (ctypes.c_int * 16)()
         phys = (ctypes.c_double * 8)()
@@ -139,12 +149,12 @@ class Case:
         self.ncube = self.info["ncube"]
 
     @classmethod
-    def from_files(cls, directory, params):
+    def from_files(cls, directory, params, device=None):
         """A case from the reference's input files BCMgrid00000.dat and
         BCM_STL_fine.stl in ``directory``.  ``params`` needs ``mach``,
         ``reynolds`` and ``sphere_diameter`` (the reference length of the
         viscosity) and may give ``seed`` and ``noise_amp``."""
-        return cls(params, _from_dir=directory)
+        return cls(params, device=device, _from_dir=directory)
 
     def array(self, name):
         """Numpy view of a case array (owned by the case)."""

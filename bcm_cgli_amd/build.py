@@ -34,10 +34,10 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-SOURCES = ["bcm_kernels.hip", "bcm_kernels_mixed.hip", "bcm_engine.cpp", "bcm_output.cpp",
-           "bcm_capi.cpp", "bcm_setup.cpp", "bcm_io.cpp"]
+SOURCES = ["bcm_kernels.hip", "bcm_kernels_mixed.hip", "bcm_ib.hip", "bcm_engine.cpp",
+           "bcm_output.cpp", "bcm_capi.cpp", "bcm_setup.cpp", "bcm_io.cpp"]
 # Sources that include the HIP runtime; the others are plain host C++.
-HIP_HOST = ("bcm_engine.cpp", "bcm_output.cpp", "bcm_capi.cpp")
+HIP_HOST = ("bcm_engine.cpp", "bcm_output.cpp", "bcm_capi.cpp", "bcm_setup.cpp")
 
 # The mixed-precision kernels carry no bitwise contract with the reference:
 # they let the compiler fuse multiply-adds.

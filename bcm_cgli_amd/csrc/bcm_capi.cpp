@@ -199,6 +199,28 @@ int bcm_case_build(const bcm_case_params* p, bcm_case** out) {
   return 0;
 }
 
+int bcm_case_build_on(const bcm_case_params* p, int device, bcm_case** out) {
+  if (!p || !out) return fail("null argument");
+  *out = nullptr;
+  std::unique_ptr<bcm_case> c(new bcm_case());
+  std::string err;
+  if (build_case_on_device(to_params(p), device, c->cs, err)) return fail(err);
+  finish_handle(c.get());
+  *out = c.release();
+  return 0;
+}
+
+int bcm_case_read_on(const char* dir, const bcm_case_params* p, int device, bcm_case** out) {
+  if (!dir || !p || !out) return fail("null argument");
+  *out = nullptr;
+  std::unique_ptr<bcm_case> c(new bcm_case());
+  std::string err;
+  if (read_case_on_device(dir, to_params(p), device, c->cs, err)) return fail(err);
+  finish_handle(c.get());
+  *out = c.release();
+  return 0;
+}
+
 int bcm_case_read(const char* dir, const bcm_case_params* p, bcm_case** out) {
   if (!dir || !p || !out) return fail("null argument");
   *out = nullptr;

@@ -584,9 +584,9 @@ int engine_add_case(Engine* e, const Case& cs, int* slot_out, std::string& err) 
                              (size_t)n * CUBE_STRIDE * sizeof(double), e->stream));
   for (int c = first; c < first + n; ++c) e->cube_case_host[c] = slot;
   if (rebuild_tables(e, err)) return 1;
-  const std::vector<double>* st[3] = {&cs.U1_, &cs.U1, &cs.U1q};
+  const double* st[3] = {cs.U1_.data(), cs.U1.data(), cs.U1q.data()};
   for (int w = 0; w < 3; ++w)
-    if (engine_case_state(e, slot, w, const_cast<double*>(st[w]->data()), true, err)) return 1;
+    if (engine_case_state(e, slot, w, const_cast<double*>(st[w]), true, err)) return 1;
   e->halo_full_next = true;
   if (slot_out) *slot_out = slot;
   return 0;

@@ -254,7 +254,8 @@ int read_ibm(const std::string& path, IBTable& tb, std::string& err) {
 // (p.sphere_diameter), seed and noise amplitude; the remaining setup
 // (boundary lists, interface tables, flags, IB tables, initial state) runs as
 // for a generated case.
-int read_case(const std::string& dir, const CaseParams& p, Case& cs, std::string& err) {
+int read_case(const std::string& dir, const CaseParams& p, Case& cs, std::string& err,
+              IbDevice* dev) {
   cs = Case();
   cs.p = p;
   if (!(p.mach > 0) || !(p.reynolds > 0) || !(p.sphere_diameter > 0)) {
@@ -266,7 +267,7 @@ int read_case(const std::string& dir, const CaseParams& p, Case& cs, std::string
   if (read_stl(dir + "/BCM_STL_fine.stl", cs.stl, err)) return 1;
   grid_geometry(cs);
   cs.fws_grid = cs.fws;
-  return finish_setup(cs, err);
+  return finish_setup(cs, err, dev);
 }
 
 // PLOT3D multi-block grid and solution files as BCM_Output.cpp:86-301 writes

@@ -3,6 +3,7 @@
 // the device are the ones the reference computes for the same grid and
 // surface.  Citations are file:line in the reference tree.
 #include "bcm_setup.h"
+#include "bcm_geom.h"
 
 #include <math.h>
 #include <stdio.h>
@@ -18,6 +19,28 @@
 namespace bcm {
 
 namespace {
+
+// Runs fn(i) for i in [0, n) on up to 16 host threads (fewer when
+// OMP_NUM_THREADS says so).  Every use writes disjoint data per i, so the
+// result does not depend on the schedule.
+template <typename F>
+void parallel_for(int n, F fn) {
+  int nth = (int)std::thread::hardware_concurrency();
+  if (const char* env = getenv("OMP_NUM_THREADS")) nth = std::min(nth, atoi(env));
+  nth = std::max(1, std::min(nth, 16));
+  if (n <= 1 || nth == 1) {
+    for (int i = 0; i < n; ++i) fn(i);
+    return;
+  }
+  nth = std::min(nth, n);
+  std::atomic<int> next(0);
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nth; ++t)
+    pool.emplace_back([&]() {
+      for (int i; (i = next.fetch_add(1)) < n;) fn(i);
+    });
+  for (auto& th : pool) th.join();
+}
 
 inline double dmin(double a, double b) { return ((a) < (b)) ? (a) : (b); }
 inline double dmax(double a, double b) { return ((a) > (b)) ? (a) : (b); }
@@ -159,8 +182,8 @@ void host_interface_copy(int ncube, const int* const* lists, const int* nadj,
   auto at = [&](int c, int i, int j, int k, int v) -> double& {
     return U[aos_index(c, i, j, k, v)];
   };
-  for (int a = 1; a <= nadj[L_X_EQ]; ++a) {
-    int c0 = lists[L_X_EQ][a], c2 = adj[adj_index(c0, 1, 2)];
+  parallel_for(nadj[L_X_EQ], [&](int a) {
+    int c0 = lists[L_X_EQ][a + 1], c2 = adj[adj_index(c0, 1, 2)];
     for (int i = 0; i < NB; ++i)
       for (int j = ILO; j <= IHI; ++j)
         for (int k = ILO; k <= IHI; ++k)
@@ -168,9 +191,9 @@ void host_interface_copy(int ncube, const int* const* lists, const int* nadj,
             at(c2, i, j, k, v) = at(c0, NC + i, j, k, v);
             at(c0, NB + NC + i, j, k, v) = at(c2, NB + i, j, k, v);
           }
-  }
-  for (int a = 1; a <= nadj[L_Y_EQ]; ++a) {
-    int c0 = lists[L_Y_EQ][a], c2 = adj[adj_index(c0, 1, 4)];
+  });
+  parallel_for(nadj[L_Y_EQ], [&](int a) {
+    int c0 = lists[L_Y_EQ][a + 1], c2 = adj[adj_index(c0, 1, 4)];
     for (int i = ILO; i <= IHI; ++i)
       for (int j = 0; j < NB; ++j)
         for (int k = ILO; k <= IHI; ++k)
@@ -178,9 +201,9 @@ void host_interface_copy(int ncube, const int* const* lists, const int* nadj,
             at(c2, i, j, k, v) = at(c0, i, NC + j, k, v);
             at(c0, i, NB + NC + j, k, v) = at(c2, i, NB + j, k, v);
           }
-  }
-  for (int a = 1; a <= nadj[L_Z_EQ]; ++a) {
-    int c0 = lists[L_Z_EQ][a], c2 = adj[adj_index(c0, 1, 6)];
+  });
+  parallel_for(nadj[L_Z_EQ], [&](int a) {
+    int c0 = lists[L_Z_EQ][a + 1], c2 = adj[adj_index(c0, 1, 6)];
     for (int i = ILO; i <= IHI; ++i)
       for (int j = ILO; j <= IHI; ++j)
         for (int k = 0; k < NB; ++k)
@@ -188,9 +211,10 @@ void host_interface_copy(int ncube, const int* const* lists, const int* nadj,
             at(c2, i, j, k, v) = at(c0, i, j, NC + k, v);
             at(c0, i, j, NB + NC + k, v) = at(c2, i, j, NB + k, v);
           }
-  }
+  });
   const int L1 = NB - 1, H1 = IHI + 1;  // layers 1 and 18
-  for (int c = 1; c < ncube; ++c) {
+  parallel_for(ncube - 1, [&](int c0) {
+    const int c = c0 + 1;
     for (int i = ILO; i <= IHI; ++i)
       for (int v = 0; v < NV; ++v) {
         at(c, i, L1, L1, v) = 0.5 * (at(c, i, L1, ILO, v) + at(c, i, ILO, L1, v));
@@ -212,7 +236,7 @@ void host_interface_copy(int ncube, const int* const* lists, const int* nadj,
         at(c, H1, L1, k, v) = 0.5 * (at(c, IHI, L1, k, v) + at(c, H1, ILO, k, v));
         at(c, H1, H1, k, v) = 0.5 * (at(c, IHI, H1, k, v) + at(c, H1, IHI, k, v));
       }
-  }
+  });
 }
 
 namespace {
@@ -226,41 +250,43 @@ namespace {
 void fws_interface(Case& cs) {
   const int ncube = cs.ncube;
   std::vector<double> tmp((size_t)ncube * CELLS_PADDED, 0.0);
-  for (int c = 1; c < ncube; ++c)
+  parallel_for(ncube - 1, [&](int c0) {
+    const size_t c = c0 + 1;
     if (cs.csl[c] == 0)
-      for (int n = 0; n < CELLS_PADDED; ++n)
-        tmp[(size_t)c * CELLS_PADDED + n] = cs.fws[(size_t)c * CELLS_PADDED + n];
+      for (int n = 0; n < CELLS_PADDED; ++n) tmp[c * CELLS_PADDED + n] = cs.fws[c * CELLS_PADDED + n];
+  });
   auto T = [&](int c, int i, int j, int k) -> double& { return tmp[flag_index(c, i, j, k)]; };
   const int* adj = cs.adj_number.data();
-  for (int a = 1; a <= cs.nadj[L_X_EQ]; ++a) {
-    int c0 = cs.adj[L_X_EQ][a], c2 = adj[adj_index(c0, 1, 2)];
+  parallel_for(cs.nadj[L_X_EQ], [&](int a) {
+    int c0 = cs.adj[L_X_EQ][a + 1], c2 = adj[adj_index(c0, 1, 2)];
     for (int i = 0; i < NB; ++i)
       for (int j = ILO; j <= IHI; ++j)
         for (int k = ILO; k <= IHI; ++k) {
           T(c2, i, j, k) = T(c0, NC + i, j, k);
           T(c0, NB + NC + i, j, k) = T(c2, NB + i, j, k);
         }
-  }
-  for (int a = 1; a <= cs.nadj[L_Y_EQ]; ++a) {
-    int c0 = cs.adj[L_Y_EQ][a], c2 = adj[adj_index(c0, 1, 4)];
+  });
+  parallel_for(cs.nadj[L_Y_EQ], [&](int a) {
+    int c0 = cs.adj[L_Y_EQ][a + 1], c2 = adj[adj_index(c0, 1, 4)];
     for (int i = ILO; i <= IHI; ++i)
       for (int j = 0; j < NB; ++j)
         for (int k = ILO; k <= IHI; ++k) {
           T(c2, i, j, k) = T(c0, i, NC + j, k);
           T(c0, i, NB + NC + j, k) = T(c2, i, NB + j, k);
         }
-  }
-  for (int a = 1; a <= cs.nadj[L_Z_EQ]; ++a) {
-    int c0 = cs.adj[L_Z_EQ][a], c2 = adj[adj_index(c0, 1, 6)];
+  });
+  parallel_for(cs.nadj[L_Z_EQ], [&](int a) {
+    int c0 = cs.adj[L_Z_EQ][a + 1], c2 = adj[adj_index(c0, 1, 6)];
     for (int i = ILO; i <= IHI; ++i)
       for (int j = ILO; j <= IHI; ++j)
         for (int k = 0; k < NB; ++k) {
           T(c2, i, j, k) = T(c0, i, j, NC + k);
           T(c0, i, j, NB + NC + k) = T(c2, i, j, NB + k);
         }
-  }
-  for (int c = 1; c < ncube; ++c) {
-    if (cs.csl[c] != 0) continue;
+  });
+  parallel_for(ncube - 1, [&](int c0) {
+    const int c = c0 + 1;
+    if (cs.csl[c] != 0) return;
     for (int n = 0; n < CELLS_PADDED; ++n) {
       double t = tmp[(size_t)c * CELLS_PADDED + n];
       int& f = cs.fws[(size_t)c * CELLS_PADDED + n];
@@ -268,9 +294,7 @@ void fws_interface(Case& cs) {
       else if (t < -0.5) f = -1;
       else f = 0;
     }
-  }
-  for (int c = 1; c < ncube; ++c) {
-    if (cs.csl[c] != 0) continue;
+    // Hole treatment of the cube (reads and writes only this cube).
     auto F = [&](int i, int j, int k) -> int& { return cs.fws[flag_index(c, i, j, k)]; };
     bool changed = true;
     while (changed) {
@@ -287,80 +311,7 @@ void fws_interface(Case& cs) {
             }
           }
     }
-  }
-}
-
-// Möller–Trumbore ray/triangle test as used by the reference
-// (BCM_Immersed_boundary.cpp:1437-1495): back faces and degenerate triangles
-// are rejected by det < 1e-7, no sign test on t.
-int intersect_triangle(const double orig[3], const double dir[3], const double v0[3],
-                       const double v1[3], const double v2[3], double* t, double* u,
-                       double* v) {
-  double e1[3], e2[3], tv[3], pv[3], qv[3];
-  for (int d = 0; d < 3; ++d) { e1[d] = v1[d] - v0[d]; e2[d] = v2[d] - v0[d]; }
-  pv[0] = dir[1] * e2[2] - dir[2] * e2[1];
-  pv[1] = dir[2] * e2[0] - dir[0] * e2[2];
-  pv[2] = dir[0] * e2[1] - dir[1] * e2[0];
-  double det = e1[0] * pv[0] + e1[1] * pv[1] + e1[2] * pv[2];
-  if (det < SMALLNUM) return 0;
-  for (int d = 0; d < 3; ++d) tv[d] = orig[d] - v0[d];
-  *u = tv[0] * pv[0] + tv[1] * pv[1] + tv[2] * pv[2];
-  if (*u < 0.0 || *u > det) return 0;
-  qv[0] = tv[1] * e1[2] - tv[2] * e1[1];
-  qv[1] = tv[2] * e1[0] - tv[0] * e1[2];
-  qv[2] = tv[0] * e1[1] - tv[1] * e1[0];
-  *v = dir[0] * qv[0] + dir[1] * qv[1] + dir[2] * qv[2];
-  if (*v < 0.0 || *u + *v > det) return 0;
-  *t = e2[0] * qv[0] + e2[1] * qv[1] + e2[2] * qv[2];
-  double inv_det = 1.0 / det;
-  *t *= inv_det;
-  *u *= inv_det;
-  *v *= inv_det;
-  return 1;
-}
-
-// Closest point on a triangle (Ericson's region tests) as restated from
-// BCM_Immersed_boundary.cpp:1286-1432, including the interior-region
-// expression w = vc + denom of :1425.
-void closest_point(const double a[3], const double b[3], const double c[3],
-                   const double p[3], double out[3]) {
-  double ab[3], ac[3], bc[3], ap[3], bp[3], cp[3];
-  for (int d = 0; d < 3; ++d) {
-    ab[d] = b[d] - a[d]; ac[d] = c[d] - a[d]; bc[d] = c[d] - b[d]; ap[d] = p[d] - a[d];
-  }
-  double d1 = ab[0] * ap[0] + ab[1] * ap[1] + ab[2] * ap[2];
-  double d2 = ac[0] * ap[0] + ac[1] * ap[1] + ac[2] * ap[2];
-  if (d1 <= 0. && d2 <= 0.) { for (int d = 0; d < 3; ++d) out[d] = a[d]; return; }
-  for (int d = 0; d < 3; ++d) bp[d] = p[d] - b[d];
-  double d3 = ab[0] * bp[0] + ab[1] * bp[1] + ab[2] * bp[2];
-  double d4 = ac[0] * bp[0] + ac[1] * bp[1] + ac[2] * bp[2];
-  if (d3 >= 0. && d4 <= d3) { for (int d = 0; d < 3; ++d) out[d] = b[d]; return; }
-  double vc = d1 * d4 - d3 * d2;
-  if (vc <= 0. && d1 >= 0. && d3 <= 0.) {
-    double v = d1 / (d1 - d3);
-    for (int d = 0; d < 3; ++d) out[d] = a[d] + v * ab[d];
-    return;
-  }
-  for (int d = 0; d < 3; ++d) cp[d] = p[d] - c[d];
-  double d5 = ab[0] * cp[0] + ab[1] * cp[1] + ab[2] * cp[2];
-  double d6 = ac[0] * cp[0] + ac[1] * cp[1] + ac[2] * cp[2];
-  if (d6 >= 0. && d5 <= d6) { for (int d = 0; d < 3; ++d) out[d] = c[d]; return; }
-  double vb = d5 * d2 - d1 * d6;
-  if (vb <= 0. && d2 >= 0. && d6 <= 0.) {
-    double w = d2 / (d2 - d6);
-    for (int d = 0; d < 3; ++d) out[d] = a[d] + w * ac[d];
-    return;
-  }
-  double va = d3 * d6 - d5 * d4;
-  if (va <= 0. && (d4 - d3) >= 0. && (d5 - d6) >= 0.) {
-    double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
-    for (int d = 0; d < 3; ++d) out[d] = b[d] + w * bc[d];
-    return;
-  }
-  double denom = 1. / (va + vb + vc);
-  double v = vb * denom;
-  double w = vc + denom;
-  for (int d = 0; d < 3; ++d) out[d] = a[d] + ab[d] * v + ac[d] * w;
+  });
 }
 
 // Gauss-Jordan inverse with partial pivoting in the operation order of
@@ -406,28 +357,115 @@ struct GhostHit {
   double tmin, dir[3], dotp, bip[3];
 };
 
-// Image-point construction and trilinear weights for every ghost cell,
-// BCM_Immersed_boundary.cpp:127-1081.
-void immersed_boundary(Case& cs) {
-  const int ncube = cs.ncube;
-  // Triangles with any vertex inside some cube box padded by 1.5 cells (:380-470).
-  std::vector<Triangle> tri;
-  for (const Triangle& T : cs.stl) {
-    bool keep = false;
-    for (int m = 0; m < 3 && !keep; ++m) {
+}  // namespace
+
+void surface_tri(const Triangle& T, SurfaceTri& s) {
+  for (int d = 0; d < 3; ++d) {
+    s.v0[d] = T.v[0][d];
+    s.v1[d] = T.v[1][d];
+    s.v2[d] = T.v[2][d];
+  }
+  double v12[3], v13[3];
+  for (int d = 0; d < 3; ++d) { v12[d] = T.v[1][d] - T.v[0][d]; v13[d] = T.v[2][d] - T.v[0][d]; }
+  s.nu[0] = v12[1] * v13[2] - v12[2] * v13[1];
+  s.nu[1] = v12[2] * v13[0] - v12[0] * v13[2];
+  s.nu[2] = v12[0] * v13[1] - v12[1] * v13[0];
+  const double dis = sqrt(s.nu[0] * s.nu[0] + s.nu[1] * s.nu[1] + s.nu[2] * s.nu[2]);
+  s.dir[0] = -s.nu[0] / dis;
+  s.dir[1] = -s.nu[1] / dis;
+  s.dir[2] = -s.nu[2] / dis;
+  // The edges and the triangle-only part of intersect_triangle.
+  for (int d = 0; d < 3; ++d) { s.e1[d] = T.v[1][d] - T.v[0][d]; s.e2[d] = T.v[2][d] - T.v[0][d]; }
+  s.pv[0] = s.dir[1] * s.e2[2] - s.dir[2] * s.e2[1];
+  s.pv[1] = s.dir[2] * s.e2[0] - s.dir[0] * s.e2[2];
+  s.pv[2] = s.dir[0] * s.e2[1] - s.dir[1] * s.e2[0];
+  s.det = s.e1[0] * s.pv[0] + s.e1[1] * s.pv[1] + s.e1[2] * s.pv[2];
+  s.inv_det = 1.0 / s.det;
+  s.dd = s.dir[0] * s.dir[0] + s.dir[1] * s.dir[1] + s.dir[2] * s.dir[2];
+}
+
+namespace {
+
+// Triangles with any vertex inside some cube box padded by 1.5 cells
+// (BCM_Immersed_boundary.cpp:380-470), on the host.
+void filter_host(const Case& cs, std::vector<uint8_t>& keep) {
+  keep.assign(cs.stl.size(), 0);
+  for (size_t t = 0; t < cs.stl.size(); ++t) {
+    const Triangle& T = cs.stl[t];
+    bool k = false;
+    for (int m = 0; m < 3 && !k; ++m) {
       const double* pv = T.v[m];
-      for (int c = 1; c < ncube; ++c) {
+      for (int c = 1; c < cs.ncube; ++c) {
         double dx = cs.cube_size[c] / NC, dy = dx, dz = dx;
         if (pv[0] >= cs.Xcube[c] - 1.5 * dx && pv[0] <= cs.Xcube[c] + cs.cube_size[c] + 1.5 * dx &&
             pv[1] >= cs.Ycube[c] - 1.5 * dy && pv[1] <= cs.Ycube[c] + cs.cube_size[c] + 1.5 * dy &&
             pv[2] >= cs.Zcube[c] - 1.5 * dz && pv[2] <= cs.Zcube[c] + cs.cube_size[c] + 1.5 * dz) {
-          keep = true;
+          k = true;
           break;
         }
       }
     }
-    if (keep) tri.push_back(T);
+    keep[t] = k;
   }
+}
+
+// Surface search of one ghost cell over all kept triangles
+// (BCM_Immersed_boundary.cpp:722-860): the ray along each triangle's inward
+// normal with the smallest |t|, else the closest point on any triangle.
+void search_host(const std::vector<SurfaceTri>& tri, const double orig[3], GhostHit& h) {
+  h.hit = false;
+  h.tmin = BIGNUM;
+  h.dotp = 0.0;
+  const int Ntri = (int)tri.size();
+  double ndis_min = BIGNUM;
+  for (int it = 0; it < Ntri; ++it) {
+    const SurfaceTri& T = tri[it];
+    double t = 0, u, v;
+    int f = intersect_triangle(orig, T.dir, T.v0, T.v1, T.v2, &t, &u, &v);
+    double ndis = fabs(t) * T.dd;
+    if (f == 1) {
+      h.hit = true;
+      if (ndis < ndis_min) {
+        ndis_min = ndis;
+        h.tmin = fabs(t);
+        h.dir[0] = T.dir[0]; h.dir[1] = T.dir[1]; h.dir[2] = T.dir[2];
+        h.dotp = T.nu[0] * (T.v0[0] - orig[0]) + T.nu[1] * (T.v0[1] - orig[1]) +
+                 T.nu[2] * (T.v0[2] - orig[2]);
+      }
+    }
+  }
+  if (!h.hit) {
+    ndis_min = BIGNUM;
+    for (int it = 0; it < Ntri; ++it) {
+      double p[3];
+      closest_point(tri[it].v0, tri[it].v1, tri[it].v2, orig, p);
+      double nd = (p[0] - orig[0]) * (p[0] - orig[0]) + (p[1] - orig[1]) * (p[1] - orig[1]) +
+                  (p[2] - orig[2]) * (p[2] - orig[2]);
+      if (nd <= ndis_min) {
+        h.bip[0] = p[0]; h.bip[1] = p[1]; h.bip[2] = p[2];
+        ndis_min = nd;
+      }
+    }
+  }
+}
+
+// Image-point construction and trilinear weights for every ghost cell,
+// BCM_Immersed_boundary.cpp:127-1081.
+int immersed_boundary(Case& cs, IbDevice* dev, std::string& err) {
+  const int ncube = cs.ncube;
+  std::vector<uint8_t> keep;
+  if (dev) {
+    if (dev->filter(cs, keep, err)) return 1;
+  } else {
+    filter_host(cs, keep);
+  }
+  std::vector<SurfaceTri> tri;
+  for (size_t t = 0; t < cs.stl.size(); ++t)
+    if (keep[t]) {
+      SurfaceTri s;
+      surface_tri(cs.stl[t], s);
+      tri.push_back(s);
+    }
   const int Ntri = (int)tri.size();
   // Ghost cells of finest cubes, interior only, cube-i-j-k order (:500-545).
   struct GC { int c, i, j, k; double x, y, z; };
@@ -443,56 +481,38 @@ void immersed_boundary(Case& cs) {
   cs.n_ghost_candidates = (int)gcs.size();
   const int Ngc = (int)gcs.size();
   std::vector<GhostHit> hits(Ngc);
-  // Surface search per ghost cell; independent across ghost cells, so the
-  // cells are split over host threads.
-  auto search = [&](int g) {
-    GhostHit& h = hits[g];
-    h.hit = false;
-    h.tmin = BIGNUM;
-    h.dotp = 0.0;
-    if (Ntri == 0) return;
-    const double orig[3] = {gcs[g].x, gcs[g].y, gcs[g].z};
-    double ndis_min = BIGNUM;
-    for (int it = 0; it < Ntri; ++it) {
-      const Triangle& T = tri[it];
-      double v12[3], v13[3], nu[3], dir[3];
-      for (int d = 0; d < 3; ++d) { v12[d] = T.v[1][d] - T.v[0][d]; v13[d] = T.v[2][d] - T.v[0][d]; }
-      nu[0] = v12[1] * v13[2] - v12[2] * v13[1];
-      nu[1] = v12[2] * v13[0] - v12[0] * v13[2];
-      nu[2] = v12[0] * v13[1] - v12[1] * v13[0];
-      double dis = sqrt(nu[0] * nu[0] + nu[1] * nu[1] + nu[2] * nu[2]);
-      dir[0] = -nu[0] / dis;
-      dir[1] = -nu[1] / dis;
-      dir[2] = -nu[2] / dis;
-      double t = 0, u, v;
-      int f = intersect_triangle(orig, dir, T.v[0], T.v[1], T.v[2], &t, &u, &v);
-      double ndis = fabs(t) * (dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
-      if (f == 1) {
-        h.hit = true;
-        if (ndis < ndis_min) {
-          ndis_min = ndis;
-          h.tmin = fabs(t);
-          h.dir[0] = dir[0]; h.dir[1] = dir[1]; h.dir[2] = dir[2];
-          h.dotp = nu[0] * (T.v[0][0] - orig[0]) + nu[1] * (T.v[0][1] - orig[1]) +
-                   nu[2] * (T.v[0][2] - orig[2]);
-        }
+  if (dev && Ntri > 0 && Ngc > 0) {
+    // The device names the winning triangles; their values are recomputed
+    // here exactly as the host search computes them.
+    std::vector<double> xyz((size_t)3 * Ngc);
+    for (int g = 0; g < Ngc; ++g) {
+      xyz[3 * g] = gcs[g].x;
+      xyz[3 * g + 1] = gcs[g].y;
+      xyz[3 * g + 2] = gcs[g].z;
+    }
+    std::vector<int> hit_any, hit_tri, near_tri;
+    if (dev->search(tri, xyz, hit_any, hit_tri, near_tri, err)) return 1;
+    for (int g = 0; g < Ngc; ++g) {
+      GhostHit& h = hits[g];
+      h.hit = hit_any[g] != 0;
+      h.tmin = BIGNUM;
+      h.dotp = 0.0;
+      const double orig[3] = {gcs[g].x, gcs[g].y, gcs[g].z};
+      if (h.hit && hit_tri[g] >= 0) {
+        const SurfaceTri& T = tri[hit_tri[g]];
+        double t = 0, u, v;
+        intersect_triangle(orig, T.dir, T.v0, T.v1, T.v2, &t, &u, &v);
+        h.tmin = fabs(t);
+        h.dir[0] = T.dir[0]; h.dir[1] = T.dir[1]; h.dir[2] = T.dir[2];
+        h.dotp = T.nu[0] * (T.v0[0] - orig[0]) + T.nu[1] * (T.v0[1] - orig[1]) +
+                 T.nu[2] * (T.v0[2] - orig[2]);
+      } else if (!h.hit && near_tri[g] >= 0) {
+        const SurfaceTri& T = tri[near_tri[g]];
+        closest_point(T.v0, T.v1, T.v2, orig, h.bip);
       }
     }
-    if (!h.hit) {
-      ndis_min = BIGNUM;
-      for (int it = 0; it < Ntri; ++it) {
-        double p[3];
-        closest_point(tri[it].v[0], tri[it].v[1], tri[it].v[2], orig, p);
-        double nd = (p[0] - orig[0]) * (p[0] - orig[0]) + (p[1] - orig[1]) * (p[1] - orig[1]) +
-                    (p[2] - orig[2]) * (p[2] - orig[2]);
-        if (nd <= ndis_min) {
-          h.bip[0] = p[0]; h.bip[1] = p[1]; h.bip[2] = p[2];
-          ndis_min = nd;
-        }
-      }
-    }
-  };
-  {
+  } else if (Ntri > 0) {
+    // Independent across ghost cells, so the cells are split over host threads.
     int nth = (int)std::thread::hardware_concurrency();
     if (const char* env = getenv("OMP_NUM_THREADS")) nth = std::min(nth, atoi(env));
     if (nth < 1) nth = 1;
@@ -502,7 +522,10 @@ void immersed_boundary(Case& cs) {
     for (int t = 0; t < nth; ++t)
       pool.emplace_back([&]() {
         for (int g; (g = next.fetch_add(16)) < Ngc;)
-          for (int q = g; q < std::min(g + 16, Ngc); ++q) search(q);
+          for (int q = g; q < std::min(g + 16, Ngc); ++q) {
+            const double orig[3] = {gcs[q].x, gcs[q].y, gcs[q].z};
+            search_host(tri, orig, hits[q]);
+          }
       });
     for (auto& th : pool) th.join();
   }
@@ -597,12 +620,13 @@ void immersed_boundary(Case& cs) {
     for (int m = 0; m < 4; ++m) { tb.gcindex.push_back(gidx[m]); tb.ipsur.push_back(pidx[m]); }
     for (int vi = 0; vi < 8; ++vi) tb.weight.push_back(round_f6(w[vi]));
   }
+  return 0;
 }
 
 }  // namespace
 
 // ---------------------------------------------------------------------------
-int build_case(const CaseParams& p, Case& cs, std::string& err) {
+int build_case(const CaseParams& p, Case& cs, std::string& err, IbDevice* dev) {
   cs = Case();
   cs.p = p;
   if (p.lattice[0] < 1 || p.lattice[1] < 1 || p.lattice[2] < 1) {
@@ -652,8 +676,9 @@ int build_case(const CaseParams& p, Case& cs, std::string& err) {
   cs.fws.assign((size_t)ncube * CELLS_PADDED, IFLUID);
   for (int n = 0; n < CELLS_PADDED; ++n) cs.fws[n] = 0;  // slot 0 is unused
   const double r = 0.5 * p.sphere_diameter;
-  cs.wallcube.assign(1, 0);
-  for (int c = 1; c < ncube; ++c) {
+  std::vector<uint8_t> is_wall(ncube, 0);
+  parallel_for(ncube - 1, [&](int c0) {
+    const int c = c0 + 1;
     bool wall = false;
     for (int i = ILO; i <= IHI; ++i)
       for (int j = ILO; j <= IHI; ++j)
@@ -666,12 +691,15 @@ int build_case(const CaseParams& p, Case& cs, std::string& err) {
             wall = true;
           }
         }
-    if (wall) cs.wallcube.push_back(c);
-  }
+    is_wall[c] = wall;
+  });
+  cs.wallcube.assign(1, 0);
+  for (int c = 1; c < ncube; ++c)
+    if (is_wall[c]) cs.wallcube.push_back(c);
   cs.n_wallcube = (int)cs.wallcube.size();
   cs.fws_grid = cs.fws;
   make_icosphere(p.ico_level, p.sphere_center, r, cs.stl);
-  return finish_setup(cs, err);
+  return finish_setup(cs, err, dev);
 }
 
 // Grid geometry derived from cube sizes and corners: cell centres
@@ -698,7 +726,7 @@ void grid_geometry(Case& cs) {
 
 // Everything after the grid (cube geometry, adjacency, grid flags, wall
 // cubes) and the surface triangles are known: main.cpp:108-1139.
-int finish_setup(Case& cs, std::string& err) {
+int finish_setup(Case& cs, std::string& err, IbDevice* dev) {
   const CaseParams& p = cs.p;
   const int ncube = cs.ncube;
 
@@ -742,12 +770,18 @@ int finish_setup(Case& cs, std::string& err) {
   // Initial state, BCM_Initial_condition.cpp:46-86, then seeded density noise
   // on fluid cells, then U1_ = U1q = U1 (main.cpp:661-685).
   const Physics& ph = cs.phys;
-  cs.U1.assign((size_t)ncube * CELLS_PADDED * NV, 0.0);
-  for (int c = 1; c < ncube; ++c)
+  const size_t nstate = (size_t)ncube * CELLS_PADDED * NV;
+  cs.U1.resize(nstate);
+  cs.U1_.resize(nstate);
+  cs.U1q.resize(nstate);
+  for (StateVec* v : {&cs.U1, &cs.U1_, &cs.U1q})
+    std::fill(v->begin(), v->begin() + (size_t)CELLS_PADDED * NV, 0.0);  // cube 0
+  parallel_for(ncube - 1, [&](int c0) {
+    const int c = c0 + 1;
     for (int i = 0; i < NP; ++i)
       for (int j = 0; j < NP; ++j)
         for (int k = 0; k < NP; ++k) {
-          double* u = &cs.U1[aos_index(c, i, j, k, 0)];
+          double u[NV];
           int f = cs.fws[flag_index(c, i, j, k)];
           if (f > ISOLID) {
             double rho = ph.rho0, U = ph.U0, V = ph.V0, W = ph.W0;
@@ -763,15 +797,17 @@ int finish_setup(Case& cs, std::string& err) {
             double fac = 1.0 + p.noise_amp * noise_unit(p.seed, c, i, j, k);
             u[0] = u[0] * fac;
           }
+          const size_t o = aos_index(c, i, j, k, 0);
+          for (int v = 0; v < NV; ++v) cs.U1[o + v] = cs.U1_[o + v] = cs.U1q[o + v] = u[v];
         }
-  cs.U1_ = cs.U1;
-  cs.U1q = cs.U1;
+  });
 
   // Ghost detection on finest cubes, main.cpp:694-803.
   {
     std::vector<int> tmp = cs.fws;
-    for (int c = 1; c < ncube; ++c) {
-      if (cs.csl[c] != 0) continue;
+    parallel_for(ncube - 1, [&](int c0) {
+      const int c = c0 + 1;
+      if (cs.csl[c] != 0) return;
       auto F = [&](int i, int j, int k) { return cs.fws[flag_index(c, i, j, k)]; };
       for (int i = 1; i <= IHI + 1; ++i)
         for (int j = 1; j <= IHI + 1; ++j)
@@ -780,16 +816,17 @@ int finish_setup(Case& cs, std::string& err) {
                 (F(i + 1, j, k) == IFLUID || F(i - 1, j, k) == IFLUID || F(i, j - 1, k) == IFLUID ||
                  F(i, j + 1, k) == IFLUID || F(i, j, k - 1) == IFLUID || F(i, j, k + 1) == IFLUID))
               tmp[flag_index(c, i, j, k)] = -2;
-    }
-    for (int c = 1; c < ncube; ++c) {
-      if (cs.csl[c] != 0) continue;
+    });
+    parallel_for(ncube - 1, [&](int c0) {
+      const int c = c0 + 1;
+      if (cs.csl[c] != 0) return;
       for (int n = 0; n < CELLS_PADDED; ++n)
         if (tmp[(size_t)c * CELLS_PADDED + n] == -2) cs.fws[(size_t)c * CELLS_PADDED + n] = IGHOST;
-    }
+    });
   }
 
   // IB tables (main.cpp:811).
-  immersed_boundary(cs);
+  if (immersed_boundary(cs, dev, err)) return 1;
 
   int* lp[L_COUNT];
   for (int l = 0; l < L_COUNT; ++l) lp[l] = cs.adj[l].data();

@@ -4,7 +4,11 @@
 #pragma once
 
 #include <stdint.h>
+
+#include <memory>
+#include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "bcm_layout.h"
@@ -46,6 +50,29 @@ struct Triangle {
   double v[3][3];   // vertices
 };
 
+// Allocator that leaves new elements uninitialised: the state arrays are
+// written in full by the setup's parallel passes, so the first touch of
+// their pages happens on the threads that fill them.
+template <typename T>
+struct NoInitAlloc : std::allocator<T> {
+  template <typename U>
+  struct rebind {
+    using other = NoInitAlloc<U>;
+  };
+  NoInitAlloc() = default;
+  template <typename U>
+  NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+  template <typename U>
+  void construct(U* p) noexcept {
+    ::new (static_cast<void*>(p)) U;
+  }
+  template <typename U, typename... A>
+  void construct(U* p, A&&... a) {
+    ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+  }
+};
+using StateVec = std::vector<double, NoInitAlloc<double>>;
+
 struct Case {
   CaseParams p;
   Physics phys;
@@ -65,17 +92,54 @@ struct Case {
   IBTable ib_minus, ib_plus;
   int n_ghost_candidates = 0;          // ghost cells examined by the IB setup
   // State arrays, AoS [ncube][20][20][20][5].
-  std::vector<double> U1_, U1, U1q;
+  StateVec U1_, U1, U1q;
+};
+
+// A surface triangle with the quantities of the ray test that depend on the
+// triangle alone (BCM_Immersed_boundary.cpp:722-750, 1437-1495), computed on
+// the host in the reference's operation order.
+struct SurfaceTri {
+  double v0[3], v1[3], v2[3];
+  double e1[3], e2[3];  // v1 - v0, v2 - v0
+  double nu[3];         // e1 x e2 (unnormalised normal)
+  double dir[3];        // -nu / |nu|: the ray direction
+  double pv[3];         // dir x e2
+  double det, inv_det;  // e1 . pv and 1 / det
+  double dd;            // dir . dir
+};
+void surface_tri(const Triangle& T, SurfaceTri& s);
+
+// Device for the two quadratic passes of the IB setup.  Both must return the
+// decisions of the host loops exactly:
+// * filter: keep[t] = 1 when a vertex of stl[t] lies in some cube's box
+//   padded by 1.5 cells (BCM_Immersed_boundary.cpp:380-470);
+// * search: per ghost-cell centre xyz[3g..3g+2], hit_any[g] = 1 when some
+//   triangle passes the ray test, hit_tri[g] = the first triangle of smallest
+//   |t| (dir . dir) below 1e8 (or -1); for cells with no hit, near_tri[g] = the
+//   last triangle of smallest squared distance to its closest point at or
+//   below 1e8 (or -1).
+struct IbDevice {
+  virtual ~IbDevice() {}
+  virtual int filter(const Case& cs, std::vector<uint8_t>& keep, std::string& err) = 0;
+  virtual int search(const std::vector<SurfaceTri>& tri, const std::vector<double>& xyz,
+                     std::vector<int>& hit_any, std::vector<int>& hit_tri,
+                     std::vector<int>& near_tri, std::string& err) = 0;
 };
 
 // Builds the whole case.  Returns 0 on success, non-zero with a message in err.
-int build_case(const CaseParams& p, Case& cs, std::string& err);
+// With dev, the triangle filter and the surface search of the IB setup run
+// there; the case is the same either way.
+int build_case(const CaseParams& p, Case& cs, std::string& err, IbDevice* dev = nullptr);
+// build_case and read_case with the IB search on a GPU (bcm_ib.hip).
+int build_case_on_device(const CaseParams& p, int device, Case& cs, std::string& err);
+int read_case_on_device(const std::string& dir, const CaseParams& p, int device, Case& cs,
+                        std::string& err);
 
 // Cell centres and size levels from cube sizes and corners.
 void grid_geometry(Case& cs);
 // The setup passes that follow the grid and the surface: boundary lists,
 // interface tables, flags, initial state, ghost detection, IB tables.
-int finish_setup(Case& cs, std::string& err);
+int finish_setup(Case& cs, std::string& err, IbDevice* dev = nullptr);
 
 // Individual passes, exposed for tests.
 void compute_physics(const CaseParams& p, Physics& ph);
@@ -99,7 +163,8 @@ int read_ibm(const std::string& path, IBTable& tb, std::string& err);
 int write_ibm(const Case& cs, const std::string& dir);
 // A case from BCMgrid00000.dat and BCM_STL_fine.stl in dir; p supplies the
 // flow parameters (Mach, Reynolds number, reference length, noise).
-int read_case(const std::string& dir, const CaseParams& p, Case& cs, std::string& err);
+int read_case(const std::string& dir, const CaseParams& p, Case& cs, std::string& err,
+              IbDevice* dev = nullptr);
 // PLOT3D grid BCMdata00001.x and solution qBCMdata<step>.q in dir for cubes
 // 1..ncube-1 with cell centres Xcnt/Ycnt/Zcnt [ncube][20] and the AoS state U
 // [ncube][20][20][20][5]; the headers announce mpi_ncube blocks.

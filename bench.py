@@ -174,7 +174,7 @@ def main():
     import bcm_cgli_amd as B
     from bcm_cgli_amd.configs import batch_cases
     t_setup = time.perf_counter()
-    case = B.Case(B.case_params(args.config))
+    case = B.Case(B.case_params(args.config), device=0)
     setup_s = time.perf_counter() - t_setup
     log(f"case {args.config} built in {setup_s:.1f} s")
     n_inner = args.inner
@@ -235,7 +235,7 @@ def main():
         }
         params = batch_cases(args.batch_cases)
         t0 = time.perf_counter()
-        bcases = [B.Case(q) for q in params]
+        bcases = [B.Case(q, device=0) for q in params]
         bsetup = time.perf_counter() - t0
         log(f"batch: {len(bcases)} cases built in {bsetup:.1f} s")
         b64 = run_pool(B, bcases, "fp64", args.batch_steps, 1, n_inner, 10, True)

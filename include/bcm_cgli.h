@@ -147,6 +147,11 @@ typedef struct bcm_case_params {
 typedef struct bcm_case bcm_case;
 
 int bcm_case_build(const bcm_case_params* p, bcm_case** out);
+/* bcm_case_build with the two quadratic passes of the IB setup (triangle
+ * filter, BCM_Immersed_boundary.cpp:380-470; surface search per ghost cell,
+ * :722-860) as kernels on GPU `device`; the case is bitwise the one
+ * bcm_case_build gives. */
+int bcm_case_build_on(const bcm_case_params* p, int device, bcm_case** out);
 void bcm_case_free(bcm_case* c);
 /* info[16]: ncube, n_wallcube, nbc_minus, nbc_plus, nXbc_l, nXbc_u, nYbc_l,
  * nYbc_u, nZbc_l, nZbc_u, nadjX_eq, nadjY_eq, nadjZ_eq, n_triangles,
@@ -170,6 +175,8 @@ int bcm_case_write_inputs(const bcm_case* c, const char* dir);
  * mu_L), seed and noise_amp; its lattice, cube_size, origin, sphere_center
  * and ico_level are not used. */
 int bcm_case_read(const char* dir, const bcm_case_params* p, bcm_case** out);
+/* bcm_case_read with the IB search on GPU `device` (as bcm_case_build_on). */
+int bcm_case_read_on(const char* dir, const bcm_case_params* p, int device, bcm_case** out);
 /* Writes the IB tables as IBM_minus00000.dat and IBM_plus00000.dat in the
  * byte format of BCM_Immersed_boundary.cpp:1009-1065 (read back by
  * BCM_Reading_IBM.cpp:48-75). */
