@@ -36,6 +36,8 @@ class CaseParamsC(ctypes.Structure):
         ("reynolds", ctypes.c_double),
         ("seed", ctypes.c_ulonglong),
         ("noise_amp", ctypes.c_double),
+        ("refine_lo", ctypes.c_int * 3),
+        ("refine_hi", ctypes.c_int * 3),
     ]
 
 
@@ -127,6 +129,11 @@ class Case:
         p.reynolds = float(params["reynolds"])
         p.seed = int(params.get("seed", 0))
         p.noise_amp = float(params.get("noise_amp", 0.0))
+        refine = params.get("refine")
+        if refine is not None:
+            for d in range(3):
+                p.refine_lo[d] = int(refine[0][d])
+                p.refine_hi[d] = int(refine[1][d])
         h = ctypes.c_void_p()
         if _from_dir is None and device is None:
             _check(lib.bcm_case_build(ctypes.byref(p), ctypes.byref(h)))

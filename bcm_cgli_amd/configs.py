@@ -1,6 +1,7 @@
 """Flow cases of the benchmark and test suite.
 
-All cases: a sphere of diameter D = 0.25 in a box of uniform 16^3-cell cubes,
+All cases: a sphere in a box of 16^3-cell cubes (one cube size, or two sizes
+for the two-level grids),
 gamma = 1.4, Pr = 0.72, freestream density 1.1842 and pressure 101300 (the
 reference's prm.h values), seeded +-1e-3 relative density noise on fluid cells.
 """
@@ -18,10 +19,20 @@ CONFIGS = {
     # icosphere at the box centre, M 0.3, Re 300.
     "cfg2": dict(_BASE, lattice=(16, 16, 8), cube_size=1.0 / 16, sphere_center=(0.5, 0.5, 0.25),
                  ico_level=5, mach=0.3, reynolds=300.0),
+    # Two-level grids (cubes of two sizes): a 2x2x2 lattice of cubes of edge 0.5
+    # with the first cube split into 8 (15 cubes), the sphere inside the split
+    # cube; and a 4x4x4 lattice of edge 0.25 with the central 2x2x2 split
+    # (120 cubes), the sphere inside the refined block.
+    "ml_tiny": dict(_BASE, lattice=(2, 2, 2), cube_size=0.5, sphere_center=(0.25, 0.25, 0.25),
+                    sphere_diameter=0.2, ico_level=4, mach=0.2, reynolds=100.0,
+                    refine=((0, 0, 0), (1, 1, 1))),
+    "ml": dict(_BASE, lattice=(4, 4, 4), cube_size=0.25, sphere_center=(0.5, 0.5, 0.5),
+               ico_level=4, mach=0.2, reynolds=100.0, refine=((1, 1, 1), (3, 3, 3))),
 }
 
 # Stepping of each case: (physical steps, inner iterations per step).
-STEPPING = {"tiny": (2, 3), "cfg1": (20, 10), "cfg2": (200, 10), "batch": (50, 10)}
+STEPPING = {"tiny": (2, 3), "cfg1": (20, 10), "cfg2": (200, 10), "batch": (50, 10),
+            "ml_tiny": (2, 3), "ml": (4, 5)}
 
 # Cube lattices of the batched workload: n^3 cubes on the unit box.
 BATCH_LATTICES = (2, 3, 4, 5, 6, 8)

@@ -52,6 +52,7 @@ struct DropIn {
   std::map<std::tuple<int, const int*, int, const int*, int>, Bc> bc;
   std::map<std::tuple<const int*, const int*, const double*, int>, GhostDevice> gh;
   std::map<std::vector<const void*>, HaloSrc*> halo;
+  std::map<std::vector<const void*>, std::pair<MlTaskDev*, int>> ml;
   std::vector<void*> owned;
 };
 
@@ -176,6 +177,10 @@ CaseParams to_params(const bcm_case_params* p) {
   cp.reynolds = p->reynolds;
   cp.seed = p->seed;
   cp.noise_amp = p->noise_amp;
+  for (int d = 0; d < 3; ++d) {
+    cp.refine_lo[d] = p->refine_lo[d];
+    cp.refine_hi[d] = p->refine_hi[d];
+  }
   return cp;
 }
 
@@ -298,9 +303,13 @@ void* bcm_case_array(bcm_case* c, const char* name, long long* count) {
   const char* bcn[6] = {"Xbc_l", "Xbc_u", "Ybc_l", "Ybc_u", "Zbc_l", "Zbc_u"};
   for (int s = 0; s < 6; ++s)
     if (n == bcn[s]) return ret(cs.bc_lists[s]);
-  if (n == "adjX_eq") return ret(cs.adj[L_X_EQ]);
-  if (n == "adjY_eq") return ret(cs.adj[L_Y_EQ]);
-  if (n == "adjZ_eq") return ret(cs.adj[L_Z_EQ]);
+  static const char* lists[L_COUNT] = {
+      "adjX_eq",       "adjY_eq",       "adjZ_eq",        "adjX_bs_plus",  "adjX_sb_plus",
+      "adjX_bs_minus", "adjX_sb_minus", "adjY_bs_plus",   "adjY_sb_plus",  "adjY_bs_minus",
+      "adjY_sb_minus", "adjZ_bs_plus",  "adjZ_sb_plus",   "adjZ_bs_minus", "adjZ_sb_minus"};
+  for (int l = 0; l < L_COUNT; ++l)
+    if (n == lists[l]) return ret(cs.adj[l]);
+  if (n == "nadj") return ret(cs.nadj);
   fail("unknown case array " + n);
   return nullptr;
 }
@@ -778,15 +787,12 @@ void BCM_Interface(int myid, int ncube, int mPI_Nadj, int ncpu_bs, int ncpu_eq, 
   (void)Scube_Ncpu_bs; (void)Rcube_Ncpu_bs; (void)send_data_curr_bs; (void)recv_data_curr_bs;
   (void)send_data_neig_bs; (void)recv_data_neig_bs; (void)Sdir_bs; (void)Rdir_bs;
   (void)ist_eq; (void)ist_sb; (void)ist_bs; (void)adjN_sb; (void)RadjN_bs; (void)SadjN_bs;
-  (void)csl; (void)adjX_bs_plus; (void)adjX_sb_plus; (void)adjX_bs_minus; (void)adjX_sb_minus;
-  (void)adjY_bs_plus; (void)adjY_sb_plus; (void)adjY_bs_minus; (void)adjY_sb_minus;
-  (void)adjZ_bs_plus; (void)adjZ_sb_plus; (void)adjZ_bs_minus; (void)adjZ_sb_minus;
+  (void)csl; (void)adjX_sb_plus; (void)adjX_sb_minus; (void)adjY_sb_plus; (void)adjY_sb_minus;
+  (void)adjZ_sb_plus; (void)adjZ_sb_minus;
+  (void)nadjX_sb_plus; (void)nadjX_sb_minus; (void)nadjY_sb_plus; (void)nadjY_sb_minus;
+  (void)nadjZ_sb_plus; (void)nadjZ_sb_minus;
   if (ncpu_bs != 0 || ncpu_eq != 0 || ncpu_sb != 0)
     fatal(where, "neighbour ranks are not supported on a single device");
-  if (nadjX_bs_plus || nadjX_sb_plus || nadjX_bs_minus || nadjX_sb_minus || nadjY_bs_plus ||
-      nadjY_sb_plus || nadjY_bs_minus || nadjY_sb_minus || nadjZ_bs_plus || nadjZ_sb_plus ||
-      nadjZ_bs_minus || nadjZ_sb_minus)
-    fatal(where, "cube pairs of different size are not supported");
   if (ncube != d.ncube) fatal(where, "ncube differs from the bound arrays");
   std::vector<const void*> key = {adjX_eq, adjY_eq, adjZ_eq, adj_number,
                                   (const void*)(intptr_t)nadjX_eq, (const void*)(intptr_t)nadjY_eq,
@@ -812,6 +818,44 @@ void BCM_Interface(int myid, int ncube, int mPI_Nadj, int ncpu_bs, int ncpu_eq, 
   }
   launch_halo_edge(h->e->stream, h->e->U, it->second, ncube, h->e->cube_case);
   check(where, hipGetLastError());
+  // Two-level faces (the bs lists; the sb lists serve only the reference's
+  // exchanges with other ranks), then the edge passes again.
+  const int nbs[6] = {nadjX_bs_plus, nadjX_bs_minus, nadjY_bs_plus,
+                      nadjY_bs_minus, nadjZ_bs_plus, nadjZ_bs_minus};
+  if (nbs[0] || nbs[1] || nbs[2] || nbs[3] || nbs[4] || nbs[5]) {
+    const int* bsl[6] = {adjX_bs_plus, adjX_bs_minus, adjY_bs_plus,
+                         adjY_bs_minus, adjZ_bs_plus, adjZ_bs_minus};
+    std::vector<const void*> mkey = {adj_number};
+    for (int q = 0; q < 6; ++q) {
+      mkey.push_back(bsl[q]);
+      mkey.push_back((const void*)(intptr_t)nbs[q]);
+    }
+    auto mt = d.ml.find(mkey);
+    if (mt == d.ml.end()) {
+      int nadj[L_COUNT] = {0};
+      const int* lists[L_COUNT] = {nullptr};
+      const int li[6] = {L_X_BS_PLUS, L_X_BS_MINUS, L_Y_BS_PLUS,
+                         L_Y_BS_MINUS, L_Z_BS_PLUS, L_Z_BS_MINUS};
+      for (int q = 0; q < 6; ++q) {
+        nadj[li[q]] = nbs[q];
+        lists[li[q]] = bsl[q];
+        for (int m = 1; m <= nbs[q]; ++m)
+          if (bsl[q][m] < 1 || bsl[q][m] >= ncube) fatal(where, "interface list entry out of range");
+      }
+      std::vector<MlTaskDev> tasks;
+      for (const MlTask& t : ml_tasks(lists, nadj, adj_number)) {
+        if (t.dst < 1 || t.dst >= ncube || t.src < 1 || t.src >= ncube)
+          fatal(where, "two-level neighbour out of range");
+        tasks.push_back(MlTaskDev{t.dst, t.src, t.axis, t.hi, t.restrict_, t.slot});
+      }
+      MlTaskDev* dp = upload_vec(d, tasks);
+      if (!dp) fatal(where, "device allocation failed");
+      mt = d.ml.emplace(mkey, std::make_pair(dp, (int)tasks.size())).first;
+    }
+    launch_halo_ml(h->e->stream, h->e->U, mt->second.first, mt->second.second, h->e->pt());
+    launch_edge(h->e->stream, h->e->U, ncube, h->e->cube_case);
+    check(where, hipGetLastError());
+  }
 }
 
 void BCM_Flux_XYZ_Viscous_Runge_kutta(int myid, int ncube, int RK, double deltaT,

@@ -57,7 +57,8 @@ void free_graph(Engine* e) {
 
 // Tables of the resident cases (rebuilt whenever a case comes or goes).
 void free_tables(Engine* e) {
-  void* ptrs[] = {e->bc_cubes, e->bc_masks, e->halo, e->halo_pairs, e->case_desc, e->bc_mask_cube};
+  void* ptrs[] = {e->bc_cubes, e->bc_masks, e->halo,        e->halo_pairs,
+                  e->case_desc, e->bc_mask_cube, e->ml};
   for (void* p : ptrs)
     if (p) hipFree(p);
   ghost_free(e->ghost);
@@ -67,7 +68,8 @@ void free_tables(Engine* e) {
   e->halo = nullptr;
   e->halo_pairs = nullptr;
   e->case_desc = nullptr;
-  e->n_halo_pairs = e->n_bc = e->n_ghost_cubes = e->n_ghost_entries = 0;
+  e->ml = nullptr;
+  e->n_halo_pairs = e->n_bc = e->n_ghost_cubes = e->n_ghost_entries = e->n_ml = 0;
 }
 
 void free_case(Engine* e) {
@@ -400,6 +402,19 @@ int build_case_tables(const Case& cs, CaseTables& tb, std::string& err) {
       tb.halo[c2].src[2 * d] = c0;      // c2's minus face from c0's top layers
       tb.halo[c0].src[2 * d + 1] = c2;  // c0's plus face from c2's bottom layers
     }
+  // Two-level faces.
+  {
+    const int* lp[L_COUNT];
+    for (int l = 0; l < L_COUNT; ++l) lp[l] = cs.adj[l].data();
+    for (const MlTask& t : ml_tasks(lp, cs.nadj.data(), cs.adj_number.data())) {
+      if (t.dst < 1 || t.dst >= cs.ncube || t.src < 1 || t.src >= cs.ncube || t.axis < 0 ||
+          t.axis > 2 || t.slot < 1 || t.slot > 4) {
+        err = "two-level interface names a missing cube";
+        return 1;
+      }
+      tb.ml.push_back(MlTaskDev{t.dst, t.src, t.axis, t.hi, t.restrict_, t.slot});
+    }
+  }
   // Cells of the IB lists that some neighbour gathers: the only interior
   // cells that change between a post-update exchange and the next stage's
   // pre-flux exchange.
@@ -445,6 +460,7 @@ int rebuild_tables(Engine* e, std::string& err) {
   std::vector<uint8_t> bm;
   GhostSchedule gs;
   std::vector<int2> pairs;
+  std::vector<MlTaskDev> ml;
   std::vector<CaseDesc> desc;
   std::vector<DevPhys> phys(e->max_cases);
   for (int s = 0; s < e->max_cases; ++s) phys[s] = e->cases[s].tb.ph;
@@ -477,6 +493,11 @@ int rebuild_tables(Engine* e, std::string& err) {
     gs.max_levels = std::max(gs.max_levels, tb.ghost.max_levels);
     gs.max_slots = std::max(gs.max_slots, tb.ghost.max_slots);
     for (int2 pr : tb.halo_pairs) pairs.push_back(make_int2(pr.x + off * PLANE, pr.y + off * PLANE));
+    for (MlTaskDev t : tb.ml) {
+      t.dst += off;
+      t.src += off;
+      ml.push_back(t);
+    }
     desc.push_back(CaseDesc{cs.first, tb.n, tb.mpi_ncube, s});
   }
   gs.cube_level_off.push_back((int)gs.level_off.size());
@@ -502,6 +523,8 @@ int rebuild_tables(Engine* e, std::string& err) {
   e->n_ghost_entries = (int)gs.entries.size();
   e->ghost_levels = gs.max_levels;
   e->n_halo_pairs = (int)pairs.size();
+  e->n_ml = (int)ml.size();
+  if (dev_upload(&e->ml, ml, err)) return 1;
   if (dev_upload(&e->bc_cubes, bc, err) || dev_upload(&e->bc_masks, bm, err) ||
       ghost_upload(gs, e->ghost, err) || dev_upload(&e->halo, hs, err) ||
       dev_upload(&e->halo_pairs, pairs, err) || dev_upload(&e->case_desc, desc, err) ||
@@ -722,18 +745,37 @@ void engine_ghost(Engine* e) {
     launch_ghost(e->stream, e->U, e->ghost.t, e->pt());
 }
 
+namespace {
+// Two-level faces, then the edge passes of every cube (the equal-size faces
+// are already in place).
+template <typename T>
+void ml_then_edges(Engine* e, T* U) {
+  launch_halo_ml(e->stream, U, e->ml, e->n_ml, e->pt());
+  launch_edge(e->stream, U, e->ncube, e->cube_case);
+}
+}  // namespace
+
 void engine_interface(Engine* e) {
-  if (e->prec == 1)
+  if (e->prec == 1) {
     launch_halo_edge(e->stream, e->Uf, e->halo, e->ncube, e->cube_case);
-  else
+    if (e->n_ml) ml_then_edges(e, e->Uf);
+  } else {
     launch_halo_edge(e->stream, e->U, e->halo, e->ncube, e->cube_case);
+    if (e->n_ml) ml_then_edges(e, e->U);
+  }
 }
 
 void engine_halo_refresh(Engine* e) {
-  if (e->prec == 1)
+  // IB ghost cells in equal-size gather layers, then (two-level grids) the
+  // two-level faces, whose restrictions read small cubes' cells near the
+  // face, then the edge passes.
+  if (e->prec == 1) {
     launch_halo_sparse(e->stream, e->Uf, e->halo_pairs, e->n_halo_pairs, e->ncube, e->cube_case);
-  else
+    if (e->n_ml) ml_then_edges(e, e->Uf);
+  } else {
     launch_halo_sparse(e->stream, e->U, e->halo_pairs, e->n_halo_pairs, e->ncube, e->cube_case);
+    if (e->n_ml) ml_then_edges(e, e->U);
+  }
 }
 
 namespace {
@@ -877,9 +919,11 @@ void engine_flux_push(Engine* e, int RK, int er_slot) {
 
 void engine_stage(Engine* e, int RK, int er_slot, bool full_pre_halo) {
   if (e->prec == 1) {
-    // The previous flux kernel wrote the face halos; the edge passes of that
-    // exchange and this stage's BCs open the stage (the edge passes are
-    // idempotent when no face changed, as at the start of a run).
+    // The previous flux kernel wrote the equal-size face halos; the
+    // two-level faces and the edge passes of that exchange and this stage's
+    // BCs open the stage (all idempotent when no interior cell changed, as at
+    // the start of a run).
+    launch_halo_ml(e->stream, e->Uf, e->ml, e->n_ml, e->pt());
     launch_edge_bc(e->stream, e->Uf, e->bc_mask_cube, e->ncube, e->pt());
     engine_ghost(e);
     if (full_pre_halo)
@@ -890,6 +934,7 @@ void engine_stage(Engine* e, int RK, int er_slot, bool full_pre_halo) {
     return;
   }
   if (e->flux_push) {
+    launch_halo_ml(e->stream, e->U, e->ml, e->n_ml, e->pt());
     launch_edge_bc(e->stream, e->U, e->bc_mask_cube, e->ncube, e->pt());
     engine_ghost(e);
     if (full_pre_halo)
@@ -991,11 +1036,11 @@ int engine_run(Engine* e, int n_steps, int n_inner, double* er_hist, bool use_gr
                 e->er_host[it * row + c * 8 + 1 + m];
     }
   }
-  // The mixed path's last flux kernel wrote the face halos; the edge passes
-  // complete that exchange (as the reference's closing BCM_Interface does).
-  if (e->prec == 1 && n_steps > 0) launch_edge(e->stream, e->Uf, e->ncube, e->cube_case);
-  if (e->prec == 0 && e->flux_push && n_steps > 0)
-    launch_edge(e->stream, e->U, e->ncube, e->cube_case);
+  // The last flux kernel wrote the equal-size face halos; the two-level
+  // faces and the edge passes complete that exchange (as the reference's
+  // closing BCM_Interface does).
+  if (e->prec == 1 && n_steps > 0) ml_then_edges(e, e->Uf);
+  if (e->prec == 0 && e->flux_push && n_steps > 0) ml_then_edges(e, e->U);
   HIP_TRY(hipGetLastError());
   return 0;
 }

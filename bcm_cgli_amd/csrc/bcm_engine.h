@@ -50,6 +50,7 @@ struct CaseTables {
   GhostSchedule ghost;
   std::vector<HaloSrc> halo;       // [n + 1]
   std::vector<int2> halo_pairs;    // (source, destination) flat cells, cube * 8000 + cell
+  std::vector<MlTaskDev> ml;       // two-level faces (empty on one-size grids)
 };
 int build_case_tables(const Case& cs, CaseTables& tb, std::string& err);
 
@@ -108,6 +109,10 @@ struct Engine {
   // that lie in a neighbour's gather source layers.
   int2* halo_pairs = nullptr;
   int n_halo_pairs = 0;
+  // Two-level faces of all resident cases, refreshed after the equal-size
+  // faces and before the edge passes wherever an exchange runs.
+  MlTaskDev* ml = nullptr;
+  int n_ml = 0;
   // Set when an inner iterate was replaced from outside (state upload, new
   // case): the next pre-flux exchange must then be the full one.
   bool halo_full_next = false;

@@ -79,6 +79,18 @@ struct HaloSrc {
   int src[6];   // -X, +X, -Y, +Y, -Z, +Z
 };
 
+// One ghost region of a face across a change of cube size (two-level grid):
+// dst's two ghost layers on side hi (1 upper, 0 lower) of axis `axis` from
+// the neighbour src.  restrict_ 0: dst is the small cube, each ghost cell
+// copies the big cube's face cell above it (BCM_Interface.cpp:1790-2021 for
+// x); restrict_ 1: dst is the big cube, the 2 x 8 x 8 ghost cells of quadrant
+// `slot` (1..4) take 0.125 times the sum of the 8 small cells beneath, in the
+// reference's order (:2026-2600 for x; y and z alike).
+struct MlTaskDev {
+  int dst, src;
+  int8_t axis, hi, restrict_, slot;
+};
+
 // Copies, with the threads of one workgroup, the ghost regions the domain BCs
 // of cube c write (mask bits 1 x-min, 2 x-max, 4 y-min, 8 y-max, 16 z-min,
 // 32 z-max) from Uin to Uout: x faces over j,k in [2,17], y faces over all i
@@ -133,6 +145,9 @@ void launch_edge_bc(hipStream_t s, float* U, const uint8_t* cube_mask, int ncube
 // Edge passes of every cube.
 void launch_edge(hipStream_t s, double* U, int ncube, const int* cube_case);
 void launch_edge(hipStream_t s, float* U, int ncube, const int* cube_case);
+// Two-level faces: one workgroup per task.
+void launch_halo_ml(hipStream_t s, double* U, const MlTaskDev* tasks, int ntask, PhysTab pt);
+void launch_halo_ml(hipStream_t s, float* U, const MlTaskDev* tasks, int ntask, PhysTab pt);
 void launch_flux(hipStream_t s, const double* U, const double* U1, const double* U1q,
                  const int8_t* fws, const double* cube_size, double* Rku, double* Res,
                  int ncube, PhysTab pt, double deltaT, double deltaTau, double e);

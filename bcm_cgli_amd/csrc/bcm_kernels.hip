@@ -1405,6 +1405,64 @@ __global__ __launch_bounds__(WG) void k_edge_bc(T* __restrict__ U,
   }
 }
 
+// Two-level faces (BCM_Interface.cpp:1790-2600 for x, :2651-3470 for y,
+// :3518-4360 for z): one task per workgroup.  Injection copies storage values
+// (the FP32 iterate holds deviations from one reference state per case, and
+// both cubes of a face belong to the same case); restriction sums the 8 full
+// states in the reference's order, (0,0,0) (1,0,0) (0,1,0) (0,0,1) (1,1,0)
+// (1,0,1) (0,1,1) (1,1,1) of (i,j,k) offsets, and scales by 0.125.
+template <typename T>
+__global__ __launch_bounds__(WG) void k_halo_ml(T* __restrict__ U,
+                                                const MlTaskDev* __restrict__ tasks,
+                                                PhysTab pt) {
+  const MlTaskDev t = tasks[blockIdx.x];
+  const int A = t.axis, T0 = A == 0 ? 1 : 0, T1 = A == 2 ? 1 : 2;
+  const int o0 = (t.slot == 2 || t.slot == 4) ? 8 : 0;
+  const int o1 = (t.slot == 3 || t.slot == 4) ? 8 : 0;
+  int d[3], q[3];
+  if (!t.restrict_) {
+    for (int n = threadIdx.x; n < 2 * 256; n += WG) {
+      const int l = n >> 8, p = ((n >> 4) & 15) + 2, r = (n & 15) + 2;
+      d[A] = t.hi ? 18 + l : l;
+      d[T0] = p;
+      d[T1] = r;
+      q[A] = t.hi ? 2 : 17;
+      q[T0] = p / 2 + 1 + o0;
+      q[T1] = r / 2 + 1 + o1;
+      const size_t dd = sidx(t.dst, 0, d[0], d[1], d[2]);
+      const size_t so = sidx(t.src, 0, q[0], q[1], q[2]);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) U[dd + (size_t)v * PLANE] = U[so + (size_t)v * PLANE];
+    }
+    return;
+  }
+  const typename OffOf<T>::type F(pt.phys[pt.cube_case[t.dst]]);
+  for (int n = threadIdx.x; n < 2 * 64; n += WG) {
+    const int l = n >> 6, p = ((n >> 3) & 7) + 2, r = (n & 7) + 2;
+    d[A] = t.hi ? 18 + l : l;
+    d[T0] = p + o0;
+    d[T1] = r + o1;
+    q[A] = t.hi ? 2 * (l + 1) : 14 + 2 * l;
+    q[T0] = (p - 1) * 2;
+    q[T1] = (r - 1) * 2;
+    const size_t dd = sidx(t.dst, 0, d[0], d[1], d[2]);
+    const size_t b = sidx(t.src, 0, q[0], q[1], q[2]);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const size_t o = b + (size_t)v * PLANE;
+      double s = F.ld(U, o, v);
+      s = s + F.ld(U, o + 400, v);
+      s = s + F.ld(U, o + 20, v);
+      s = s + F.ld(U, o + 1, v);
+      s = s + F.ld(U, o + 420, v);
+      s = s + F.ld(U, o + 401, v);
+      s = s + F.ld(U, o + 21, v);
+      s = s + F.ld(U, o + 421, v);
+      F.st(U, dd + (size_t)v * PLANE, v, 0.125 * s);
+    }
+  }
+}
+
 // Physical-time shift of interior cells (main.cpp:1292-1315).
 __global__ __launch_bounds__(WG) void k_shift(const double* __restrict__ U,
                                               double* __restrict__ U1,
@@ -1487,6 +1545,16 @@ void launch_edge_bc(hipStream_t s, float* U, const uint8_t* cube_mask, int ncube
 template <typename T>
 void launch_edge_t(hipStream_t s, T* U, int ncube, const int* cube_case) {
   if (ncube > 1) hipLaunchKernelGGL(k_edge<T>, dim3(ncube - 1), dim3(WG), 0, s, U, cube_case);
+}
+template <typename T>
+void launch_halo_ml_t(hipStream_t s, T* U, const MlTaskDev* tasks, int ntask, PhysTab pt) {
+  if (ntask > 0) hipLaunchKernelGGL(k_halo_ml<T>, dim3(ntask), dim3(WG), 0, s, U, tasks, pt);
+}
+void launch_halo_ml(hipStream_t s, double* U, const MlTaskDev* tasks, int ntask, PhysTab pt) {
+  launch_halo_ml_t(s, U, tasks, ntask, pt);
+}
+void launch_halo_ml(hipStream_t s, float* U, const MlTaskDev* tasks, int ntask, PhysTab pt) {
+  launch_halo_ml_t(s, U, tasks, ntask, pt);
 }
 void launch_edge(hipStream_t s, double* U, int ncube, const int* cube_case) {
   launch_edge_t(s, U, ncube, cube_case);

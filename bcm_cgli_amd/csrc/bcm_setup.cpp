@@ -175,6 +175,90 @@ void interface_tables(int ncube, const int* csl, const int* adj, int* nadj,
 }
 
 // ---------------------------------------------------------------------------
+std::vector<MlTask> ml_tasks(const int* const* lists, const int* nadj, const int* adj) {
+  std::vector<MlTask> out;
+  const int bs_plus[3] = {L_X_BS_PLUS, L_Y_BS_PLUS, L_Z_BS_PLUS};
+  const int bs_minus[3] = {L_X_BS_MINUS, L_Y_BS_MINUS, L_Z_BS_MINUS};
+  for (int A = 0; A < 3; ++A) {
+    const int dplus = 2 * A + 2, dminus = 2 * A + 1;
+    // Small cubes on the big cube's upper side: their lower ghosts.
+    for (int a = 1; a <= nadj[bs_plus[A]]; ++a) {
+      const int big = lists[bs_plus[A]][a];
+      for (int s = 1; s <= 4; ++s) {
+        const int small = adj[adj_index(big, s, dplus)];
+        if (small > 0) out.push_back({small, big, (int8_t)A, 0, 0, (int8_t)s});
+      }
+    }
+    // Small cubes on the big cube's lower side: their upper ghosts.
+    for (int a = 1; a <= nadj[bs_minus[A]]; ++a) {
+      const int big = lists[bs_minus[A]][a];
+      for (int s = 1; s <= 4; ++s) {
+        const int small = adj[adj_index(big, s, dminus)];
+        if (small > 0) out.push_back({small, big, (int8_t)A, 1, 0, (int8_t)s});
+      }
+    }
+    // The big cube's upper ghosts from the small cubes above it.
+    for (int a = 1; a <= nadj[bs_plus[A]]; ++a) {
+      const int big = lists[bs_plus[A]][a];
+      for (int s = 1; s <= 4; ++s) {
+        const int small = adj[adj_index(big, s, dplus)];
+        if (small > 0) out.push_back({big, small, (int8_t)A, 1, 1, (int8_t)s});
+      }
+    }
+    // The big cube's lower ghosts from the small cubes below it.
+    for (int a = 1; a <= nadj[bs_minus[A]]; ++a) {
+      const int big = lists[bs_minus[A]][a];
+      for (int s = 1; s <= 4; ++s) {
+        const int small = adj[adj_index(big, s, dminus)];
+        if (small > 0) out.push_back({big, small, (int8_t)A, 0, 1, (int8_t)s});
+      }
+    }
+  }
+  return out;
+}
+
+void ml_apply_host(const MlTask& t, double* U) {
+  const int A = t.axis;
+  const int T0 = A == 0 ? 1 : 0, T1 = A == 2 ? 1 : 2;   // tangential axes
+  const int o0 = (t.slot == 2 || t.slot == 4) ? NC / 2 : 0;
+  const int o1 = (t.slot == 3 || t.slot == 4) ? NC / 2 : 0;
+  int d[3], q[3];
+  if (!t.restrict_) {
+    for (int l = 0; l < NB; ++l)
+      for (int p = ILO; p <= IHI; ++p)
+        for (int r = ILO; r <= IHI; ++r) {
+          d[A] = t.hi ? IHI + 1 + l : l;
+          d[T0] = p;
+          d[T1] = r;
+          q[A] = t.hi ? NB + l / NB : IHI + l / NB;
+          q[T0] = p / NB + 1 + o0;
+          q[T1] = r / NB + 1 + o1;
+          for (int v = 0; v < NV; ++v)
+            U[aos_index(t.dst, d[0], d[1], d[2], v)] = U[aos_index(t.src, q[0], q[1], q[2], v)];
+        }
+    return;
+  }
+  for (int l = 0; l < NB; ++l)
+    for (int p = ILO; p <= (IHI - NB + 1) / 2 + 1; ++p)
+      for (int r = ILO; r <= (IHI - NB + 1) / 2 + 1; ++r) {
+        d[A] = t.hi ? NB + NC + l : l;
+        d[T0] = p + o0;
+        d[T1] = r + o1;
+        q[A] = t.hi ? NB * (l + 1) : IHI - 2 * NB + NB * (l + 1) - 1;
+        q[T0] = (p - NB + 1) * 2;
+        q[T1] = (r - NB + 1) * 2;
+        const int i2 = q[0], j2 = q[1], k2 = q[2];
+        for (int v = 0; v < NV; ++v) {
+          auto S = [&](int a, int b, int c) { return U[aos_index(t.src, a, b, c, v)]; };
+          U[aos_index(t.dst, d[0], d[1], d[2], v)] =
+              0.125 * (S(i2, j2, k2) + S(i2 + 1, j2, k2) + S(i2, j2 + 1, k2) + S(i2, j2, k2 + 1) +
+                       S(i2 + 1, j2 + 1, k2) + S(i2 + 1, j2, k2 + 1) + S(i2, j2 + 1, k2 + 1) +
+                       S(i2 + 1, j2 + 1, k2 + 1));
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
 // Equal-size halo copies and the edge-ghost averaging of BCM_Interface on a
 // host state array (Interface.cpp:1745-1782, 2604-2641, 3474-3511, 5640-5732).
 void host_interface_copy(int ncube, const int* const* lists, const int* nadj,
@@ -212,6 +296,10 @@ void host_interface_copy(int ncube, const int* const* lists, const int* nadj,
             at(c0, i, j, NB + NC + k, v) = at(c2, i, j, NB + k, v);
           }
   });
+  // Two-level faces: each task writes its own ghost region from interior
+  // cells, so the tasks are independent.
+  const std::vector<MlTask> ml = ml_tasks(lists, nadj, adj);
+  parallel_for((int)ml.size(), [&](int n) { ml_apply_host(ml[n], U); });
   const int L1 = NB - 1, H1 = IHI + 1;  // layers 1 and 18
   parallel_for(ncube - 1, [&](int c0) {
     const int c = c0 + 1;
@@ -626,19 +714,10 @@ int immersed_boundary(Case& cs, IbDevice* dev, std::string& err) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
-int build_case(const CaseParams& p, Case& cs, std::string& err, IbDevice* dev) {
-  cs = Case();
-  cs.p = p;
-  if (p.lattice[0] < 1 || p.lattice[1] < 1 || p.lattice[2] < 1) {
-    err = "lattice dimensions must be positive";
-    return 1;
-  }
-  if (!(p.cube_size > 0) || !(p.sphere_diameter >= 0) || p.ico_level < 0 || p.ico_level > 7 ||
-      !(p.mach > 0) || !(p.reynolds > 0)) {
-    err = "invalid case parameters";
-    return 1;
-  }
-  compute_physics(p, cs.phys);
+namespace {
+
+// nx x ny x nz cubes of one size, numbered x fastest, one neighbour per face.
+void uniform_lattice(const CaseParams& p, Case& cs) {
   const int nxc = p.lattice[0], nyc = p.lattice[1], nzc = p.lattice[2];
   const int ncube = nxc * nyc * nzc + 1;
   cs.ncube = ncube;
@@ -646,10 +725,6 @@ int build_case(const CaseParams& p, Case& cs, std::string& err, IbDevice* dev) {
   cs.Xcube.assign(ncube, 0.0);
   cs.Ycube.assign(ncube, 0.0);
   cs.Zcube.assign(ncube, 0.0);
-  cs.Xcnt.assign((size_t)ncube * NP, 0.0);
-  cs.Ycnt.assign((size_t)ncube * NP, 0.0);
-  cs.Zcnt.assign((size_t)ncube * NP, 0.0);
-  cs.csl.assign(ncube, 0);
   cs.adj_number.assign((size_t)ncube * 35, 0);
   auto cid = [&](int ix, int iy, int iz) { return 1 + ix + nxc * (iy + nyc * iz); };
   for (int iz = 0; iz < nzc; ++iz)
@@ -668,6 +743,120 @@ int build_case(const CaseParams& p, Case& cs, std::string& err, IbDevice* dev) {
         a[1 * 7 + 5] = iz > 0 ? cid(ix, iy, iz - 1) : 0;
         a[1 * 7 + 6] = iz < nzc - 1 ? cid(ix, iy, iz + 1) : 0;
       }
+}
+
+// Two-level grid: lattice cubes of edge cube_size, those inside the
+// refinement box split into 8 cubes of half the edge (x fastest within the
+// parent).  Cubes are numbered in lattice order, a split cube's 8 children in
+// place of it.  Adjacency follows the slot convention of BCM_Interface.cpp:
+// a face with one neighbour (same size or the bigger cube) uses slot 1; a
+// face with four smaller neighbours lists them in slots 1..4 with slot 2 on
+// the upper half of the first tangential axis and slot 3 on the upper half of
+// the second (tangential axes in x, y, z order), e.g. :1825-1883 for +x.
+int two_level_lattice(const CaseParams& p, Case& cs) {
+  const int L[3] = {p.lattice[0], p.lattice[1], p.lattice[2]};
+  const int F[3] = {2 * L[0], 2 * L[1], 2 * L[2]};   // fine positions per axis
+  const double h = p.cube_size / 2;
+  std::vector<int> owner((size_t)F[0] * F[1] * F[2], 0);
+  auto own = [&](int x, int y, int z) -> int& { return owner[((size_t)z * F[1] + y) * F[0] + x]; };
+  struct Cube { int x0[3]; int s; };
+  std::vector<Cube> cubes(1);
+  for (int iz = 0; iz < L[2]; ++iz)
+    for (int iy = 0; iy < L[1]; ++iy)
+      for (int ix = 0; ix < L[0]; ++ix) {
+        const int ii[3] = {ix, iy, iz};
+        bool in = true;
+        for (int d = 0; d < 3; ++d) in = in && ii[d] >= p.refine_lo[d] && ii[d] < p.refine_hi[d];
+        if (in) {
+          for (int dz = 0; dz < 2; ++dz)
+            for (int dy = 0; dy < 2; ++dy)
+              for (int dx = 0; dx < 2; ++dx)
+                cubes.push_back({{2 * ix + dx, 2 * iy + dy, 2 * iz + dz}, 1});
+        } else {
+          cubes.push_back({{2 * ix, 2 * iy, 2 * iz}, 2});
+        }
+      }
+  const int ncube = (int)cubes.size();
+  for (int c = 1; c < ncube; ++c) {
+    const Cube& q = cubes[c];
+    for (int a = 0; a < q.s; ++a)
+      for (int b = 0; b < q.s; ++b)
+        for (int e = 0; e < q.s; ++e) own(q.x0[0] + e, q.x0[1] + b, q.x0[2] + a) = c;
+  }
+  cs.ncube = ncube;
+  cs.cube_size.assign(ncube, 0.0);
+  cs.Xcube.assign(ncube, 0.0);
+  cs.Ycube.assign(ncube, 0.0);
+  cs.Zcube.assign(ncube, 0.0);
+  cs.adj_number.assign((size_t)ncube * 35, 0);
+  for (int c = 1; c < ncube; ++c) {
+    const Cube& q = cubes[c];
+    cs.cube_size[c] = q.s * h;
+    cs.Xcube[c] = p.origin[0] + q.x0[0] * h;
+    cs.Ycube[c] = p.origin[1] + q.x0[1] * h;
+    cs.Zcube[c] = p.origin[2] + q.x0[2] * h;
+    for (int dir = 1; dir <= 6; ++dir) {
+      const int A = (dir - 1) / 2;
+      const bool plus = dir % 2 == 0;
+      const int t0 = A == 0 ? 1 : 0, t1 = A == 2 ? 1 : 2;
+      const int pa = plus ? q.x0[A] + q.s : q.x0[A] - 1;
+      if (pa < 0 || pa >= F[A]) continue;
+      int ids[4], n = 0;
+      for (int u1 = 0; u1 < q.s; ++u1)
+        for (int u0 = 0; u0 < q.s; ++u0) {
+          int pos[3];
+          pos[A] = pa;
+          pos[t0] = q.x0[t0] + u0;
+          pos[t1] = q.x0[t1] + u1;
+          ids[n++] = own(pos[0], pos[1], pos[2]);
+        }
+      int* a = &cs.adj_number[(size_t)c * 35];
+      if (n == 1 || (ids[0] == ids[1] && ids[0] == ids[2] && ids[0] == ids[3])) {
+        a[1 * 7 + dir] = ids[0];
+      } else {
+        for (int m = 0; m < 4; ++m) {
+          if (cubes[ids[m]].s != 1) return 1;   // neither one neighbour nor four small ones
+          a[(1 + m) * 7 + dir] = ids[m];
+        }
+      }
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+int build_case(const CaseParams& p, Case& cs, std::string& err, IbDevice* dev) {
+  cs = Case();
+  cs.p = p;
+  if (p.lattice[0] < 1 || p.lattice[1] < 1 || p.lattice[2] < 1) {
+    err = "lattice dimensions must be positive";
+    return 1;
+  }
+  if (!(p.cube_size > 0) || !(p.sphere_diameter >= 0) || p.ico_level < 0 || p.ico_level > 7 ||
+      !(p.mach > 0) || !(p.reynolds > 0)) {
+    err = "invalid case parameters";
+    return 1;
+  }
+  compute_physics(p, cs.phys);
+  bool refined = true;
+  for (int d = 0; d < 3; ++d) {
+    if (p.refine_hi[d] <= p.refine_lo[d]) refined = false;
+    if (p.refine_lo[d] < 0 || p.refine_hi[d] > p.lattice[d]) {
+      err = "refinement box outside the lattice";
+      return 1;
+    }
+  }
+  if (refined) {
+    if (two_level_lattice(p, cs)) {
+      err = "refined lattice could not be built";
+      return 1;
+    }
+  } else {
+    uniform_lattice(p, cs);
+  }
+  const int ncube = cs.ncube;
   grid_geometry(cs);
 
   // Grid flags: all cells fluid, interior cells whose centre lies inside the
@@ -758,11 +947,16 @@ int finish_setup(Case& cs, std::string& err, IbDevice* dev) {
     for (int l = 0; l < L_COUNT; ++l) lp[l] = cs.adj[l].data();
     interface_tables(ncube, cs.csl.data(), cs.adj_number.data(), cs.nadj.data(), lp);
   }
-  for (int l = L_X_BS_PLUS; l < L_COUNT; ++l)
-    if (cs.nadj[l] != 0) {
-      err = "cube lattices with size changes are not supported";
-      return 2;
-    }
+  // Size changes of more than one level are not two-level faces.
+  for (int c = 1; c < ncube; ++c)
+    for (int dir = 1; dir <= 6; ++dir)
+      for (int sl = 1; sl <= 4; ++sl) {
+        const int n = cs.adj_number[adj_index(c, sl, dir)];
+        if (n > 0 && abs(cs.csl[n] - cs.csl[c]) > 1) {
+          err = "neighbouring cubes differ by more than one size level";
+          return 2;
+        }
+      }
 
   // First flag halo exchange and hole treatment (main.cpp:634-647).
   fws_interface(cs);

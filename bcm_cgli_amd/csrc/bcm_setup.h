@@ -25,6 +25,10 @@ struct CaseParams {
   double mach, reynolds;
   uint64_t seed;             // initial density noise seed
   double noise_amp;          // relative density noise amplitude (1e-3)
+  // Optional refinement: the lattice cubes with index in [refine_lo, refine_hi)
+  // along every axis are each split into 8 cubes of half the edge (a two-level
+  // grid); no refinement when the box is empty on some axis.
+  int refine_lo[3], refine_hi[3];
 };
 
 // Immersed-boundary table of one list (minus or plus) in the 1-based flat
@@ -125,6 +129,30 @@ struct IbDevice {
                      std::vector<int>& hit_any, std::vector<int>& hit_tri,
                      std::vector<int>& near_tri, std::string& err) = 0;
 };
+
+// One face ghost region filled across a change of cube size, the on-rank
+// two-level part of BCM_Interface (x: Interface.cpp:1790-2600, y: :2651-3470,
+// z: :3518-4360).  The big cube's face meets four small cubes, one per slot:
+// slot 2 covers the upper half of the first tangential axis, slot 3 the upper
+// half of the second (tangential axes in x, y, z order), slot 4 both.
+// * inject (restrict_ == 0): dst is the small cube; its two ghost layers on
+//   side `hi` take the big cube's cell at the face (index 17 for the small
+//   cube's lower side, 2 for its upper side) under each pair of ghost cells;
+// * restrict (restrict_ == 1): dst is the big cube; the 2 x 8 x 8 ghost cells
+//   of its quadrant on side `hi` take the mean of the 2 x 2 x 2 small cells
+//   beneath, summed in the reference's order.
+struct MlTask {
+  int dst, src;
+  int8_t axis;       // 0 x, 1 y, 2 z
+  int8_t hi;         // 1: dst's upper ghost layers (18, 19); 0: lower (0, 1)
+  int8_t restrict_;  // 0 inject into the small cube, 1 restrict into the big one
+  int8_t slot;       // 1..4
+};
+// Tasks of the two-level faces in reference order, from the interface lists
+// (lists[L_*_BS_PLUS], lists[L_*_BS_MINUS]) and adj_number.
+std::vector<MlTask> ml_tasks(const int* const* lists, const int* nadj, const int* adj_number);
+// Applies one task to an AoS state array.
+void ml_apply_host(const MlTask& t, double* U);
 
 // Builds the whole case.  Returns 0 on success, non-zero with a message in err.
 // With dev, the triangle filter and the surface search of the IB setup run

@@ -142,6 +142,8 @@ typedef struct bcm_case_params {
   double mach, reynolds;
   unsigned long long seed;    /* density-noise seed */
   double noise_amp;           /* relative density noise amplitude */
+  int refine_lo[3];           /* lattice cubes with index in [refine_lo, refine_hi) */
+  int refine_hi[3];           /* on every axis are split into 8 (a two-level grid) */
 } bcm_case_params;
 
 typedef struct bcm_case bcm_case;
@@ -161,8 +163,10 @@ int bcm_case_info(const bcm_case* c, int* info, double* phys);
  * "cube_size", "Xcube", "Ycube", "Zcube", "Xcnt", "Ycnt", "Zcnt" (double); "FWS",
  * "FWS_grid" (the flags as the grid file holds them), "csl",
  * "adj_number", "ib_minus_gc", "ib_minus_ip", "ib_plus_gc", "ib_plus_ip",
- * "Xbc_l", "Xbc_u", "Ybc_l", "Ybc_u", "Zbc_l", "Zbc_u", "adjX_eq", "adjY_eq",
- * "adjZ_eq" (int); "ib_minus_w", "ib_plus_w" (double). */
+ * "Xbc_l", "Xbc_u", "Ybc_l", "Ybc_u", "Zbc_l", "Zbc_u", the 15 interface lists of
+ * BCM_Interface_table.cpp ("adjX_eq", "adjY_eq", "adjZ_eq", "adjX_bs_plus",
+ * "adjX_sb_plus", "adjX_bs_minus", "adjX_sb_minus", then the same for Y and
+ * Z), "nadj" (their 15 lengths) (int); "ib_minus_w", "ib_plus_w" (double). */
 void* bcm_case_array(bcm_case* c, const char* name, long long* count);
 /* Writes BCMgrid00000.dat and BCM_STL_fine.stl, the reference's input files. */
 int bcm_case_write_inputs(const bcm_case* c, const char* dir);

@@ -230,13 +230,67 @@ void orc_ghostcell(int nbc, const double* weight, const int* gcindex, const int*
 }
 
 /* ------------------------------------------------------------------------
- * On-rank part of BCM_Interface for equal-size neighbours:
- * face copies X (Interface.cpp:1745-1782), Y (:2604-2641), Z (:3474-3511),
- * then the per-cube edge-ghost averaging (:5640-5732).
- * eq[d] lists cubes whose +d neighbour (adj_number[c][1][2d+2]) has the same size.
+ * On-rank part of BCM_Interface:
+ * equal-size face copies X (Interface.cpp:1745-1782), Y (:2604-2641),
+ * Z (:3474-3511); two-level faces X (:1790-2600), Y (:2651-3470),
+ * Z (:3518-4360); then the per-cube edge-ghost averaging (:5640-5732).
+ * eq[d] lists cubes whose +d neighbour (adj_number[c][1][2d+2]) has the same
+ * size; bs[2d] (bs_plus) and bs[2d+1] (bs_minus) list big cubes with four
+ * small neighbours on their + and - side of axis d.
  */
-void orc_interface(int ncube, const int* neq, const int* const* eq, const int* adj_number,
-                   double* U) {
+#define ADJ(c, s, d) adj_number[((c) * 5 + (s)) * 7 + (d)]
+
+/* Position of a point given its axis-d index a and tangential indices (p, r)
+ * in x, y, z order of the other two axes. */
+static inline void place(int d, int a, int p, int r, int* x) {
+  x[d] = a;
+  x[d == 0 ? 1 : 0] = p;
+  x[d == 2 ? 1 : 2] = r;
+}
+
+/* Big cube -> small cube: the small cube's two ghost layers copy the big
+ * cube's face cell over each 2 x 2 x 1 group (:1795-1898 and :1908-2021 for x). */
+static void inject(int d, int big, int small, int slot, int small_upper, double* U) {
+  int op = (slot == 2 || slot == 4) ? 8 : 0, orr = (slot == 3 || slot == 4) ? 8 : 0;
+  for (int l = 0; l < 2; ++l)
+    for (int p = ILO; p <= IHI; ++p)
+      for (int r = ILO; r <= IHI; ++r) {
+        int dst[3], src[3];
+        place(d, small_upper ? IHI + l + 1 : l, p, r, dst);
+        place(d, small_upper ? 2 + l / 2 : IHI + l / 2, p / 2 + 1 + op, r / 2 + 1 + orr, src);
+        for (int v = 0; v < NV; ++v)
+          S(U, small, dst[0], dst[1], dst[2], v) = S(U, big, src[0], src[1], src[2], v);
+      }
+}
+
+/* Small cube -> big cube: the big cube's ghost cells of one face quadrant
+ * take 0.125 times the sum of the 8 small cells beneath, in the order
+ * (0,0,0) (1,0,0) (0,1,0) (0,0,1) (1,1,0) (1,0,1) (0,1,1) (1,1,1) of (i,j,k)
+ * offsets (:2026-2310 and :2314-2600 for x). */
+static void restrict8(int d, int big, int small, int slot, int big_upper, double* U) {
+  int op = (slot == 2 || slot == 4) ? 8 : 0, orr = (slot == 3 || slot == 4) ? 8 : 0;
+  for (int l = 0; l < 2; ++l)
+    for (int p = ILO; p <= 9; ++p)
+      for (int r = ILO; r <= 9; ++r) {
+        int dst[3], b[3];
+        place(d, big_upper ? 18 + l : l, p + op, r + orr, dst);
+        place(d, big_upper ? 2 * (l + 1) : 17 - 4 + 2 * (l + 1) - 1, (p - 1) * 2, (r - 1) * 2, b);
+        for (int v = 0; v < NV; ++v) {
+          double s = S(U, small, b[0], b[1], b[2], v);
+          s += S(U, small, b[0] + 1, b[1], b[2], v);
+          s += S(U, small, b[0], b[1] + 1, b[2], v);
+          s += S(U, small, b[0], b[1], b[2] + 1, v);
+          s += S(U, small, b[0] + 1, b[1] + 1, b[2], v);
+          s += S(U, small, b[0] + 1, b[1], b[2] + 1, v);
+          s += S(U, small, b[0], b[1] + 1, b[2] + 1, v);
+          s += S(U, small, b[0] + 1, b[1] + 1, b[2] + 1, v);
+          S(U, big, dst[0], dst[1], dst[2], v) = 0.125 * s;
+        }
+      }
+}
+
+void orc_interface(int ncube, const int* neq, const int* const* eq, const int* nbs,
+                   const int* const* bs, const int* adj_number, double* U) {
   for (int d = 0; d < 3; ++d) {
     for (int a = 1; a <= neq[d]; ++a) {
       int c0 = eq[d][a];
@@ -256,6 +310,26 @@ void orc_interface(int ncube, const int* neq, const int* const* eq, const int* a
             }
           }
     }
+    for (int a = 1; a <= nbs[2 * d]; ++a)
+      for (int s = 1; s <= 4; ++s) {
+        int big = bs[2 * d][a], small = ADJ(big, s, 2 * d + 2);
+        if (small > 0) inject(d, big, small, s, 0, U);
+      }
+    for (int a = 1; a <= nbs[2 * d + 1]; ++a)
+      for (int s = 1; s <= 4; ++s) {
+        int big = bs[2 * d + 1][a], small = ADJ(big, s, 2 * d + 1);
+        if (small > 0) inject(d, big, small, s, 1, U);
+      }
+    for (int a = 1; a <= nbs[2 * d]; ++a)
+      for (int s = 1; s <= 4; ++s) {
+        int big = bs[2 * d][a], small = ADJ(big, s, 2 * d + 2);
+        if (small > 0) restrict8(d, big, small, s, 1, U);
+      }
+    for (int a = 1; a <= nbs[2 * d + 1]; ++a)
+      for (int s = 1; s <= 4; ++s) {
+        int big = bs[2 * d + 1][a], small = ADJ(big, s, 2 * d + 1);
+        if (small > 0) restrict8(d, big, small, s, 0, U);
+      }
   }
   for (int c = 1; c < ncube; ++c) {
     for (int i = ILO; i <= IHI; ++i)
@@ -680,6 +754,8 @@ typedef struct {
   const int* adj_number;
   int neq[3];
   const int* eq[3];
+  int nbs[6];       /* X bs_plus, X bs_minus, Y bs_plus, Y bs_minus, Z bs_plus, Z bs_minus */
+  const int* bs[6];
   int nbc[6];
   const int* bc[6]; /* Xbc_l, Xbc_u, Ybc_l, Ybc_u, Zbc_l, Zbc_u (1-based) */
   int nib[2];
@@ -698,10 +774,10 @@ void orc_stage(const orc_case* cs, int RK, double deltaT, double deltaTau, doubl
   orc_bc_z(cs->nbc[4], cs->nbc[5], cs->bc[4], cs->bc[5], U1_);
   for (int s = 0; s < 2; ++s)
     orc_ghostcell(cs->nib[s], cs->ib_weight[s], cs->ib_gc[s], cs->ib_ip[s], U1_, &cs->phys);
-  orc_interface(cs->ncube, cs->neq, cs->eq, cs->adj_number, U1_);
+  orc_interface(cs->ncube, cs->neq, cs->eq, cs->nbs, cs->bs, cs->adj_number, U1_);
   orc_flux_rk(cs->ncube, cs->mpi_ncube, RK, deltaT, deltaTau, e, cs->FWS, cs->cube_size, U1_,
               U1, U1q, U1p1, U1p2, Rku1, Residual1, er, &cs->phys, nthreads);
-  orc_interface(cs->ncube, cs->neq, cs->eq, cs->adj_number, U1_);
+  orc_interface(cs->ncube, cs->neq, cs->eq, cs->nbs, cs->bs, cs->adj_number, U1_);
 }
 
 /* n_steps physical steps of n_inner inner iterations each (fixed counts, no

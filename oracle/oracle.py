@@ -32,6 +32,8 @@ class OrcCase(ctypes.Structure):
         ("adj_number", IP),
         ("neq", ctypes.c_int * 3),
         ("eq", IP * 3),
+        ("nbs", ctypes.c_int * 6),
+        ("bs", IP * 6),
         ("nbc", ctypes.c_int * 6),
         ("bc", IP * 6),
         ("nib", ctypes.c_int * 2),
@@ -53,7 +55,7 @@ def load():
         lib.orc_bc_y.argtypes = [ci, ci, vp, vp, vp]
         lib.orc_bc_z.argtypes = [ci, ci, vp, vp, vp]
         lib.orc_ghostcell.argtypes = [ci, vp, vp, vp, vp, ctypes.POINTER(OrcPhys)]
-        lib.orc_interface.argtypes = [ci, vp, vp, vp, vp]
+        lib.orc_interface.argtypes = [ci, vp, vp, vp, vp, vp, vp]
         lib.orc_flux_rk.argtypes = [ci, ci, ci, cd, cd, cd, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                     ctypes.POINTER(OrcPhys), ci]
         lib.orc_shift.argtypes = [ci, vp, vp, vp]
@@ -95,9 +97,17 @@ class OracleRun:
         oc.FWS = _iptr(keep("FWS", case.array("FWS").copy()))
         oc.cube_size = _dptr(keep("cube_size", case.array("cube_size").copy()))
         oc.adj_number = _iptr(keep("adj", case.array("adj_number").copy()))
+        nadj = case.array("nadj").copy()
         for d, n in enumerate(("adjX_eq", "adjY_eq", "adjZ_eq")):
-            oc.neq[d] = case.info["nadj" + "XYZ"[d] + "_eq"]
+            oc.neq[d] = int(nadj[d])
             oc.eq[d] = _iptr(keep(n, case.array(n).copy()))
+        # Big cubes with four small neighbours on their + / - side, per axis
+        # (list indices 3 + 4d and 5 + 4d of the 15 interface lists).
+        for d, ax in enumerate("XYZ"):
+            for s, (n, li) in enumerate(((f"adj{ax}_bs_plus", 3 + 4 * d),
+                                         (f"adj{ax}_bs_minus", 5 + 4 * d))):
+                oc.nbs[2 * d + s] = int(nadj[li])
+                oc.bs[2 * d + s] = _iptr(keep(n, case.array(n).copy()))
         for s, n in enumerate(("Xbc_l", "Xbc_u", "Ybc_l", "Ybc_u", "Zbc_l", "Zbc_u")):
             a = keep(n, case.array(n).copy())
             oc.nbc[s] = a.size - 1
@@ -143,7 +153,9 @@ class OracleRun:
         oc = self.oc
         neq = (ctypes.c_int * 3)(*oc.neq)
         eq = (IP * 3)(*oc.eq)
-        self.lib.orc_interface(oc.ncube, neq, eq, oc.adj_number, self._s("U1_"))
+        nbs = (ctypes.c_int * 6)(*oc.nbs)
+        bs = (IP * 6)(*oc.bs)
+        self.lib.orc_interface(oc.ncube, neq, eq, nbs, bs, oc.adj_number, self._s("U1_"))
 
     def flux_update(self, RK):
         oc = self.oc

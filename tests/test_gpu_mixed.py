@@ -41,7 +41,7 @@ def accuracy(case, st, er, ref_state, ref_er):
     return l2, cd_err
 
 
-@pytest.mark.parametrize("name", ["tiny", "cfg1"])
+@pytest.mark.parametrize("name", ["tiny", "cfg1", "ml_tiny"])
 def test_mixed_against_oracle(B, O, name):
     case = B.Case(B.case_params(name))
     n_steps, n_inner = STEPPING[name]

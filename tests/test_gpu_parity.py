@@ -49,7 +49,7 @@ def compare(eng, orc, names):
             raise AssertionError(f"{n}: {describe_mismatch(a, b)}")
 
 
-@pytest.mark.parametrize("name", ["tiny"])
+@pytest.mark.parametrize("name", ["tiny", "ml_tiny"])
 def test_each_kernel_bitwise(B, O, engine, name):
     case = B.Case(B.case_params(name))
     engine.load_case(case)
@@ -67,7 +67,7 @@ def test_each_kernel_bitwise(B, O, engine, name):
     engine.part("shift"); orc.shift(); compare(engine, orc, STATE)
 
 
-@pytest.mark.parametrize("name", ["tiny", "cfg1"])
+@pytest.mark.parametrize("name", ["tiny", "cfg1", "ml_tiny", "ml"])
 def test_run_matches_reference_fixture(B, engine, name):
     g = golden(name)
     case = B.Case(B.case_params(name))
@@ -82,12 +82,14 @@ def test_run_matches_reference_fixture(B, engine, name):
 FLUX_CHOICES = [(0, 1, 0), (1, 1, 1), (1, 1, 2), (1, 1, 0), (1, 2, 0), (1, 4, 0), (1, 16, 0)]
 
 
+@pytest.mark.parametrize("name", ["tiny", "ml_tiny"])
 @pytest.mark.parametrize("variant,nseg,fuse", FLUX_CHOICES)
-def test_flux_variants_bitwise(B, O, engine, variant, nseg, fuse):
+def test_flux_variants_bitwise(B, O, engine, variant, nseg, fuse, name):
     # Every flux kernel choice the autotuner may pick gives the oracle's bits,
     # including cubes where neighbouring cells choose different face orders,
-    # and the RK update fused behind the marching flux.
-    case = B.Case(B.case_params("tiny"))
+    # and the RK update fused behind the marching flux; on one-size and
+    # two-level grids.
+    case = B.Case(B.case_params(name))
     engine.load_case(case)
     engine.set_flux(variant, nseg, fuse)
     orc = O.OracleRun(case, nthreads=8)
@@ -97,7 +99,7 @@ def test_flux_variants_bitwise(B, O, engine, variant, nseg, fuse):
     assert er_close(h, ho)
 
 
-@pytest.mark.parametrize("name", ["tiny", "cfg1"])
+@pytest.mark.parametrize("name", ["tiny", "cfg1", "ml_tiny", "ml"])
 @pytest.mark.parametrize("use_graph", [True, False])
 def test_push_stages_match_reference_fixture(B, engine, name, use_graph):
     # Stages whose flux kernel writes the face halos into the other copy of
@@ -290,6 +292,17 @@ def _dropin_run(B, case, n_steps, n_inner, output_dir=None):
     assert lib.bcm_sync_to_host(h) == 0
     lib.bcm_engine_destroy(h)
     return arrays, np.array(hist)
+
+
+def test_dropin_main_loop_two_level_grid(B):
+    # The drop-in BCM_Interface with two-level lists (injection into small
+    # cubes, restriction into big ones) against the reference fixture.
+    g = golden("ml_tiny")
+    case = B.Case(B.case_params("ml_tiny"))
+    arrays, hist = _dropin_run(B, case, *STEPPING["ml_tiny"])
+    for n in STATE:
+        assert sha(arrays[n]) == g["sha256"][f"final_{n}.bin"], n
+    assert er_close(hist, g["er"])
 
 
 def test_dropin_main_loop_matches_reference_fixture(B, tmp_path):

@@ -119,3 +119,33 @@ def test_pool_rejects_overflow(B):
             e.add_case(cases[2])
     finally:
         e.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_two_level_case_pooled_matches_alone(B, precision):
+    # A two-level case between one-size cases in one pool: its two-level
+    # faces use pool numbering and touch no other case.
+    params = [B.case_params("tiny"), B.case_params("ml_tiny"), B.case_params("tiny", seed=5)]
+    alone = []
+    for p in params:
+        c = B.Case(p)
+        e = B.Engine(0)
+        try:
+            e.alloc_pool(c.ncube - 1, 1, precision)
+            s = e.add_case(c)
+            er = e.run(2, 2)[:, s, :]
+            alone.append(({n: e.case_state(s, n) for n in ("U1_", "U1", "U1q")}, er))
+        finally:
+            e.close()
+    cases = [B.Case(p) for p in params]
+    e = B.Engine(0)
+    try:
+        e.alloc_pool(sum(c.ncube - 1 for c in cases), 3, precision)
+        slots = [e.add_case(c) for c in cases]
+        er = e.run(2, 2)
+        for s, (st, er1) in zip(slots, alone):
+            for n, a in st.items():
+                bitwise(e.case_state(s, n), a, f"slot {s} {n}")
+            bitwise(er[:, s, :], er1, f"slot {s} er")
+    finally:
+        e.close()

@@ -48,3 +48,16 @@ def test_oracle_cfg1_bitwise(B, O):
     for n in STATE:
         assert sha(orc.state[n]) == g["sha256"][f"final_{n}.bin"], n
     assert np.array_equal(hist.view(np.int64), g["er"].view(np.int64))
+
+
+@pytest.mark.parametrize("name", ["ml_tiny", "ml"])
+def test_oracle_two_level_bitwise(B, O, name):
+    # Two-level grids: injection into small cubes' ghosts and 8-cell means
+    # into big cubes' ghosts across every size change.
+    g = golden(name)
+    orc, hist, stages = run_oracle(B, O, name, stage_hashes=True)
+    for k, v in stages.items():
+        assert v == g["sha256"][k], k
+    for n in STATE:
+        assert sha(orc.state[n]) == g["sha256"][f"final_{n}.bin"], n
+    assert np.array_equal(hist.view(np.int64), g["er"].view(np.int64))

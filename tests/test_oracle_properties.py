@@ -168,3 +168,63 @@ def test_interface_copies_and_edge_means(B, O):
         assert np.array_equal(U[c, 18, 18, 1:19], 0.5 * (U[c, 17, 18, 1:19] + U[c, 18, 17, 1:19]))
     # interior untouched
     assert np.array_equal(U[1:, 2:18, 2:18, 2:18], before[1:, 2:18, 2:18, 2:18])
+
+
+def test_two_level_faces_follow_geometry(B, O):
+    # On a two-level grid every small cube's ghost cell across a size change
+    # takes the value of the big cube's cell that contains its centre, and
+    # every big cube's ghost cell the mean of the 8 small cells inside it;
+    # for a field linear in position that mean is the value at its centre.
+    case = B.Case(B.case_params("ml_tiny", sphere_diameter=0.0, noise_amp=0.0))
+    orc = O.OracleRun(case, nthreads=1)
+    nc = case.ncube
+    X, Y, Z = (case.array(a) for a in ("Xcnt", "Ycnt", "Zcnt"))
+    size = case.array("cube_size")
+    lin = np.array([0.3, 1.7, -0.9, 2.3])
+    U = orc.state["U1_"]
+    U[...] = 0.0
+    for c in range(1, nc):
+        f = (lin[0] + lin[1] * X[c][:, None, None] + lin[2] * Y[c][None, :, None]
+             + lin[3] * Z[c][None, None, :])
+        for v in range(5):
+            U[c, 2:18, 2:18, 2:18, v] = (f * (v + 1))[2:18, 2:18, 2:18]
+    orc.interface()
+    adj = case.array("adj_number")
+    csl = case.array("csl")
+    checked_inj = checked_res = 0
+    for c in range(1, nc):
+        h = size[c] / 16
+        for d in range(1, 7):
+            nb = adj[c, 1:5, d]
+            nb = nb[nb > 0]
+            if len(nb) == 0 or csl[nb[0]] == csl[c]:
+                continue
+            ax, up = (d - 1) // 2, d % 2 == 0
+            layers = (18, 19) if up else (0, 1)
+            for l in layers:
+                for p in range(2, 18):
+                    for r in range(2, 18):
+                        idx = [0, 0, 0]
+                        t0, t1 = (1, 2) if ax == 0 else ((0, 2) if ax == 1 else (0, 1))
+                        idx[ax], idx[t0], idx[t1] = l, p, r
+                        centre = np.array([X[c][idx[0]], Y[c][idx[1]], Z[c][idx[2]]])
+                        got = U[c, idx[0], idx[1], idx[2], 0]
+                        if csl[c] < csl[nb[0]]:
+                            # small cube: the big cell containing the centre
+                            big = nb[0]
+                            H = size[big] / 16
+                            lo = np.array([case.array("Xcube")[big], case.array("Ycube")[big],
+                                           case.array("Zcube")[big]])
+                            cell = np.floor((centre - lo) / H).astype(int) + 2
+                            assert got == U[big, cell[0], cell[1], cell[2], 0]
+                            checked_inj += 1
+                        elif p <= 17 and r <= 17:
+                            # big cube: mean of the small cells inside, exact
+                            # for the linear field up to rounding
+                            expect = lin[0] + lin[1:] @ centre
+                            assert abs(got - expect) <= 1e-12 * (1 + abs(expect))
+                            checked_res += 1
+    assert checked_inj > 0 and checked_res > 0
+    # interior untouched
+    for c in range(1, nc):
+        assert np.all(np.isfinite(U[c, 2:18, 2:18, 2:18]))
