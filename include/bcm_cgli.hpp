@@ -50,4 +50,20 @@ inline void BCM_Physical_time_shift(bcm_cgli_types::State U1_, bcm_cgli_types::S
   BCM_Physical_time_shift(&U1_[0][0][0][0][0], &U1[0][0][0][0][0], &U1q[0][0][0][0][0]);
 }
 
+// BCM_Grid (main.h:25-43) with the Array.h pointer types.
+inline void BCM_Grid(int myid, int ncube, int n_wallcube, double* cube_size, int* csl,
+                     double* Xcube, double* Ycube, double* Zcube, double (*Xcnt)[20],
+                     double (*Ycnt)[20], double (*Zcnt)[20], bcm_cgli_types::Flags FWS,
+                     int (*adj_number)[5][7], int* wallcube) {
+  BCM_Grid(myid, ncube, n_wallcube, cube_size, csl, Xcube, Ycube, Zcube, &Xcnt[0][0], &Ycnt[0][0],
+           &Zcnt[0][0], &FWS[0][0][0][0], &adj_number[0][0][0], wallcube);
+}
+// BCM_Output (main.h:515-533) for one rank; rank_map is int[2][MPI_Ncube].
+inline void BCM_Output(int myid, int ncube, int step, int* rank_map, bcm_cgli_types::State U1,
+                       bcm_cgli_types::State U1q, double* cube_size, double (*Xcnt)[20],
+                       double (*Ycnt)[20], double (*Zcnt)[20]) {
+  BCM_Output(myid, ncube, step, rank_map, &U1[0][0][0][0][0], &U1q[0][0][0][0][0], cube_size,
+             &Xcnt[0][0], &Ycnt[0][0], &Zcnt[0][0]);
+}
+
 #endif  // BCM_CGLI_HPP
