@@ -23,7 +23,7 @@ import pytest
 from bcm_cgli_amd.configs import STEPPING
 from helpers import golden, golden_ib
 
-CASES = ["tiny", "cfg1"]
+CASES = ["tiny", "cfg1", "ml_tiny", "ml"]
 
 
 def file_sha(path):
@@ -93,7 +93,9 @@ def test_case_from_files_equals_generated(built, tmp_path):
     for a in ("FWS", "FWS_grid", "U1_", "U1", "U1q", "cube_size", "Xcube", "Ycube", "Zcube",
               "Xcnt", "Ycnt", "Zcnt", "csl", "adj_number", "ib_minus_gc", "ib_minus_ip",
               "ib_minus_w", "ib_plus_gc", "ib_plus_ip", "ib_plus_w", "Xbc_l", "Xbc_u", "Ybc_l",
-              "Ybc_u", "Zbc_l", "Zbc_u", "adjX_eq", "adjY_eq", "adjZ_eq"):
+              "Ybc_u", "Zbc_l", "Zbc_u", "nadj", "adjX_eq", "adjY_eq", "adjZ_eq",
+              *(f"adj{ax}_{kind}" for ax in "XYZ"
+                for kind in ("bs_plus", "sb_plus", "bs_minus", "sb_minus"))):
         assert np.array_equal(back.array(a), case.array(a)), a
 
 
