@@ -22,6 +22,7 @@ with a fixed inner-iteration count, and stores:
 import hashlib
 import json
 import os
+import shutil
 import subprocess
 import sys
 
@@ -101,6 +102,12 @@ def make(name):
     gp, ipp, wp = parse_ibm(os.path.join(run_dir, "IBM_plus00000.dat"))
     np.savez_compressed(os.path.join(HERE, f"{name}_ib.npz"), minus_gc=gm, minus_ip=im, minus_w=wm,
                         plus_gc=gp, plus_ip=ipp, plus_w=wp)
+    # Only the hashes are kept: the dumped arrays and the PLOT3D files go, so
+    # oracle/_ref holds just the reference build and the case inputs.
+    shutil.rmtree(dump)
+    for f in os.listdir(run_dir):
+        if f.endswith((".x", ".q")):
+            os.remove(os.path.join(run_dir, f))
     print(f"{name}: {len(hashes)} arrays, {len(er)} norm rows, IB {len(gm)} minus / {len(gp)} plus")
 
 
