@@ -78,6 +78,25 @@ def read_traffic(kernel, config):
     return float(m.group(1)) if m else None
 
 
+# Vector issue rate of one MI355X: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+# (FP64 and unpacked FP32 instructions issue at this rate).
+VALU_PEAK_TLANE = 256 * 4 * 16 * 2.4e9 / 1e12
+
+
+def read_valu(kernel):
+    """VALU instructions per cell-stage of a flux kernel from the committed
+    SQ counter summary (SQ_INSTS_VALU per wave / 1,024 cell-stages)."""
+    path = os.path.join(ROOT, "profiles", "flux_cfg2_counters.txt")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    at = text.find(kernel)
+    if at < 0:
+        return None
+    m = re.search(r"VALU/wave\s+\d+\s+\((\d+) per cell-stage\)", text[at:])
+    return int(m.group(1)) if m else None
+
+
 def run_pool(B, cases, precision, steps, warmup, n_inner, flux_reps, keep_state):
     """Loads cases into one pool, autotunes, warms up, times `steps` steps.
     Returns timing, the last inner iteration's er per case and (keep_state)
@@ -144,6 +163,25 @@ def roofline(precision, cells, flux_ms, config, push=True):
         "traffic": read_traffic(FLUX_KERNEL[precision], config),
         "algorithmic_bytes_per_launch": algo,
         "kernel_ms": round(flux_ms, 4),
+        "valu": valu_roofline(precision, cells, flux_ms),
+    }
+
+
+def valu_roofline(precision, cells, flux_ms):
+    """The flux kernels are bound by instruction issue, not HBM: the vector
+    instruction rate they reach against the MI355X issue rate."""
+    kern = {"fp64": "k_flux_march<true>", "mixed": "k_flux_mixed"}[precision]
+    per_cell = read_valu(kern)
+    if per_cell is None:
+        return None
+    achieved = per_cell * cells / (flux_ms * 1e-3) / 1e12
+    return {
+        "instr_per_cell_stage": per_cell,
+        "achieved": round(achieved, 3),
+        "peak": round(VALU_PEAK_TLANE, 3),
+        "unit": "T lane-instr/s",
+        "frac": round(achieved / VALU_PEAK_TLANE, 4),
+        "source": "profiles/flux_cfg2_counters.txt (SQ_INSTS_VALU)",
     }
 
 
@@ -275,10 +313,19 @@ def main():
         n_threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
         n_threads = max(1, min(n_threads, int(os.environ.get("OMP_NUM_THREADS", n_threads))))
         v, dt = cpu_baseline(case, n_threads)
+        model = None
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
         out["cpu_baseline"] = {
             "value": round(v, 4),
             "unit": "Mcell-iter/s",
             "cores": n_threads,
+            "host": {"model": model, "nproc": os.cpu_count()},
             "kind": "port",
             "sample": f"one inner iteration (3 RK stages) of the full {args.config} case, "
                       f"{dt:.1f} s, FP64 C oracle with OpenMP over cubes",
