@@ -198,6 +198,33 @@ def cpu_baseline(case, n_threads):
     return case.cells() / dt / 1e6, dt
 
 
+def cpu_baseline_reference(case, params, n_threads, config):
+    """Times the reference solver itself, compiled from its own sources by
+    oracle/build_ref.sh for this case's cube count (present when the build
+    ran where the reference tree is): one inner iteration (3 stages) of the
+    full case, the time loop alone as the binary reports it."""
+    exe = os.path.join(ROOT, "oracle", "_ref", config, "build", "ref_driver")
+    if not os.access(exe, os.X_OK):
+        return None
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        case.write_inputs(d)
+        env = dict(os.environ, OMP_NUM_THREADS=str(n_threads))
+        t0 = time.perf_counter()
+        try:
+            r = subprocess.run([exe, "1", "1", str(params["seed"]), repr(params["noise_amp"]), "-"],
+                               cwd=d, env=env, capture_output=True, text=True, timeout=600)
+        except (OSError, subprocess.TimeoutExpired):
+            return None
+        total = time.perf_counter() - t0
+    m = re.search(r"loop_seconds\s+([0-9.]+)", r.stdout)
+    if r.returncode != 0 or not m:
+        return None
+    loop = float(m.group(1))
+    return case.cells() / loop / 1e6, loop, total
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -321,7 +348,7 @@ def main():
                     break
         except OSError:
             pass
-        out["cpu_baseline"] = {
+        port = {
             "value": round(v, 4),
             "unit": "Mcell-iter/s",
             "cores": n_threads,
@@ -330,6 +357,23 @@ def main():
             "sample": f"one inner iteration (3 RK stages) of the full {args.config} case, "
                       f"{dt:.1f} s, FP64 C oracle with OpenMP over cubes",
         }
+        ref = cpu_baseline_reference(case, B.case_params(args.config), n_threads, args.config)
+        if ref is None:
+            out["cpu_baseline"] = port
+        else:
+            rv, loop, total = ref
+            out["cpu_baseline"] = {
+                "value": round(rv, 4),
+                "unit": "Mcell-iter/s",
+                "cores": n_threads,
+                "host": port["host"],
+                "kind": "reference",
+                "sample": f"one inner iteration (3 RK stages) of the full {args.config} case: the "
+                          f"reference's own routines (oracle/build_ref.sh, -O2, OpenMP over "
+                          f"{n_threads} threads), time loop {loop:.1f} s of a {total:.1f} s run "
+                          f"(setup excluded)",
+                "port_value": port["value"],
+            }
     print(json.dumps(out), flush=True)
 
 

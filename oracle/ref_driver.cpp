@@ -13,6 +13,8 @@
 //
 // Usage (cwd = case directory holding BCMgrid00000.dat and BCM_STL_fine.stl):
 //   ref_driver <n_steps> <n_inner> <seed> <noise_amp> <dump_dir> [dump_stages]
+// With dump_dir "-" nothing is written; the wall time of the time loop is
+// printed as "loop_seconds <s>" (the CPU baseline of the benchmark).
 #include <math.h>
 #include <mpi.h>
 #include <stdint.h>
@@ -74,7 +76,9 @@ static double noise_unit(uint64_t seed, int cube, int i, int j, int k) {
 }
 
 static std::string g_dump;
+static bool g_timing = false;  // dump_dir "-": time the loop, write nothing
 static void dump(const char* name, const void* p, size_t bytes) {
+  if (g_timing) return;
   std::string path = g_dump + "/" + name;
   FILE* f = fopen(path.c_str(), "wb");
   if (!f) { fprintf(stderr, "cannot write %s\n", path.c_str()); exit(3); }
@@ -105,6 +109,7 @@ int main(int argc, char** argv) {
   const uint64_t seed = strtoull(argv[3], nullptr, 10);
   const double noise_amp = atof(argv[4]);
   g_dump = argv[5];
+  g_timing = g_dump == "-";
   const int dump_stages = argc > 6 ? atoi(argv[6]) : 0;
   MPI_Init(&argc, &argv);
   int myid = 0;
@@ -287,7 +292,8 @@ int main(int argc, char** argv) {
   }
 
   // Time loop (main.cpp:1146-1357) with a fixed inner-iteration count.
-  FILE* fer = fopen((g_dump + "/er.txt").c_str(), "w");
+  FILE* fer = g_timing ? fopen("/dev/null", "w") : fopen((g_dump + "/er.txt").c_str(), "w");
+  const double t_loop = MPI_Wtime();
   double E[6] = {0, 0, 0, 0, 0, 0};
   int stage = 0;
   for (int step = 1; step <= n_steps; ++step) {
@@ -331,6 +337,12 @@ int main(int argc, char** argv) {
             }
   }
   fclose(fer);
+  if (g_timing) {
+    // Wall time of the time loop alone (setup excluded), for CPU baselines.
+    printf("loop_seconds %.6f\n", MPI_Wtime() - t_loop);
+    MPI_Finalize();
+    return 0;
+  }
   // PLOT3D files of the last step, written as main.cpp:1346-1351 does after
   // the time-level shift (into the case directory).
   BCM_Output(myid, ncube, n_steps, rank_map, U1_, U1q, cube_size, Xcnt, Ycnt, Zcnt);
