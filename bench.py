@@ -229,7 +229,7 @@ def main():
     args = parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
-              "cases are independent and would shard by cube or by case, see DESIGN.md",
+              "cases are independent and would shard by cube or by case",
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "serve":
