@@ -57,8 +57,8 @@ void free_graph(Engine* e) {
 
 // Tables of the resident cases (rebuilt whenever a case comes or goes).
 void free_tables(Engine* e) {
-  void* ptrs[] = {e->bc_cubes, e->bc_masks, e->halo,        e->halo_pairs,
-                  e->case_desc, e->bc_mask_cube, e->ml};
+  void* ptrs[] = {e->bc_cubes,     e->bc_masks, e->halo,     e->halo_pairs,
+                  e->case_desc,    e->bc_mask_cube, e->ml, e->edge_list};
   for (void* p : ptrs)
     if (p) hipFree(p);
   ghost_free(e->ghost);
@@ -69,7 +69,9 @@ void free_tables(Engine* e) {
   e->halo_pairs = nullptr;
   e->case_desc = nullptr;
   e->ml = nullptr;
+  e->edge_list = nullptr;
   e->n_halo_pairs = e->n_bc = e->n_ghost_cubes = e->n_ghost_entries = e->n_ml = 0;
+  e->n_edge_list = 0;
 }
 
 void free_case(Engine* e) {
@@ -525,6 +527,17 @@ int rebuild_tables(Engine* e, std::string& err) {
   e->n_halo_pairs = (int)pairs.size();
   e->n_ml = (int)ml.size();
   if (dev_upload(&e->ml, ml, err)) return 1;
+  {
+    std::vector<char> mark(top, 0);
+    for (int c : bc) mark[c] = 1;
+    for (const int2& pr : pairs) mark[pr.y / PLANE] = 1;
+    for (const MlTaskDev& t : ml) mark[t.dst] = 1;
+    std::vector<int> list;
+    for (int c = 1; c < top; ++c)
+      if (mark[c]) list.push_back(c);
+    e->n_edge_list = (int)list.size();
+    if (dev_upload(&e->edge_list, list, err)) return 1;
+  }
   if (dev_upload(&e->bc_cubes, bc, err) || dev_upload(&e->bc_masks, bm, err) ||
       ghost_upload(gs, e->ghost, err) || dev_upload(&e->halo, hs, err) ||
       dev_upload(&e->halo_pairs, pairs, err) || dev_upload(&e->case_desc, desc, err) ||
@@ -765,17 +778,24 @@ void engine_interface(Engine* e) {
   }
 }
 
+namespace {
+// IB ghost cells in equal-size gather layers, then (two-level grids) the
+// two-level faces, whose restrictions read small cubes' cells near the face,
+// then the edge passes of the cubes whose face ghosts changed since the
+// stage's opening edge pass (elsewhere they would give the same values).
+template <typename T>
+void halo_refresh(Engine* e, T* U) {
+  launch_halo_pairs(e->stream, U, e->halo_pairs, e->n_halo_pairs);
+  launch_halo_ml(e->stream, U, e->ml, e->n_ml, e->pt());
+  launch_edge_list(e->stream, U, e->edge_list, e->n_edge_list);
+}
+}  // namespace
+
 void engine_halo_refresh(Engine* e) {
-  // IB ghost cells in equal-size gather layers, then (two-level grids) the
-  // two-level faces, whose restrictions read small cubes' cells near the
-  // face, then the edge passes.
-  if (e->prec == 1) {
-    launch_halo_sparse(e->stream, e->Uf, e->halo_pairs, e->n_halo_pairs, e->ncube, e->cube_case);
-    if (e->n_ml) ml_then_edges(e, e->Uf);
-  } else {
-    launch_halo_sparse(e->stream, e->U, e->halo_pairs, e->n_halo_pairs, e->ncube, e->cube_case);
-    if (e->n_ml) ml_then_edges(e, e->U);
-  }
+  if (e->prec == 1)
+    halo_refresh(e, e->Uf);
+  else
+    halo_refresh(e, e->U);
 }
 
 namespace {

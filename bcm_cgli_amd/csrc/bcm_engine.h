@@ -113,6 +113,11 @@ struct Engine {
   // faces and before the edge passes wherever an exchange runs.
   MlTaskDev* ml = nullptr;
   int n_ml = 0;
+  // Cubes whose face ghosts can change between a stage's opening edge pass
+  // and its pre-flux exchange (BC, IB-refresh and two-level destinations):
+  // the pre-flux edge pass covers only these.
+  int* edge_list = nullptr;
+  int n_edge_list = 0;
   // Set when an inner iterate was replaced from outside (state upload, new
   // case): the next pre-flux exchange must then be the full one.
   bool halo_full_next = false;

@@ -131,12 +131,12 @@ void launch_ghost(hipStream_t s, double* U, const GhostTables& g, PhysTab pt);
 void launch_ghost(hipStream_t s, float* U, const GhostTables& g, PhysTab pt);
 void launch_halo_edge(hipStream_t s, double* U, const HaloSrc* src, int ncube, const int* cube_case);
 void launch_halo_edge(hipStream_t s, float* U, const HaloSrc* src, int ncube, const int* cube_case);
-// Pre-flux refresh: copy (source, destination) cell pairs (cube*8000 + cell),
-// then the edge passes of every cube.
-void launch_halo_sparse(hipStream_t s, double* U, const int2* pairs, int npairs, int ncube,
-                        const int* cube_case);
-void launch_halo_sparse(hipStream_t s, float* U, const int2* pairs, int npairs, int ncube,
-                        const int* cube_case);
+// Pre-flux refresh: copy (source, destination) cell pairs (cube*8000 + cell).
+void launch_halo_pairs(hipStream_t s, double* U, const int2* pairs, int npairs);
+void launch_halo_pairs(hipStream_t s, float* U, const int2* pairs, int npairs);
+// Edge passes of the listed cubes.
+void launch_edge_list(hipStream_t s, double* U, const int* list, int n);
+void launch_edge_list(hipStream_t s, float* U, const int* list, int n);
 // Edge passes of every cube, then the domain BCs of boundary cubes
 // (cube_mask[c]: the k_bc side bits of pool cube c, 0 for none).  Used after a
 // flux kernel that wrote the face halos itself.

@@ -1388,6 +1388,12 @@ __global__ __launch_bounds__(WG) void k_edge(T* __restrict__ U,
   edge_fill(U, blockIdx.x + 1);
 }
 
+// Edge passes of the listed cubes only.
+template <typename T>
+__global__ __launch_bounds__(WG) void k_edge_list(T* __restrict__ U, const int* __restrict__ list) {
+  edge_fill(U, list[blockIdx.x]);
+}
+
 // Stage start after a flux kernel that wrote its face halos into the
 // neighbours (flux_push): the edge passes of the post-update exchange
 // (Interface.cpp:5640-5732), then the domain BCs of the next stage, per cube.
@@ -1578,20 +1584,27 @@ void launch_halo_edge(hipStream_t s, float* U, const HaloSrc* src, int ncube,
 }
 
 template <typename T>
-void launch_halo_sparse_t(hipStream_t s, T* U, const int2* pairs, int npairs, int ncube,
-                          const int* cube_case) {
+void launch_halo_pairs_t(hipStream_t s, T* U, const int2* pairs, int npairs) {
   if (npairs > 0)
     hipLaunchKernelGGL(k_halo_sparse<T>, dim3((npairs + WG - 1) / WG), dim3(WG), 0, s, U, pairs,
                        npairs);
-  if (ncube > 1) hipLaunchKernelGGL(k_edge<T>, dim3(ncube - 1), dim3(WG), 0, s, U, cube_case);
 }
-void launch_halo_sparse(hipStream_t s, double* U, const int2* pairs, int npairs, int ncube,
-                        const int* cube_case) {
-  launch_halo_sparse_t(s, U, pairs, npairs, ncube, cube_case);
+void launch_halo_pairs(hipStream_t s, double* U, const int2* pairs, int npairs) {
+  launch_halo_pairs_t(s, U, pairs, npairs);
 }
-void launch_halo_sparse(hipStream_t s, float* U, const int2* pairs, int npairs, int ncube,
-                        const int* cube_case) {
-  launch_halo_sparse_t(s, U, pairs, npairs, ncube, cube_case);
+void launch_halo_pairs(hipStream_t s, float* U, const int2* pairs, int npairs) {
+  launch_halo_pairs_t(s, U, pairs, npairs);
+}
+
+template <typename T>
+void launch_edge_list_t(hipStream_t s, T* U, const int* list, int n) {
+  if (n > 0) hipLaunchKernelGGL(k_edge_list<T>, dim3(n), dim3(WG), 0, s, U, list);
+}
+void launch_edge_list(hipStream_t s, double* U, const int* list, int n) {
+  launch_edge_list_t(s, U, list, n);
+}
+void launch_edge_list(hipStream_t s, float* U, const int* list, int n) {
+  launch_edge_list_t(s, U, list, n);
 }
 
 void launch_flux(hipStream_t s, const double* U, const double* U1, const double* U1q,
