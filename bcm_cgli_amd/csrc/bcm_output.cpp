@@ -37,13 +37,24 @@ struct PinnedBuf {
   size_t bytes = 0;
 };
 
+// One queued output.  Its destructor waits for the writer thread and for any
+// copy into its host buffer before releasing events and memory, so an output
+// abandoned on an error path never frees memory a copy still targets.
 struct OutputJob {
   std::thread th;
   PinnedBuf host;
   hipEvent_t converted = nullptr;
   hipEvent_t copied = nullptr;
+  hipStream_t copy_stream = nullptr;  // set once a copy into host is enqueued
   int rc = 0;
   std::string err;
+  ~OutputJob() {
+    if (th.joinable()) th.join();
+    if (copy_stream) hipStreamSynchronize(copy_stream);
+    if (converted) hipEventDestroy(converted);
+    if (copied) hipEventDestroy(copied);
+    if (host.p) hipHostFree(host.p);
+  }
 };
 
 }  // namespace
@@ -66,9 +77,10 @@ void retire(OutputQueue* q) {
   q->jobs.pop_front();
   if (j->th.joinable()) j->th.join();
   if (j->rc && q->first_err.empty()) q->first_err = j->err;
-  if (j->converted) hipEventDestroy(j->converted);
-  if (j->copied) hipEventDestroy(j->copied);
-  if (j->host.p) q->spare.push_back(j->host);
+  if (j->host.p) {
+    q->spare.push_back(j->host);
+    j->host = PinnedBuf();
+  }
 }
 
 int take_pinned(OutputQueue* q, size_t bytes, PinnedBuf& out, std::string& err) {
@@ -142,6 +154,7 @@ int engine_output(Engine* e, int slot, int step, const std::string& dir, std::st
   HIP_TRY(hipEventRecord(j->converted, e->stream));
   HIP_TRY(hipStreamWaitEvent(q->stream, j->converted, 0));
   HIP_TRY(hipMemcpyAsync(j->host.p + CUBE_STRIDE, q->dev, bytes, hipMemcpyDeviceToHost, q->stream));
+  j->copy_stream = q->stream;
   HIP_TRY(hipEventRecord(j->copied, q->stream));
   HIP_TRY(hipEventRecord(q->dev_free, q->stream));
   q->dev_free_recorded = true;
