@@ -800,3 +800,39 @@ void orc_run(const orc_case* cs, int n_steps, int n_inner, double deltaT, double
     orc_shift(cs->ncube, U1, U1q, U1_);
   }
 }
+
+/* n_steps physical steps under the reference's inner-iteration control
+ * (main.cpp:1249-1320): after each inner iteration E[m] = max(E[m], er[m])
+ * and er[m] /= E[m] for m = 1..5 (the max macro of main.cpp:14); the inner
+ * loop ends when all five are below tol or at iteration max_inner (the
+ * reference's iteration_end_step), then the time levels shift.  E holds the
+ * five running maxima E1..E5 (zero at the start of a run, main.cpp:66-70) and
+ * is updated.  iters[step] receives each step's inner-iteration count; rows of
+ * er_hist (max_inner per step, 7 doubles each, raw er[1..7]) after the break
+ * are left as they were. */
+void orc_run_converge(const orc_case* cs, int n_steps, int max_inner, double tol, double* E,
+                      int* iters, double deltaT, double deltaTau, double e, double* U1_,
+                      double* U1, double* U1q, double* U1p1, double* U1p2, double* Rku1,
+                      double* Residual1, double* er_hist, int nthreads) {
+  double er[8] = {0};
+  for (int step = 0; step < n_steps; ++step) {
+    for (int it = 1;; ++it) {
+      for (int RK = 1; RK <= 3; ++RK)
+        orc_stage(cs, RK, deltaT, deltaTau, e, U1_, U1, U1q, U1p1, U1p2, Rku1, Residual1, er,
+                  nthreads);
+      if (er_hist)
+        for (int m = 0; m < 7; ++m) er_hist[((size_t)step * max_inner + it - 1) * 7 + m] = er[m + 1];
+      int below = 1;
+      for (int m = 1; m <= 5; ++m) {
+        E[m - 1] = E[m - 1] > er[m] ? E[m - 1] : er[m];
+        er[m] = er[m] / E[m - 1];
+        below = below && er[m] < tol;
+      }
+      if (below || it == max_inner) {
+        iters[step] = it;
+        break;
+      }
+    }
+    orc_shift(cs->ncube, U1, U1q, U1_);
+  }
+}

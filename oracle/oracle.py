@@ -63,6 +63,8 @@ def load():
                                   vp, vp, ci]
         lib.orc_run.argtypes = [ctypes.POINTER(OrcCase), ci, ci, cd, cd, cd, vp, vp, vp, vp, vp, vp,
                                 vp, vp, ci]
+        lib.orc_run_converge.argtypes = [ctypes.POINTER(OrcCase), ci, ci, cd, vp, vp, cd, cd, cd, vp,
+                                         vp, vp, vp, vp, vp, vp, vp, ci]
         _lib = lib
     return _lib
 
@@ -181,3 +183,18 @@ class OracleRun:
                          self._s("U1p2"), self._s("Rku1"), self._s("Residual1"),
                          hist.ctypes.data, self.nthreads)
         return hist
+
+    def run_converge(self, n_steps, max_inner, tol, E=None):
+        """Steps under the reference's inner-iteration control; returns
+        (iterations per step, er[1..7] per inner iteration with rows after
+        each step's break left zero, running maxima E1..E5)."""
+        assert max_inner >= 1
+        E = np.zeros(5) if E is None else np.array(E, dtype=np.float64)
+        iters = np.zeros(n_steps, dtype=np.int32)
+        hist = np.zeros((n_steps * max_inner, 7))
+        self.lib.orc_run_converge(ctypes.byref(self.oc), n_steps, max_inner, tol, E.ctypes.data,
+                                  iters.ctypes.data, self.deltaT, self.deltaTau, self.e,
+                                  self._s("U1_"), self._s("U1"), self._s("U1q"), self._s("U1p1"),
+                                  self._s("U1p2"), self._s("Rku1"), self._s("Residual1"),
+                                  hist.ctypes.data, self.nthreads)
+        return iters, hist, E

@@ -4,15 +4,20 @@
 // It links the reference solver's own translation units and calls them in
 // the order of the reference main program (main.cpp:99-1357) on the grid and
 // surface files of one generated case, with two changes: the inner-iteration
-// and physical-step counts are arguments with the convergence break disabled,
-// and fluid densities get the seeded noise the cases prescribe between the
-// initial condition and the U1 -> U1_/U1q copy.  Arrays are dumped as raw
+// and physical-step counts are arguments (the convergence break is disabled
+// unless a tolerance is given), and fluid densities get the seeded noise the
+// cases prescribe between the initial condition and the U1 -> U1_/U1q copy.  Arrays are dumped as raw
 // little-endian binaries in Array.h layout for the fixture writer; the
 // reference's own writers leave IBM_minus/plus00000.dat and the PLOT3D files
 // BCMdata00001.x and qBCMdata<n_steps>.q in the case directory.
 //
 // Usage (cwd = case directory holding BCMgrid00000.dat and BCM_STL_fine.stl):
-//   ref_driver <n_steps> <n_inner> <seed> <noise_amp> <dump_dir> [dump_stages]
+//   ref_driver <n_steps> <n_inner> <seed> <noise_amp> <dump_dir> [dump_stages] [tol]
+// With tol > 0 the inner loop ends as main.cpp:1249-1320 ends it: after each
+// inner iteration E[m] = max(E[m], er[m]) and er[m] /= E[m] (m = 1..5), and
+// the loop breaks when all five are below tol or at iteration n_inner (the
+// reference's iteration_end_step; its tolerance is 0.02).  The inner-iteration
+// count of every step is written to iters.txt.
 // With dump_dir "-" nothing is written; the wall time of the time loop is
 // printed as "loop_seconds <s>" (the CPU baseline of the benchmark).
 #include <math.h>
@@ -101,7 +106,7 @@ static int read_count(const char* file, const char* marker) {
 
 int main(int argc, char** argv) {
   if (argc < 6) {
-    fprintf(stderr, "usage: ref_driver n_steps n_inner seed noise_amp dump_dir [dump_stages]\n");
+    fprintf(stderr, "usage: ref_driver n_steps n_inner seed noise_amp dump_dir [dump_stages] [tol]\n");
     return 2;
   }
   const int n_steps = atoi(argv[1]);
@@ -111,6 +116,7 @@ int main(int argc, char** argv) {
   g_dump = argv[5];
   g_timing = g_dump == "-";
   const int dump_stages = argc > 6 ? atoi(argv[6]) : 0;
+  const double tol = argc > 7 ? atof(argv[7]) : 0.0;
   MPI_Init(&argc, &argv);
   int myid = 0;
 
@@ -293,6 +299,7 @@ int main(int argc, char** argv) {
 
   // Time loop (main.cpp:1146-1357) with a fixed inner-iteration count.
   FILE* fer = g_timing ? fopen("/dev/null", "w") : fopen((g_dump + "/er.txt").c_str(), "w");
+  FILE* fit = (tol > 0 && !g_timing) ? fopen((g_dump + "/iters.txt").c_str(), "w") : nullptr;
   const double t_loop = MPI_Wtime();
   double E[6] = {0, 0, 0, 0, 0, 0};
   int stage = 0;
@@ -322,9 +329,15 @@ int main(int argc, char** argv) {
       }
       fprintf(fer, "%d %d %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", step, iteration, er[1],
               er[2], er[3], er[4], er[5], er[6], er[7]);
+      bool below = true;
       for (int m = 1; m <= 5; ++m) {
-        E[m] = fmax(E[m], er[m]);
+        E[m] = E[m] > er[m] ? E[m] : er[m];  // the max macro of main.cpp:14
         er[m] = er[m] / E[m];
+        below = below && er[m] < tol;
+      }
+      if (tol > 0 && (below || iteration == n_inner)) {
+        if (fit) fprintf(fit, "%d %d\n", step, iteration);
+        break;
       }
     }
     for (int c = 1; c < ncube; ++c)
@@ -337,6 +350,7 @@ int main(int argc, char** argv) {
             }
   }
   fclose(fer);
+  if (fit) fclose(fit);
   if (g_timing) {
     // Wall time of the time loop alone (setup excluded), for CPU baselines.
     printf("loop_seconds %.6f\n", MPI_Wtime() - t_loop);

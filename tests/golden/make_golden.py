@@ -2,7 +2,7 @@
 
 Runs in the build container only (it needs the reference tree and MPICH):
 
-    python tests/golden/make_golden.py [tiny] [cfg1]
+    python tests/golden/make_golden.py [tiny] [cfg1] [tiny:converge] ...
 
 For each case it writes the grid and surface files with the case generator,
 builds the reference with oracle/build_ref.sh (MPI_Ncube, U0, Uin and mu_L of
@@ -18,6 +18,11 @@ with a fixed inner-iteration count, and stores:
   and the PLOT3D grid and solution files BCMdata00001.x, qBCMdata<n_steps>.q;
 * ``<case>_ib.npz`` -- the IB tables the reference wrote (IBM_minus/plus),
   parsed into integer and float arrays.
+
+``<case>:converge`` runs the driver with the convergence break enabled
+(steps, iteration cap and tolerance from ``CONVERGE``) and writes
+``<case>_converge_golden.json``: the inner-iteration count of every step,
+SHA-256 of the final arrays and the norm history of the iterations run.
 """
 import hashlib
 import json
@@ -33,7 +38,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 import bcm_cgli_amd as B  # noqa: E402
-from bcm_cgli_amd.configs import STEPPING  # noqa: E402
+from bcm_cgli_amd.configs import CONVERGE, STEPPING  # noqa: E402
 
 REF_ROOT = os.path.join(ROOT, "oracle", "_ref")
 
@@ -111,6 +116,57 @@ def make(name):
     print(f"{name}: {len(hashes)} arrays, {len(er)} norm rows, IB {len(gm)} minus / {len(gp)} plus")
 
 
+def make_converge(name):
+    """Fixture of a run under the reference's inner-iteration control: the
+    driver's tolerance argument enables the break of main.cpp:1271."""
+    params = B.case_params(name)
+    n_steps, max_inner, tol = CONVERGE[name]
+    case = B.Case(params)
+    work = os.path.join(REF_ROOT, name)
+    run_dir = os.path.join(work, "run")
+    dump = os.path.join(work, "dump_converge")
+    os.makedirs(run_dir, exist_ok=True)
+    os.makedirs(dump, exist_ok=True)
+    case.write_inputs(run_dir)
+    subprocess.run(["bash", os.path.join(ROOT, "oracle", "build_ref.sh"), os.path.join(work, "build"),
+                    str(case.ncube - 1), repr(case.phys["U0"]), repr(case.phys["mu_L"])],
+                   check=True, stdout=subprocess.DEVNULL)
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    subprocess.run([os.path.join(work, "build", "ref_driver"), str(n_steps), str(max_inner),
+                    str(params["seed"]), repr(params["noise_amp"]), dump, "0", repr(tol)],
+                   cwd=run_dir, env=env, check=True, stdout=subprocess.DEVNULL)
+    hashes = {f: sha(os.path.join(dump, f)) for f in sorted(os.listdir(dump))
+              if f.startswith("final_") and f.endswith(".bin")}
+    with open(os.path.join(dump, "iters.txt")) as f:
+        iters = [int(line.split()[1]) for line in f if line.strip()]
+    er = []
+    with open(os.path.join(dump, "er.txt")) as f:
+        for line in f:
+            v = line.split()
+            er.append([int(v[0]), int(v[1])] + [float(x).hex() for x in v[2:9]])
+    out = {
+        "case": name,
+        "n_steps": n_steps,
+        "max_inner": max_inner,
+        "tol": tol,
+        "iters": iters,
+        "sha256": hashes,
+        "er_hex": er,
+        "source": "reference sources compiled by oracle/build_ref.sh, driven by oracle/ref_driver.cpp "
+                  "with the convergence break enabled",
+    }
+    with open(os.path.join(HERE, f"{name}_converge_golden.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    shutil.rmtree(dump)
+    for f in os.listdir(run_dir):
+        if f.endswith((".x", ".q")):
+            os.remove(os.path.join(run_dir, f))
+    print(f"{name} converge: iterations per step {iters}")
+
+
 if __name__ == "__main__":
     for n in (sys.argv[1:] or ["tiny", "cfg1"]):
-        make(n)
+        if n.endswith(":converge"):
+            make_converge(n.split(":")[0])
+        else:
+            make(n)

@@ -19,6 +19,19 @@ def golden(name):
     return g
 
 
+def golden_converge(name):
+    """Fixture of a run under the reference's convergence break: iterations per
+    step, final-array hashes and the raw norms er[1..7] of every iteration run,
+    laid out as (steps * max_inner, 7) with zero rows after each break."""
+    with open(os.path.join(GOLDEN, f"{name}_converge_golden.json")) as f:
+        g = json.load(f)
+    er = np.zeros((g["n_steps"] * g["max_inner"], 7))
+    for row in g["er_hex"]:
+        er[(row[0] - 1) * g["max_inner"] + row[1] - 1] = [float.fromhex(x) for x in row[2:]]
+    g["er"] = er
+    return g
+
+
 def golden_ib(name):
     return np.load(os.path.join(GOLDEN, f"{name}_ib.npz"))
 

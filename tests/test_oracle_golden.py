@@ -7,8 +7,8 @@ reference's sequential order, so even the norms match exactly).
 import numpy as np
 import pytest
 
-from bcm_cgli_amd.configs import STEPPING
-from helpers import golden, sha
+from bcm_cgli_amd.configs import CONVERGE, STEPPING
+from helpers import golden, golden_converge, sha
 
 STATE = ("U1_", "U1", "U1q", "U1p1", "U1p2", "Rku1", "Residual1")
 
@@ -61,3 +61,21 @@ def test_oracle_two_level_bitwise(B, O, name):
     for n in STATE:
         assert sha(orc.state[n]) == g["sha256"][f"final_{n}.bin"], n
     assert np.array_equal(hist.view(np.int64), g["er"].view(np.int64))
+
+
+@pytest.mark.parametrize("name", ["tiny", "ml_tiny"])
+def test_oracle_convergence_break_bitwise(B, O, name):
+    # The reference's inner-iteration control (running-max normalised norms
+    # below the tolerance, or the iteration cap) ends each step's inner loop
+    # at the iteration the reference ends it, and the runs agree bitwise.
+    g = golden_converge(name)
+    n_steps, max_inner, tol = CONVERGE[name]
+    assert len(set(g["iters"])) > 2 and max(g["iters"]) == max_inner
+    orc = O.OracleRun(B.Case(B.case_params(name)), nthreads=8)
+    iters, hist, E = orc.run_converge(n_steps, max_inner, tol)
+    assert iters.tolist() == g["iters"]
+    for n in STATE:
+        assert sha(orc.state[n]) == g["sha256"][f"final_{n}.bin"], n
+    assert np.array_equal(hist.view(np.int64), g["er"].view(np.int64))
+    # E1..E5 are the running maxima of the norms over every iteration run.
+    assert np.array_equal(E, g["er"][:, :5].max(axis=0))
