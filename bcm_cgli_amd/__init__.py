@@ -68,6 +68,8 @@ def load_library(path=LIB_PATH):
     lib.bcm_engine_load_case.argtypes = [c_void_p, c_void_p]
     lib.bcm_engine_set_time.argtypes = [c_void_p, c_double, c_double, c_double]
     lib.bcm_engine_run.argtypes = [c_void_p, c_int, c_int, c_void_p, c_int]
+    lib.bcm_engine_run_converge.argtypes = [c_void_p, c_int, c_int, ctypes.c_double, c_void_p,
+                                            c_void_p, c_void_p]
     lib.bcm_engine_part.argtypes = [c_void_p, c_int, c_int]
     lib.bcm_engine_get_state.argtypes = [c_void_p, c_int, c_void_p]
     lib.bcm_engine_set_state.argtypes = [c_void_p, c_int, c_void_p]
@@ -351,6 +353,28 @@ class Engine:
             return None
         er = er[: n_steps * n_inner]
         return er[:, 0, :] if self.case is not None else er
+
+    def run_converge(self, n_steps, max_inner, tol=0.02, E=None, record_er=True):
+        """Steps under the reference's inner-iteration control
+        (main.cpp:1249-1320): each case's inner loop ends when its five
+        running-max normalised norms are below tol, or after max_inner
+        iterations.  E: running maxima E1..E5 per case slot (zeros, the
+        reference's start, when None).  Returns (iterations per step, norms
+        er[1..7] per inner iteration with zero rows after each break, E);
+        per-slot axes are dropped for a single case."""
+        mc = self.max_cases
+        E = np.zeros((mc, 5)) if E is None else np.array(E, dtype=np.float64).reshape(mc, 5).copy()
+        iters = np.zeros((max(n_steps, 1), mc), dtype=np.int32)
+        er = np.zeros((max(n_steps, 1) * max_inner, mc, 7)) if record_er else None
+        _check(_lib.bcm_engine_run_converge(self._h, int(n_steps), int(max_inner), float(tol),
+                                            E.ctypes.data, iters.ctypes.data,
+                                            er.ctypes.data if record_er else None))
+        iters = iters[:n_steps]
+        if record_er:
+            er = er[: n_steps * max_inner]
+        if self.case is not None:
+            return iters[:, 0], (er[:, 0, :] if record_er else None), E[0]
+        return iters, er, E
 
     def part(self, name, RK=1):
         _check(_lib.bcm_engine_part(self._h, PARTS[name], int(RK)))

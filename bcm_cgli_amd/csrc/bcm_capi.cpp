@@ -521,6 +521,17 @@ int bcm_engine_run(bcm_engine* h, int n_steps, int n_inner, double* er_hist, int
   return 0;
 }
 
+int bcm_engine_run_converge(bcm_engine* h, int n_steps, int max_inner, double tol, double* E,
+                            int* iters, double* er_hist) {
+  if (!h) return fail("null engine");
+  if (n_steps < 0 || max_inner < 1) return fail("invalid step counts");
+  if (!E || !iters) return fail("null norm maxima or iteration counts");
+  std::string err;
+  if (engine_run_converge(h->e, n_steps, max_inner, tol, E, iters, er_hist, err)) return fail(err);
+  if (engine_sync(h->e, err)) return fail(err);
+  return 0;
+}
+
 int bcm_engine_part(bcm_engine* h, int part, int RK) {
   if (!h) return fail("null engine");
   Engine* e = h->e;

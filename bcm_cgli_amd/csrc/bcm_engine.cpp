@@ -1065,6 +1065,143 @@ int engine_run(Engine* e, int n_steps, int n_inner, double* er_hist, bool use_gr
   return 0;
 }
 
+namespace {
+
+// Completes the exchange the last flux kernel began (two-level faces and edge
+// passes), so that the iterate holds every ghost value a closing
+// BCM_Interface would give.
+void complete_exchange(Engine* e) {
+  if (e->prec == 1)
+    ml_then_edges(e, e->Uf);
+  else if (e->flux_push)
+    ml_then_edges(e, e->U);
+}
+
+// Physical-time shift of one case's cubes (pool cubes first..first+n-1): the
+// shift kernels see the case's range as cubes 1..n of a smaller pool.
+void shift_case(Engine* e, const CaseSlot& S) {
+  const size_t off = (size_t)(S.first - 1) * CUBE_STRIDE;
+  int* cc = e->cube_case + (S.first - 1);
+  if (e->prec == 1)
+    launch_shift_mixed(e->stream, e->Uf + off, e->U1 + off, e->U1q + off, S.tb.n + 1,
+                       PhysTab{e->phys_dev, cc});
+  else
+    launch_shift(e->stream, e->U + off, e->U1 + off, e->U1q + off, S.tb.n + 1, cc);
+}
+
+// Copies one case's cubes of the current iterate to (save) or from the
+// staging buffer, which no run uses otherwise.
+void stash_case(Engine* e, const CaseSlot& S, bool save) {
+  const size_t off = (size_t)S.first * CUBE_STRIDE, n = (size_t)S.tb.n * CUBE_STRIDE;
+  if (e->prec == 1) {
+    float* st = reinterpret_cast<float*>(e->staging) + off;
+    if (save)
+      launch_copy_f(e->stream, e->Uf + off, st, n);
+    else
+      launch_copy_f(e->stream, st, e->Uf + off, n);
+  } else {
+    double* st = e->staging + off;
+    if (save)
+      launch_copy_d(e->stream, e->U + off, st, n);
+    else
+      launch_copy_d(e->stream, st, e->U + off, n);
+  }
+}
+
+}  // namespace
+
+int engine_run_converge(Engine* e, int n_steps, int max_inner, double tol, double* E, int* iters,
+                        double* er_hist, std::string& err) {
+  HIP_TRY(hipSetDevice(e->device));
+  if (e->n_active == 0) {
+    err = "no case loaded";
+    return 1;
+  }
+  if (n_steps < 0 || max_inner < 1) {
+    err = "invalid step counts";
+    return 1;
+  }
+  if (ensure_er(e, 1, err)) return 1;
+  if (!e->flux_tuned) {
+    int v, n;
+    if (engine_autotune(e, &v, &n, err)) return 1;
+  }
+  const int mc = e->max_cases;
+  const size_t row = (size_t)mc * 8;
+  std::vector<int> done(mc);
+  std::vector<int> stashed;
+  for (int s = 0; s < n_steps; ++s) {
+    float* startf = e->Uf;
+    double* startd = e->U;
+    int running = 0;
+    for (int c = 0; c < mc; ++c) {
+      done[c] = e->cases[c].active ? 0 : -1;
+      running += e->cases[c].active;
+    }
+    stashed.clear();
+    const bool full = e->halo_full_next;
+    e->halo_full_next = false;
+    for (int it = 1; running > 0; ++it) {
+      for (int RK = 1; RK <= 3; ++RK) engine_stage(e, RK, 0, full && it == 1 && RK == 1);
+      HIP_TRY(hipMemcpyAsync(e->er_host, e->er_dev, row * sizeof(double), hipMemcpyDeviceToHost,
+                             e->stream));
+      HIP_TRY(hipStreamSynchronize(e->stream));
+      std::vector<int> ended;
+      for (int c = 0; c < mc; ++c) {
+        if (done[c] != 0) continue;
+        double* er = e->er_host + c * 8;
+        if (er_hist)
+          for (int m = 0; m < 7; ++m)
+            er_hist[(((size_t)s * max_inner + it - 1) * mc + c) * 7 + m] = er[1 + m];
+        // main.cpp:1249-1271: running maxima (the max macro of main.cpp:14),
+        // normalised norms, and the break test.
+        bool below = true;
+        for (int m = 1; m <= 5; ++m) {
+          double& Em = E[(size_t)c * 5 + m - 1];
+          Em = Em > er[m] ? Em : er[m];
+          below = below && er[m] / Em < tol;
+        }
+        if (below || it == max_inner) ended.push_back(c);
+      }
+      for (int c : ended) done[c] = it;
+      running -= (int)ended.size();
+      if (running > 0 && !ended.empty()) {
+        // Cases that end their inner loop while others go on: shift them now
+        // and keep their iterate aside until the step ends; their cubes keep
+        // being computed with the rest, and the results are discarded.
+        complete_exchange(e);
+        for (int c : ended) {
+          shift_case(e, e->cases[c]);
+          stash_case(e, e->cases[c], true);
+          stashed.push_back(c);
+        }
+      }
+    }
+    if (stashed.empty()) {
+      engine_shift(e);
+    } else {
+      for (int c = 0; c < mc; ++c)
+        if (done[c] > 0 && std::find(stashed.begin(), stashed.end(), c) == stashed.end())
+          shift_case(e, e->cases[c]);
+      for (int c : stashed) stash_case(e, e->cases[c], false);
+      // Restored iterates: the next pre-flux exchange is the full one.
+      e->halo_full_next = true;
+    }
+    if (e->prec == 1 && e->Uf != startf) {
+      launch_copy_f(e->stream, e->Uf, startf, (size_t)e->ncube * CUBE_STRIDE);
+      std::swap(e->Uf, e->Uf2);
+    }
+    if (e->prec == 0 && e->U != startd) {
+      launch_copy_d(e->stream, e->U, startd, (size_t)e->ncube * CUBE_STRIDE);
+      std::swap(e->U, e->U2);
+    }
+    for (int c = 0; c < mc; ++c) iters[(size_t)s * mc + c] = done[c] > 0 ? done[c] : 0;
+  }
+  if (n_steps > 0) complete_exchange(e);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 int engine_sync(Engine* e, std::string& err) {
   HIP_TRY(hipStreamSynchronize(e->stream));
   HIP_TRY(hipGetLastError());

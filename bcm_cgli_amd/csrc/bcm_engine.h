@@ -193,6 +193,16 @@ void engine_stage(Engine* e, int RK, int er_slot, bool full_pre_halo = false);
 // er_hist (may be null) receives er[1..7] as [step][inner][max_cases][7].
 int engine_run(Engine* e, int n_steps, int n_inner, double* er_hist, bool use_graph,
                std::string& err);
+// n_steps physical steps under the reference's inner-iteration control
+// (main.cpp:1249-1320): per case, after each inner iteration E[m] =
+// max(E[m], er[m]) and the step's inner loop ends when every er[m] / E[m]
+// (m = 1..5) is below tol or after max_inner iterations; then the case's
+// time levels shift.  E: [max_cases][5] running maxima, read and updated.
+// iters: [n_steps][max_cases] inner iterations run (0 for empty slots).
+// er_hist (may be null): er[1..7] as [step][inner][max_cases][7], rows after
+// a case's break untouched.
+int engine_run_converge(Engine* e, int n_steps, int max_inner, double tol, double* E, int* iters,
+                        double* er_hist, std::string& err);
 int engine_sync(Engine* e, std::string& err);
 // Times one RK stage for every flux choice on the resident cases (state
 // restored afterwards) and keeps the fastest.  Returns the chosen

@@ -216,6 +216,20 @@ int bcm_engine_set_time(bcm_engine* e, double deltaT, double deltaTau, double e_
  * inner iteration: [n_steps][n_inner][max_cases][7] doubles (max_cases is 1
  * after bcm_engine_load_case). */
 int bcm_engine_run(bcm_engine* e, int n_steps, int n_inner, double* er_hist, int use_graph);
+/* n_steps physical steps under the reference's inner-iteration control, the
+ * loop of main.cpp:1146-1320: per case, after every inner iteration
+ * E[m] = max(E[m], er[m]) and er[m] /= E[m] (m = 1..5); the inner loop ends
+ * when all five are below tol or at iteration max_inner, and the case's time
+ * levels shift (the reference's tol is 0.02, max_inner its
+ * iteration_end_step).  Each case decides for itself; a case that ends before
+ * others is shifted at once and its iterate is kept aside until the step ends.
+ * E: [max_cases][5] running maxima E1..E5, read and updated (zeros at the
+ * start of a run, as main.cpp:66-70).  iters: [n_steps][max_cases] inner
+ * iterations of each step (0 for empty slots).  er_hist (may be null):
+ * [n_steps][max_inner][max_cases][7] doubles, er[1..7] of the iterations each
+ * case ran; rows after its break are left untouched. */
+int bcm_engine_run_converge(bcm_engine* e, int n_steps, int max_inner, double tol, double* E,
+                            int* iters, double* er_hist);
 /* One piece of a stage: 0 domain BCs, 1 IB ghost fill, 2 halo + edges,
  * 3 flux, 4 RK update (with norms at RK 3), 5 time shift, 6 whole stage,
  * 7 flux and RK update as a stage runs them (fused when the choice allows),
