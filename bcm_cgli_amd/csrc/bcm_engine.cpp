@@ -58,7 +58,7 @@ void free_graph(Engine* e) {
 // Tables of the resident cases (rebuilt whenever a case comes or goes).
 void free_tables(Engine* e) {
   void* ptrs[] = {e->bc_cubes,     e->bc_masks, e->halo,     e->halo_pairs,
-                  e->case_desc,    e->bc_mask_cube, e->ml, e->edge_list};
+                  e->case_desc,    e->bc_mask_cube, e->ml, e->edge_list, e->open_list};
   for (void* p : ptrs)
     if (p) hipFree(p);
   ghost_free(e->ghost);
@@ -70,8 +70,9 @@ void free_tables(Engine* e) {
   e->case_desc = nullptr;
   e->ml = nullptr;
   e->edge_list = nullptr;
+  e->open_list = nullptr;
   e->n_halo_pairs = e->n_bc = e->n_ghost_cubes = e->n_ghost_entries = e->n_ml = 0;
-  e->n_edge_list = 0;
+  e->n_edge_list = e->n_open_list = 0;
 }
 
 void free_case(Engine* e) {
@@ -538,6 +539,32 @@ int rebuild_tables(Engine* e, std::string& err) {
     e->n_edge_list = (int)list.size();
     if (dev_upload(&e->edge_list, list, err)) return 1;
   }
+  {
+    // 2: the IB fill touches an edge cell (all edge passes), 1: boundary cube
+    // only (the edge cells its BCs read).
+    std::vector<char> mark(top, 0);
+    for (int c : bc) mark[c] = 1;
+    for (const GhostEntry& E : gs.entries) {
+      bool touch = edge_cell(E.gi, E.gj, E.gk);
+      for (int d = 0; d < 8; ++d)
+        touch = touch || edge_cell(E.pi + (d & 1), E.pj + ((d >> 1) & 1), E.pk + ((d >> 2) & 1));
+      if (touch) mark[E.cube] = 2;
+    }
+    // The refresh copies land on face ghost cells, which the flux kernels
+    // read from memory; an edge destination would be skipped by them.
+    for (const int2& pr : pairs) {
+      const int cell = pr.y % PLANE;
+      if (edge_cell(cell / 400, cell / 20 % 20, cell % 20)) {
+        err = "an IB refresh copy lands on an edge ghost cell";
+        return 1;
+      }
+    }
+    std::vector<int> list;
+    for (int c = 1; c < top; ++c)
+      if (mark[c]) list.push_back(mark[c] == 2 ? c : -c);
+    e->n_open_list = (int)list.size();
+    if (dev_upload(&e->open_list, list, err)) return 1;
+  }
   if (dev_upload(&e->bc_cubes, bc, err) || dev_upload(&e->bc_masks, bm, err) ||
       ghost_upload(gs, e->ghost, err) || dev_upload(&e->halo, hs, err) ||
       dev_upload(&e->halo_pairs, pairs, err) || dev_upload(&e->case_desc, desc, err) ||
@@ -783,20 +810,39 @@ namespace {
 // two-level faces, whose restrictions read small cubes' cells near the face,
 // then the edge passes of the cubes whose face ghosts changed since the
 // stage's opening edge pass (elsewhere they would give the same values).
+// Before a halo-pushing flux kernel (which forms the edge cells it reads)
+// the edge passes are left out.
 template <typename T>
-void halo_refresh(Engine* e, T* U) {
+void halo_refresh(Engine* e, T* U, bool edges) {
   launch_halo_pairs(e->stream, U, e->halo_pairs, e->n_halo_pairs);
   launch_halo_ml(e->stream, U, e->ml, e->n_ml, e->pt());
-  launch_edge_list(e->stream, U, e->edge_list, e->n_edge_list);
+  if (edges) launch_edge_list(e->stream, U, e->edge_list, e->n_edge_list);
 }
 }  // namespace
 
 void engine_halo_refresh(Engine* e) {
   if (e->prec == 1)
-    halo_refresh(e, e->Uf);
+    halo_refresh(e, e->Uf, true);
   else
-    halo_refresh(e, e->U);
+    halo_refresh(e, e->U, true);
 }
+
+namespace {
+// Opening of a stage after a halo-pushing flux kernel: two-level faces, then
+// the edge passes and BCs of the cubes that need their edge cells in memory,
+// then the IB fill and the pre-flux exchange (full, or the refresh without
+// edge passes).
+template <typename T>
+void push_stage_open(Engine* e, T* U, bool full_pre_halo) {
+  launch_halo_ml(e->stream, U, e->ml, e->n_ml, e->pt());
+  launch_edge_bc_list(e->stream, U, e->bc_mask_cube, e->open_list, e->n_open_list, e->pt());
+  engine_ghost(e);
+  if (full_pre_halo)
+    engine_interface(e);
+  else
+    halo_refresh(e, U, false);
+}
+}  // namespace
 
 namespace {
 double* er_row(Engine* e, int er_slot) { return e->er_dev + (size_t)er_slot * e->max_cases * 8; }
@@ -940,27 +986,15 @@ void engine_flux_push(Engine* e, int RK, int er_slot) {
 void engine_stage(Engine* e, int RK, int er_slot, bool full_pre_halo) {
   if (e->prec == 1) {
     // The previous flux kernel wrote the equal-size face halos; the
-    // two-level faces and the edge passes of that exchange and this stage's
-    // BCs open the stage (all idempotent when no interior cell changed, as at
-    // the start of a run).
-    launch_halo_ml(e->stream, e->Uf, e->ml, e->n_ml, e->pt());
-    launch_edge_bc(e->stream, e->Uf, e->bc_mask_cube, e->ncube, e->pt());
-    engine_ghost(e);
-    if (full_pre_halo)
-      engine_interface(e);
-    else
-      engine_halo_refresh(e);
+    // two-level faces and, where needed, the edge passes of that exchange and
+    // this stage's BCs open the stage (all idempotent when no interior cell
+    // changed, as at the start of a run).
+    push_stage_open(e, e->Uf, full_pre_halo);
     engine_flux_update(e, RK, er_slot);
     return;
   }
   if (e->flux_push) {
-    launch_halo_ml(e->stream, e->U, e->ml, e->n_ml, e->pt());
-    launch_edge_bc(e->stream, e->U, e->bc_mask_cube, e->ncube, e->pt());
-    engine_ghost(e);
-    if (full_pre_halo)
-      engine_interface(e);
-    else
-      engine_halo_refresh(e);
+    push_stage_open(e, e->U, full_pre_halo);
     engine_flux_push(e, RK, er_slot);
     return;
   }

@@ -118,6 +118,14 @@ struct Engine {
   // the pre-flux edge pass covers only these.
   int* edge_list = nullptr;
   int n_edge_list = 0;
+  // Cubes whose edge ghost cells must be in memory when a stage opens after a
+  // flux kernel that pushed its face halos: cubes whose IB fill touches an
+  // edge cell (entry c: all edge passes, then BCs) and other boundary cubes
+  // (entry -c: the k-edge cells the z-BC copies, then BCs).  Such stages run
+  // the opening edge passes and BCs on these cubes only; the pushing flux
+  // kernels form the edge cells they read themselves.
+  int* open_list = nullptr;
+  int n_open_list = 0;
   // Set when an inner iterate was replaced from outside (state upload, new
   // case): the next pre-flux exchange must then be the full one.
   bool halo_full_next = false;

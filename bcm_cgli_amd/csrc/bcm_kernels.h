@@ -29,6 +29,41 @@ __host__ __device__ inline void ref_state(const DevPhys& ph, double (&u)[5]) {
   u[4] = ph.P0 / (1.4 - 1) + 0.5 * ph.rho0 * (ph.Uin * ph.Uin + ph.V0 * ph.V0 + ph.W0 * ph.W0);
 }
 
+// Edge ghost cells (layer-1/18 lines where two ghost layers meet).  The edge
+// passes of an exchange (Interface.cpp:5640-5732) set each edge cell a flux
+// kernel reads to the mean of two face ghost cells of the same cube: the
+// corners (j, k in {1, 18}) of interior planes i = 2..17 (i-edges), and in
+// the ghost planes i = 1, 18 the border cells with one of j, k in {1, 18}
+// (k-edges along j = 1, 18, j-edges along k = 1, 18).  For a cell (i, j, k)
+// with i, j, k in 1..18 this returns true and the two source cells (flat
+// i*400 + j*20 + k) when the cell is one of those; cube corners, which no
+// flux stencil reads, and all other cells return false.
+__host__ __device__ inline bool edge_sources(int i, int j, int k, int& a, int& b) {
+  const bool ej = j == 1 || j == 18, ek = k == 1 || k == 18;
+  const int in = i == 1 ? 2 : 17, jn = j == 1 ? 2 : 17, kn = k == 1 ? 2 : 17;
+  if (i != 1 && i != 18) {
+    if (!(ej && ek)) return false;
+    a = i * 400 + j * 20 + kn;
+    b = i * 400 + jn * 20 + k;
+  } else if (ej && !ek) {
+    a = i * 400 + jn * 20 + k;
+    b = in * 400 + j * 20 + k;
+  } else if (ek && !ej) {
+    a = i * 400 + j * 20 + kn;
+    b = in * 400 + j * 20 + k;
+  } else {
+    return false;
+  }
+  return true;
+}
+
+// A cell where two or three ghost layers meet (edge or corner of the padded
+// cube).
+__host__ __device__ inline bool edge_cell(int i, int j, int k) {
+  const int g = (i < 2 || i > 17) + (j < 2 || j > 17) + (k < 2 || k > 17);
+  return g >= 2;
+}
+
 // Physics of every cube of a pool that holds several independent cases:
 // phys[cube_case[c]] is the case constants of cube c; cube_case[c] < 0 marks
 // a cube that belongs to no case (kernels skip it).
@@ -137,11 +172,14 @@ void launch_halo_pairs(hipStream_t s, float* U, const int2* pairs, int npairs);
 // Edge passes of the listed cubes.
 void launch_edge_list(hipStream_t s, double* U, const int* list, int n);
 void launch_edge_list(hipStream_t s, float* U, const int* list, int n);
-// Edge passes of every cube, then the domain BCs of boundary cubes
-// (cube_mask[c]: the k_bc side bits of pool cube c, 0 for none).  Used after a
-// flux kernel that wrote the face halos itself.
-void launch_edge_bc(hipStream_t s, double* U, const uint8_t* cube_mask, int ncube, PhysTab pt);
-void launch_edge_bc(hipStream_t s, float* U, const uint8_t* cube_mask, int ncube, PhysTab pt);
+// Edge passes of the n listed cubes (entry c; entry -c: only the k-edge cells
+// the z-BC reads), then the domain BCs of those that are boundary cubes
+// (cube_mask[c]: the k_bc side bits of pool cube c, 0 for none).  Used after
+// a flux kernel that wrote the face halos itself.
+void launch_edge_bc_list(hipStream_t s, double* U, const uint8_t* cube_mask, const int* list,
+                         int n, PhysTab pt);
+void launch_edge_bc_list(hipStream_t s, float* U, const uint8_t* cube_mask, const int* list,
+                         int n, PhysTab pt);
 // Edge passes of every cube.
 void launch_edge(hipStream_t s, double* U, int ncube, const int* cube_case);
 void launch_edge(hipStream_t s, float* U, int ncube, const int* cube_case);

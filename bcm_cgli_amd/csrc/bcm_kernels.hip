@@ -512,14 +512,47 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
   int ex;
   const bool pow2 = frexp(dx, &ex) == 0.5;
 
+  // PUSH: the edge ghost cells the viscous stencil reads are not read from
+  // memory (no edge pass runs between the stage's face updates and this
+  // kernel) but formed here as the pre-flux exchange's edge passes would
+  // leave them, the mean of two face ghost cells.  In the ghost planes 1 and
+  // 18 these are the border cells other than the cube corners.  In the other
+  // planes they are the four corners, whose sources are cells of the same
+  // plane loaded by lanes 1 and 18 (for the corner of lane 0) and 16 and 35
+  // (for lane 17) of the first wave: the second pass over the plane runs
+  // backwards so that this holds in both passes.
   auto fill_prims = [&](int ip) {
     const int slot = ip % 3;
-    for (int n = t; n < PR_PLANE; n += WG) {
+    const bool gplane = ip == 1 || ip == 18;
+    for (int rd = 0; rd < 2; ++rd) {
+      if (rd == 1 && t >= PR_PLANE - WG) continue;
+      const int n = rd == 0 ? t : PR_PLANE - 1 - t;
       const int jj = n / 18 + 1, kk = n % 18 + 1;
       const size_t o = sidx(c, 0, ip, jj, kk);
-      const double r = U[o];
-      const double u = U[o + PLANE] / r, v = U[o + 2 * PLANE] / r, w = U[o + 3 * PLANE] / r;
-      const double P = (U[o + 4 * PLANE] - 0.5 * r * (u * u + v * v + w * w)) * KM1;
+      double q[5];
+      int ea, eb;
+      if (PUSH && gplane && edge_sources(ip, jj, kk, ea, eb)) {
+        const size_t cb = (size_t)c * CUBE_STRIDE;
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+          q[v] = 0.5 * (U[cb + (size_t)v * PLANE + ea] + U[cb + (size_t)v * PLANE + eb]);
+      } else {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) q[v] = U[o + (size_t)v * PLANE];
+      }
+      if (PUSH && !gplane) {
+        const int la = t == 0 ? 1 : (t == 17 ? 16 : t);
+        const int lb = t == 0 ? 18 : (t == 17 ? 35 : t);
+        const bool corner = t == 0 || t == 17;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          const double x = __shfl(q[v], la, 64), y = __shfl(q[v], lb, 64);
+          if (corner) q[v] = 0.5 * (x + y);
+        }
+      }
+      const double r = q[0];
+      const double u = q[1] / r, v = q[2] / r, w = q[3] / r;
+      const double P = (q[4] - 0.5 * r * (u * u + v * v + w * w)) * KM1;
       pr[0][slot * PR_PLANE + n] = u;
       pr[1][slot * PR_PLANE + n] = v;
       pr[2][slot * PR_PLANE + n] = w;
@@ -1395,16 +1428,32 @@ __global__ __launch_bounds__(WG) void k_edge_list(T* __restrict__ U, const int* 
 }
 
 // Stage start after a flux kernel that wrote its face halos into the
-// neighbours (flux_push): the edge passes of the post-update exchange
-// (Interface.cpp:5640-5732), then the domain BCs of the next stage, per cube.
+// neighbours (flux_push), for each listed cube: the edge passes of the
+// post-update exchange (Interface.cpp:5640-5732), then the domain BCs of the
+// next stage.  A negative entry -c asks only for the edge cells the BCs read:
+// the z-BC (BCM_Z_boundary_condition.cpp:47-88) copies the plane k = 2 (or
+// 17) whole, four of whose cells are k-edges, each the mean of two face
+// ghost cells taken before the X- and Y-BCs overwrite them.
 template <typename T>
 __global__ __launch_bounds__(WG) void k_edge_bc(T* __restrict__ U,
                                                 const uint8_t* __restrict__ cube_mask,
-                                                PhysTab pt) {
-  const int c = blockIdx.x + 1;
+                                                PhysTab pt, const int* __restrict__ list) {
+  const int entry = list[blockIdx.x];
+  const int c = entry > 0 ? entry : -entry;
   if (pt.cube_case[c] < 0) return;
-  edge_fill(U, c);
   const int m = cube_mask[c];
+  if (entry > 0) {
+    edge_fill(U, c);
+  } else if (threadIdx.x < 40) {
+    const int side = threadIdx.x / 20, cell = (threadIdx.x / 5) % 4, v = threadIdx.x % 5;
+    if (m & (side ? 32 : 16)) {
+      const int ii = cell & 1 ? 18 : 1, jj = cell & 2 ? 18 : 1, kk = side ? 17 : 2;
+      int a, b;
+      edge_sources(ii, jj, kk, a, b);
+      const size_t o = (size_t)c * CUBE_STRIDE + (size_t)v * PLANE;
+      U[o + ii * 400 + jj * 20 + kk] = 0.5 * (U[o + a] + U[o + b]);
+    }
+  }
   if (m) {
     __syncthreads();
     bc_cube(U, c, m, pt);
@@ -1539,14 +1588,17 @@ void launch_ghost(hipStream_t s, float* U, const GhostTables& g, PhysTab pt) {
 }
 
 template <typename T>
-void launch_edge_bc_t(hipStream_t s, T* U, const uint8_t* cube_mask, int ncube, PhysTab pt) {
-  if (ncube > 1) hipLaunchKernelGGL(k_edge_bc<T>, dim3(ncube - 1), dim3(WG), 0, s, U, cube_mask, pt);
+void launch_edge_bc_list_t(hipStream_t s, T* U, const uint8_t* cube_mask, const int* list, int n,
+                           PhysTab pt) {
+  if (n > 0) hipLaunchKernelGGL(k_edge_bc<T>, dim3(n), dim3(WG), 0, s, U, cube_mask, pt, list);
 }
-void launch_edge_bc(hipStream_t s, double* U, const uint8_t* cube_mask, int ncube, PhysTab pt) {
-  launch_edge_bc_t(s, U, cube_mask, ncube, pt);
+void launch_edge_bc_list(hipStream_t s, double* U, const uint8_t* cube_mask, const int* list,
+                         int n, PhysTab pt) {
+  launch_edge_bc_list_t(s, U, cube_mask, list, n, pt);
 }
-void launch_edge_bc(hipStream_t s, float* U, const uint8_t* cube_mask, int ncube, PhysTab pt) {
-  launch_edge_bc_t(s, U, cube_mask, ncube, pt);
+void launch_edge_bc_list(hipStream_t s, float* U, const uint8_t* cube_mask, const int* list,
+                         int n, PhysTab pt) {
+  launch_edge_bc_list_t(s, U, cube_mask, list, n, pt);
 }
 template <typename T>
 void launch_edge_t(hipStream_t s, T* U, int ncube, const int* cube_case) {

@@ -248,12 +248,16 @@ __device__ __forceinline__ int border20(int t) {
   return (2 + u / 4) * 20 + edge[u % 4];
 }
 // Border cells of the 18x18 primitive plane (j or k in {1,18}): 68 of them,
-// as (j,k) in padded-plane numbering.
+// as (j,k) in padded-plane numbering.  Rows j = 1 and 18 first, then the
+// columns k = 1 and k = 18; the last column starts with j = 2 and 17, so that
+// the four corners (lanes 0, 17, 18, 35) and the cells they average (lanes 1
+// and 36, 16 and 52, 19 and 51, 34 and 53) all lie in the first wave.
 __device__ __forceinline__ int border18(int t) {
   if (t < 18) return 1 * 20 + (1 + t);
   if (t < 36) return 18 * 20 + (1 + t - 18);
   if (t < 52) return (2 + t - 36) * 20 + 1;
-  return (2 + t - 52) * 20 + 18;
+  const int u = t - 52;
+  return (u == 0 ? 2 : (u == 1 ? 17 : u + 1)) * 20 + 18;
 }
 
 __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
@@ -330,7 +334,24 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
     dst[3 * PR + o] = T;
     if (t < 68) {
       const int cell = border18(t);
-      prims(bq, u, v, ww, T);
+      // Interior planes: the corners are edge ghost cells, formed from two
+      // face ghost cells of the plane as the pre-flux exchange's edge passes
+      // would leave them (no edge pass runs between the stage's face updates
+      // and this kernel).  Ghost planes: formed when loaded.
+      float q[5];
+#pragma unroll
+      for (int m = 0; m < 5; ++m) q[m] = bq[m];
+      if (p >= 2 && p <= 17) {
+        const int la = t == 0 ? 1 : (t == 17 ? 16 : (t == 18 ? 19 : (t == 35 ? 34 : t)));
+        const int lb = t == 0 ? 36 : (t == 17 ? 52 : (t == 18 ? 51 : (t == 35 ? 53 : t)));
+        const bool corner = t == 0 || t == 17 || t == 18 || t == 35;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+          const float x = __shfl(q[m], la, 64), y = __shfl(q[m], lb, 64);
+          if (corner) q[m] = 0.5f * (x + y);
+        }
+      }
+      prims(q, u, v, ww, T);
       const int ob = (cell / 20 - 1) * 18 + (cell % 20 - 1);
       dst[ob] = u;
       dst[PR + ob] = v;
@@ -340,12 +361,20 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
   };
   // Border values are loaded one iteration ahead of their use: bcp/bfl the
   // 144 border cells of the next plane to stage, bpr the 68 primitive-ring
-  // border cells of the plane after it.
+  // border cells of the plane after it.  In the ghost planes 1 and 18 the
+  // border cells other than the cube corners are edge ghost cells, formed
+  // here as the mean of two face ghost cells.
   auto load_b18 = [&](int p, float (&q)[5]) {
     if (t < 68 && p <= 19) {
       const int cell = border18(t);
+      int ea, eb;
+      if ((p == 1 || p == 18) && edge_sources(p, cell / 20, cell % 20, ea, eb)) {
 #pragma unroll
-      for (int v = 0; v < 5; ++v) q[v] = Uc[v * PLANE + p * 400 + cell];
+        for (int v = 0; v < 5; ++v) q[v] = 0.5f * (Uc[v * PLANE + ea] + Uc[v * PLANE + eb]);
+      } else {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) q[v] = Uc[v * PLANE + p * 400 + cell];
+      }
     }
   };
   auto load_b20 = [&](int p, float (&q)[5], int& f) {
