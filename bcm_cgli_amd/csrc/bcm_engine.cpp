@@ -58,7 +58,8 @@ void free_graph(Engine* e) {
 // Tables of the resident cases (rebuilt whenever a case comes or goes).
 void free_tables(Engine* e) {
   void* ptrs[] = {e->bc_cubes,     e->bc_masks, e->halo,     e->halo_pairs,
-                  e->case_desc,    e->bc_mask_cube, e->ml, e->edge_list, e->open_list};
+                  e->case_desc,    e->bc_mask_cube, e->ml, e->edge_list, e->open_list,
+                  e->open_items};
   for (void* p : ptrs)
     if (p) hipFree(p);
   ghost_free(e->ghost);
@@ -71,8 +72,9 @@ void free_tables(Engine* e) {
   e->ml = nullptr;
   e->edge_list = nullptr;
   e->open_list = nullptr;
+  e->open_items = nullptr;
   e->n_halo_pairs = e->n_bc = e->n_ghost_cubes = e->n_ghost_entries = e->n_ml = 0;
-  e->n_edge_list = e->n_open_list = 0;
+  e->n_edge_list = e->n_open_list = e->n_open_items = 0;
 }
 
 void free_case(Engine* e) {
@@ -564,6 +566,26 @@ int rebuild_tables(Engine* e, std::string& err) {
       if (mark[c]) list.push_back(mark[c] == 2 ? c : -c);
     e->n_open_list = (int)list.size();
     if (dev_upload(&e->open_list, list, err)) return 1;
+    // One item per cube: IB cubes first (their fills are the longest chains),
+    // then the other listed cubes.
+    std::vector<int> gidx(top, -1);
+    const int ng = (int)gs.cube_level_off.size() - 1;
+    for (int g = 0; g < ng; ++g) {
+      if (gs.cube_level_off[g] == gs.cube_level_off[g + 1]) continue;
+      const int c = gs.entries[gs.level_off[gs.cube_level_off[g]]].cube;
+      if (gidx[c] >= 0) {
+        err = "a cube appears twice in the IB fill schedule";
+        return 1;
+      }
+      gidx[c] = g;
+    }
+    std::vector<int2> items;
+    for (int c = 1; c < top; ++c)
+      if (gidx[c] >= 0) items.push_back(make_int2(mark[c] ? (mark[c] == 2 ? c : -c) : 0, gidx[c]));
+    for (int c = 1; c < top; ++c)
+      if (gidx[c] < 0 && mark[c]) items.push_back(make_int2(mark[c] == 2 ? c : -c, -1));
+    e->n_open_items = (int)items.size();
+    if (dev_upload(&e->open_items, items, err)) return 1;
   }
   if (dev_upload(&e->bc_cubes, bc, err) || dev_upload(&e->bc_masks, bm, err) ||
       ghost_upload(gs, e->ghost, err) || dev_upload(&e->halo, hs, err) ||
@@ -835,8 +857,7 @@ namespace {
 template <typename T>
 void push_stage_open(Engine* e, T* U, bool full_pre_halo) {
   launch_halo_ml(e->stream, U, e->ml, e->n_ml, e->pt());
-  launch_edge_bc_list(e->stream, U, e->bc_mask_cube, e->open_list, e->n_open_list, e->pt());
-  engine_ghost(e);
+  launch_open(e->stream, U, e->open_items, e->n_open_items, e->bc_mask_cube, e->ghost.t, e->pt());
   if (full_pre_halo)
     engine_interface(e);
   else

@@ -126,6 +126,10 @@ struct Engine {
   // kernels form the edge cells they read themselves.
   int* open_list = nullptr;
   int n_open_list = 0;
+  // The same entries merged with the IB fill per cube for one opening launch:
+  // (open entry or 0, ghost-table index or -1), IB cubes first.
+  int2* open_items = nullptr;
+  int n_open_items = 0;
   // Set when an inner iterate was replaced from outside (state upload, new
   // case): the next pre-flux exchange must then be the full one.
   bool halo_full_next = false;

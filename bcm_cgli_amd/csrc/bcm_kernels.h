@@ -176,6 +176,13 @@ void launch_edge_list(hipStream_t s, float* U, const int* list, int n);
 // the z-BC reads), then the domain BCs of those that are boundary cubes
 // (cube_mask[c]: the k_bc side bits of pool cube c, 0 for none).  Used after
 // a flux kernel that wrote the face halos itself.
+// Opening of a pushed stage in one launch: per item (x: edge/BC entry as
+// above or 0, y: IB ghost-table index or -1) a cube's edge passes and BCs,
+// then its IB fill.
+void launch_open(hipStream_t s, double* U, const int2* items, int n, const uint8_t* cube_mask,
+                 const GhostTables& g, PhysTab pt);
+void launch_open(hipStream_t s, float* U, const int2* items, int n, const uint8_t* cube_mask,
+                 const GhostTables& g, PhysTab pt);
 void launch_edge_bc_list(hipStream_t s, double* U, const uint8_t* cube_mask, const int* list,
                          int n, PhysTab pt);
 void launch_edge_bc_list(hipStream_t s, float* U, const uint8_t* cube_mask, const int* list,

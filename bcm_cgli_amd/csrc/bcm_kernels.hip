@@ -1143,18 +1143,19 @@ __global__ __launch_bounds__(WG) void k_bc(T* __restrict__ U, const int* __restr
 // to LDS is rounded as storage would round it, so both ways give the same
 // results.
 
+// IB fill of ghost-table cube g by the calling workgroup; gsm holds
+// [5][lds_cap] doubles.
 template <typename T>
-__global__ __launch_bounds__(WG) void k_ghost(T* __restrict__ U, GhostTables gt, int lds_cap,
-                                              PhysTab pt) {
-  extern __shared__ double gsm[];   // [5][lds_cap]
-  const int l0 = gt.cube_level_off[blockIdx.x], l1 = gt.cube_level_off[blockIdx.x + 1];
+__device__ __forceinline__ void ghost_cube(T* __restrict__ U, const GhostTables& gt, int g,
+                                           int lds_cap, PhysTab pt, double* gsm) {
+  const int l0 = gt.cube_level_off[g], l1 = gt.cube_level_off[g + 1];
   if (l0 == l1) return;
   const int c = gt.ent[gt.level_off[l0]].cube;
   const DevPhys ph = pt.phys[pt.cube_case[c]];
   const typename OffOf<T>::type F(ph);
   const double rho0 = ph.rho0, P0 = ph.P0;
-  const int t0 = gt.touched_off[blockIdx.x];
-  const int nt = gt.touched_off[blockIdx.x + 1] - t0;
+  const int t0 = gt.touched_off[g];
+  const int nt = gt.touched_off[g + 1] - t0;
   const int ne = gt.level_off[l1] - gt.level_off[l0];
   const bool lds = nt + ne <= lds_cap;
   if (lds) {
@@ -1264,15 +1265,15 @@ __global__ __launch_bounds__(WG) void k_ghost(T* __restrict__ U, GhostTables gt,
 #pragma unroll
         for (int v = 0; v < 5; ++v) gsm[v * lds_cap + E.lg] = F.rt(out[v], v);
       } else {
-        const size_t g = sidx(c, 0, E.gi, E.gj, E.gk);
+        const size_t go = sidx(c, 0, E.gi, E.gj, E.gk);
 #pragma unroll
-        for (int v = 0; v < 5; ++v) F.st(U, g + (size_t)v * PLANE, v, out[v]);
+        for (int v = 0; v < 5; ++v) F.st(U, go + (size_t)v * PLANE, v, out[v]);
       }
     }
     __syncthreads();
   }
   if (lds) {
-    const int nw = gt.touched_nw[blockIdx.x];
+    const int nw = gt.touched_nw[g];
     for (int n = threadIdx.x; n < nw; n += WG) {
       const size_t o = (size_t)c * CUBE_STRIDE + gt.touched[t0 + n];
       const int sl = gt.touched_final[t0 + n];
@@ -1280,6 +1281,13 @@ __global__ __launch_bounds__(WG) void k_ghost(T* __restrict__ U, GhostTables gt,
       for (int v = 0; v < 5; ++v) F.st(U, o + (size_t)v * PLANE, v, gsm[v * lds_cap + sl]);
     }
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(WG) void k_ghost(T* __restrict__ U, GhostTables gt, int lds_cap,
+                                              PhysTab pt) {
+  extern __shared__ double gsm[];   // [5][lds_cap]
+  ghost_cube(U, gt, blockIdx.x, lds_cap, pt, gsm);
 }
 
 // Edge-ghost averaging of one cube in the reference's three sequential passes
@@ -1428,17 +1436,15 @@ __global__ __launch_bounds__(WG) void k_edge_list(T* __restrict__ U, const int* 
 }
 
 // Stage start after a flux kernel that wrote its face halos into the
-// neighbours (flux_push), for each listed cube: the edge passes of the
+// neighbours (flux_push), for one cube (entry c): the edge passes of the
 // post-update exchange (Interface.cpp:5640-5732), then the domain BCs of the
-// next stage.  A negative entry -c asks only for the edge cells the BCs read:
-// the z-BC (BCM_Z_boundary_condition.cpp:47-88) copies the plane k = 2 (or
-// 17) whole, four of whose cells are k-edges, each the mean of two face
-// ghost cells taken before the X- and Y-BCs overwrite them.
+// next stage.  Entry -c asks only for the edge cells the BCs read: the z-BC
+// (BCM_Z_boundary_condition.cpp:47-88) copies the plane k = 2 (or 17) whole,
+// four of whose cells are k-edges, each the mean of two face ghost cells
+// taken before the X- and Y-BCs overwrite them.
 template <typename T>
-__global__ __launch_bounds__(WG) void k_edge_bc(T* __restrict__ U,
-                                                const uint8_t* __restrict__ cube_mask,
-                                                PhysTab pt, const int* __restrict__ list) {
-  const int entry = list[blockIdx.x];
+__device__ __forceinline__ void edge_bc_cube(T* __restrict__ U, int entry,
+                                             const uint8_t* __restrict__ cube_mask, PhysTab pt) {
   const int c = entry > 0 ? entry : -entry;
   if (pt.cube_case[c] < 0) return;
   const int m = cube_mask[c];
@@ -1457,6 +1463,30 @@ __global__ __launch_bounds__(WG) void k_edge_bc(T* __restrict__ U,
   if (m) {
     __syncthreads();
     bc_cube(U, c, m, pt);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(WG) void k_edge_bc(T* __restrict__ U,
+                                                const uint8_t* __restrict__ cube_mask,
+                                                PhysTab pt, const int* __restrict__ list) {
+  edge_bc_cube(U, list[blockIdx.x], cube_mask, pt);
+}
+
+// Opening of a pushed stage in one launch: per item, a cube's edge passes and
+// BCs (x: entry as for edge_bc_cube, 0 for none), then its IB fill (y: ghost
+// table index, -1 for none).  Both touch only the cube's own cells, so cubes
+// proceed independently, each in the reference's order.
+template <typename T>
+__global__ __launch_bounds__(WG) void k_open(T* __restrict__ U, const int2* __restrict__ items,
+                                             const uint8_t* __restrict__ cube_mask,
+                                             GhostTables gt, int lds_cap, PhysTab pt) {
+  extern __shared__ double gsm[];   // [5][lds_cap]
+  const int2 it = items[blockIdx.x];
+  if (it.x != 0) edge_bc_cube(U, it.x, cube_mask, pt);
+  if (it.y >= 0) {
+    __syncthreads();
+    ghost_cube(U, gt, it.y, lds_cap, pt, gsm);
   }
 }
 
@@ -1585,6 +1615,23 @@ void launch_ghost(hipStream_t s, double* U, const GhostTables& g, PhysTab pt) {
 }
 void launch_ghost(hipStream_t s, float* U, const GhostTables& g, PhysTab pt) {
   launch_ghost_t(s, U, g, pt);
+}
+
+template <typename T>
+void launch_open_t(hipStream_t s, T* U, const int2* items, int n, const uint8_t* cube_mask,
+                   const GhostTables& g, PhysTab pt) {
+  if (n <= 0) return;
+  const int cap = g.max_slots < GHOST_LDS_SLOTS ? g.max_slots : GHOST_LDS_SLOTS;
+  const size_t shmem = (size_t)(cap > 0 ? cap : 1) * 5 * sizeof(double);
+  hipLaunchKernelGGL(k_open<T>, dim3(n), dim3(WG), shmem, s, U, items, cube_mask, g, cap, pt);
+}
+void launch_open(hipStream_t s, double* U, const int2* items, int n, const uint8_t* cube_mask,
+                 const GhostTables& g, PhysTab pt) {
+  launch_open_t(s, U, items, n, cube_mask, g, pt);
+}
+void launch_open(hipStream_t s, float* U, const int2* items, int n, const uint8_t* cube_mask,
+                 const GhostTables& g, PhysTab pt) {
+  launch_open_t(s, U, items, n, cube_mask, g, pt);
 }
 
 template <typename T>
