@@ -514,24 +514,23 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
 
   // PUSH: the edge ghost cells the viscous stencil reads are not read from
   // memory (no edge pass runs between the stage's face updates and this
-  // kernel) but formed here as the pre-flux exchange's edge passes would
-  // leave them, the mean of two face ghost cells.  In the ghost planes 1 and
-  // 18 these are the border cells other than the cube corners.  In the other
-  // planes they are the four corners, whose sources are cells of the same
-  // plane loaded by lanes 1 and 18 (for the corner of lane 0) and 16 and 35
-  // (for lane 17) of the first wave: the second pass over the plane runs
-  // backwards so that this holds in both passes.
+  // kernel) but formed as the pre-flux exchange's edge passes would leave
+  // them, the mean of two face ghost cells.  In the ghost planes 1 and 18
+  // they are the border cells other than the cube corners, formed here (every
+  // other cell is the mean of itself and itself, which is exact).  In the
+  // other planes they are the four corners, read only by the viscous stencil
+  // of their own plane: they are formed from the staged plane during its face
+  // pass (corner_prims below).
   auto fill_prims = [&](int ip) {
     const int slot = ip % 3;
     const bool gplane = ip == 1 || ip == 18;
-    for (int rd = 0; rd < 2; ++rd) {
-      if (rd == 1 && t >= PR_PLANE - WG) continue;
-      const int n = rd == 0 ? t : PR_PLANE - 1 - t;
+    for (int n = t; n < PR_PLANE; n += WG) {
       const int jj = n / 18 + 1, kk = n % 18 + 1;
       const size_t o = sidx(c, 0, ip, jj, kk);
       double q[5];
-      int ea, eb;
-      if (PUSH && gplane && edge_sources(ip, jj, kk, ea, eb)) {
+      if (PUSH) {
+        int ea = 0, eb = 0;
+        if (!(gplane && edge_sources(ip, jj, kk, ea, eb))) ea = eb = ip * 400 + jj * 20 + kk;
         const size_t cb = (size_t)c * CUBE_STRIDE;
 #pragma unroll
         for (int v = 0; v < 5; ++v)
@@ -539,16 +538,6 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
       } else {
 #pragma unroll
         for (int v = 0; v < 5; ++v) q[v] = U[o + (size_t)v * PLANE];
-      }
-      if (PUSH && !gplane) {
-        const int la = t == 0 ? 1 : (t == 17 ? 16 : t);
-        const int lb = t == 0 ? 18 : (t == 17 ? 35 : t);
-        const bool corner = t == 0 || t == 17;
-#pragma unroll
-        for (int v = 0; v < 5; ++v) {
-          const double x = __shfl(q[v], la, 64), y = __shfl(q[v], lb, 64);
-          if (corner) q[v] = 0.5 * (x + y);
-        }
       }
       const double r = q[0];
       const double u = q[1] / r, v = q[2] / r, w = q[3] / r;
@@ -558,6 +547,29 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
       pr[2][slot * PR_PLANE + n] = w;
       pr[3][slot * PR_PLANE + n] = P / (r * RG);
     }
+  };
+  // PUSH: the four corner cells of interior plane ip, formed from the staged
+  // plane (cpl) and written over the primitive ring's corners by lanes that
+  // have one face fewer to evaluate in the face pass.
+  auto corner_prims = [&](int ip) {
+    const int q4 = t - (WG - 4);
+    if (q4 < 0) return;
+    const int jj = q4 & 2 ? 18 : 1, kk = q4 & 1 ? 18 : 1;
+    int ea, eb;
+    edge_sources(ip, jj, kk, ea, eb);
+    ea %= 400;
+    eb %= 400;
+    double q[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) q[v] = 0.5 * (cpl[v * 400 + ea] + cpl[v * 400 + eb]);
+    const double r = q[0];
+    const double u = q[1] / r, v = q[2] / r, w = q[3] / r;
+    const double P = (q[4] - 0.5 * r * (u * u + v * v + w * w)) * KM1;
+    const int n = (jj - 1) * 18 + (kk - 1), slot = ip % 3;
+    pr[0][slot * PR_PLANE + n] = u;
+    pr[1][slot * PR_PLANE + n] = v;
+    pr[2][slot * PR_PLANE + n] = w;
+    pr[3][slot * PR_PLANE + n] = P / (r * RG);
   };
   fill_prims(i0 - 1);
   fill_prims(i0);
@@ -631,6 +643,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
     __syncthreads();
     // y-faces (f < 272: joins j = p and p+1 at k) and z-faces (joins k = p
     // and p+1 at j) of plane i in one pass.
+    if (PUSH) corner_prims(i);
     for (int f = t; f < 2 * NFACE; f += WG) {
       if (f < NFACE) {
         const int p = f / 16 + 1, kk = f % 16 + 2;
