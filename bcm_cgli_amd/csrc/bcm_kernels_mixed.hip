@@ -248,16 +248,29 @@ __device__ __forceinline__ int border20(int t) {
   return (2 + u / 4) * 20 + edge[u % 4];
 }
 // Border cells of the 18x18 primitive plane (j or k in {1,18}): 68 of them,
-// as (j,k) in padded-plane numbering.  Rows j = 1 and 18 first, then the
-// columns k = 1 and k = 18; the last column starts with j = 2 and 17, so that
-// the four corners (lanes 0, 17, 18, 35) and the cells they average (lanes 1
-// and 36, 16 and 52, 19 and 51, 34 and 53) all lie in the first wave.
+// as (j,k) in padded-plane numbering.
 __device__ __forceinline__ int border18(int t) {
   if (t < 18) return 1 * 20 + (1 + t);
   if (t < 36) return 18 * 20 + (1 + t - 18);
   if (t < 52) return (2 + t - 36) * 20 + 1;
-  const int u = t - 52;
-  return (u == 0 ? 2 : (u == 1 ? 17 : u + 1)) * 20 + 18;
+  return (2 + t - 52) * 20 + 18;
+}
+// The border lanes holding the two face ghost cells each corner of the plane
+// averages, in corner order (1,1), (1,18), (18,1), (18,18): (1,2) and (2,1),
+// (1,17) and (2,18), (18,2) and (17,1), (18,17) and (17,18).  Returns the
+// lane's slot 0..7, or -1.
+__device__ __forceinline__ int corner_source_slot(int t) {
+  switch (t) {
+    case 1: return 0;
+    case 36: return 1;
+    case 16: return 2;
+    case 52: return 3;
+    case 19: return 4;
+    case 51: return 5;
+    case 34: return 6;
+    case 67: return 7;
+    default: return -1;
+  }
 }
 
 __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
@@ -274,6 +287,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
   __shared__ float vy[4][NF];         // viscous fluxes of the y- and z-faces
   __shared__ float vz[4][NF];
   __shared__ double red[(WG / 64) * 7];
+  __shared__ float esrc[2][8][5];     // corner sources of interior planes (by parity)
 
   const int c = blockIdx.x + 1;
   const int cc = pt.cube_case[c];
@@ -297,7 +311,6 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
   const HaloSrc hs = halo[c];
   const int8_t* __restrict__ Fc = fws + (size_t)c * PLANE;
   const size_t gcol = (size_t)c * CUBE_STRIDE + col;
-
   // Own column: w[m] = plane i-3+m, fw[m] = flag of plane i-2+m.
   float w[7][5];
   int fw[5];
@@ -334,21 +347,24 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
     dst[3 * PR + o] = T;
     if (t < 68) {
       const int cell = border18(t);
-      // Interior planes: the corners are edge ghost cells, formed from two
-      // face ghost cells of the plane as the pre-flux exchange's edge passes
-      // would leave them (no edge pass runs between the stage's face updates
-      // and this kernel).  Ghost planes: formed when loaded.
       float q[5];
 #pragma unroll
       for (int m = 0; m < 5; ++m) q[m] = bq[m];
-      if (p >= 2 && p <= 17) {
-        const int la = t == 0 ? 1 : (t == 17 ? 16 : (t == 18 ? 19 : (t == 35 ? 34 : t)));
-        const int lb = t == 0 ? 36 : (t == 17 ? 52 : (t == 18 ? 51 : (t == 35 ? 53 : t)));
-        const bool corner = t == 0 || t == 17 || t == 18 || t == 35;
+      int ea, eb;
+      if (p == 1 || p == 18) {
+        // Ghost planes: the border cells other than the cube corners are
+        // edge ghost cells, the mean of two face ghost cells.
+        if (edge_sources(p, cell / 20, cell % 20, ea, eb)) {
 #pragma unroll
-        for (int m = 0; m < 5; ++m) {
-          const float x = __shfl(q[m], la, 64), y = __shfl(q[m], lb, 64);
-          if (corner) q[m] = 0.5f * (x + y);
+          for (int m = 0; m < 5; ++m) q[m] = 0.5f * (Uc[m * PLANE + ea] + Uc[m * PLANE + eb]);
+        }
+      } else {
+        // Interior planes: the sources of the plane's corners, edge ghost
+        // cells formed by corner_prims.
+        const int es = corner_source_slot(t);
+        if (es >= 0) {
+#pragma unroll
+          for (int m = 0; m < 5; ++m) esrc[p & 1][es][m] = bq[m];
         }
       }
       prims(q, u, v, ww, T);
@@ -359,22 +375,35 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
       dst[3 * PR + ob] = T;
     }
   };
+  // The four corners of interior plane p are edge ghost cells: the mean of
+  // two face ghost cells each, as the pre-flux exchange's edge passes would
+  // leave them (no edge pass runs between the stage's face updates and this
+  // kernel).  Only the plane's own face pass reads them; four lanes with one
+  // face fewer to evaluate form them from the sources fill_prims kept.
+  auto corner_prims = [&](int p) {
+    const int q4 = t - (WG - 4);
+    if (q4 < 0) return;
+    float q[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) q[m] = 0.5f * (esrc[p & 1][2 * q4][m] + esrc[p & 1][2 * q4 + 1][m]);
+    float u, v, ww, T;
+    prims(q, u, v, ww, T);
+    const int o = (q4 & 2 ? 17 * 18 : 0) + (q4 & 1 ? 17 : 0);
+    float* dst = &pr[p & 3][0][0];
+    dst[o] = u;
+    dst[PR + o] = v;
+    dst[2 * PR + o] = ww;
+    dst[3 * PR + o] = T;
+  };
   // Border values are loaded one iteration ahead of their use: bcp/bfl the
   // 144 border cells of the next plane to stage, bpr the 68 primitive-ring
-  // border cells of the plane after it.  In the ghost planes 1 and 18 the
-  // border cells other than the cube corners are edge ghost cells, formed
-  // here as the mean of two face ghost cells.
+  // border cells of the plane after it.
   auto load_b18 = [&](int p, float (&q)[5]) {
     if (t < 68 && p <= 19) {
       const int cell = border18(t);
-      int ea, eb;
-      if ((p == 1 || p == 18) && edge_sources(p, cell / 20, cell % 20, ea, eb)) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) q[v] = 0.5f * (Uc[v * PLANE + ea] + Uc[v * PLANE + eb]);
-      } else {
-#pragma unroll
-        for (int v = 0; v < 5; ++v) q[v] = Uc[v * PLANE + p * 400 + cell];
-      }
+      for (int v = 0; v < 5; ++v)
+        q[v] = Uc[v * PLANE + p * 400 + cell];
     }
   };
   auto load_b20 = [&](int p, float (&q)[5], int& f) {
@@ -391,6 +420,8 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
   fill_prims(1, w[2], bpr);
   load_b18(2, bpr);
   fill_prims(2, w[3], bpr);
+  __syncthreads();
+  corner_prims(2);
   load_b20(2, bcp, bfl);
   load_b18(3, bpr);
 
@@ -427,6 +458,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
     __syncthreads();
 
     // ---- y- and z-faces of plane i: 272 + 272 faces over 256 lanes ----
+    if (i + 1 <= 17) corner_prims(i + 1);
     for (int f = t; f < 2 * NF; f += WG) {
       const bool yface = f < NF;
       const int g = yface ? f : f - NF;
