@@ -510,6 +510,21 @@ int rebuild_tables(Engine* e, std::string& err) {
   gs.touched_off.push_back((int)gs.touched.size());
   std::vector<uint8_t> cube_mask(e->cap, 0);
   for (size_t a = 0; a < bc.size(); ++a) cube_mask[bc[a]] = bm[a];
+  {
+    // A z-BC reads the plane k = 2 (or 17) of its own cube.  Where nothing
+    // writes the cube between the stage's BCs and its flux (no IB fill, no
+    // IB-refresh copy, no two-level face), the pushing flux kernels apply the
+    // z-BC to the interior planes themselves.
+    std::vector<char> busy(top, 0);
+    for (const GhostEntry& E : gs.entries) busy[E.cube] = 1;
+    for (const int2& pr : pairs) busy[pr.y / PLANE] = 1;
+    for (const MlTaskDev& t : ml) busy[t.dst] = 1;
+    for (int c : bc) {
+      if (busy[c]) continue;
+      if (cube_mask[c] & 16) cube_mask[c] |= ZBC_IN_FLUX;
+      if (cube_mask[c] & 32) cube_mask[c] |= ZBC_IN_FLUX << 1;
+    }
+  }
   std::vector<HaloSrc> hs(top);
   for (auto& h : hs) memset(&h, 0, sizeof h);
   for (int s : order) {

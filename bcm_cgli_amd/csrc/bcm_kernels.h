@@ -126,6 +126,18 @@ struct MlTaskDev {
   int8_t axis, hi, restrict_, slot;
 };
 
+// Boundary-cube mask bit (shifted left by 1 for z-max) marking a z-BC face
+// whose BC the pushing flux kernels apply to the interior planes i = 2..17
+// themselves (BCM_Z_boundary_condition.cpp:47-88 reads only the plane's cell
+// k = 2, or 17, which nothing changes between a stage's BCs and its flux on
+// such cubes); the stage opening then does only the ghost planes.
+constexpr int ZBC_IN_FLUX = 64;
+
+// The z-min BC's w-momentum, -x, in storage form: FP64 stores the state, FP32
+// its deviation from the case's reference state (r3 = rho0 W0).
+__device__ inline double zbc_neg(double x, double) { return -x; }
+__device__ inline float zbc_neg(float x, double r3) { return (float)(-((double)x + r3) - r3); }
+
 // Copies, with the threads of one workgroup, the ghost regions the domain BCs
 // of cube c write (mask bits 1 x-min, 2 x-max, 4 y-min, 8 y-max, 16 z-min,
 // 32 z-max) from Uin to Uout: x faces over j,k in [2,17], y faces over all i
@@ -133,7 +145,7 @@ struct MlTaskDev {
 // neighbour writes them.
 template <typename T>
 __device__ inline void copy_bc_regions(const T* __restrict__ Uin, T* __restrict__ Uout, int c,
-                                       int m) {
+                                       int m, double r3) {
   const size_t base = (size_t)c * CUBE_STRIDE;
   auto cp = [&](int cell) {
 #pragma unroll
@@ -147,9 +159,24 @@ __device__ inline void copy_bc_regions(const T* __restrict__ Uin, T* __restrict_
     if (m & (4 << side))
       for (int n = threadIdx.x; n < 640; n += blockDim.x)
         cp(((n % 320) / 16) * 400 + (off + n / 320) * 20 + (n % 16 + 2));
-    if (m & (16 << side))
-      for (int n = threadIdx.x; n < 800; n += blockDim.x)
-        cp(((n % 400) / 20) * 400 + ((n % 400) % 20) * 20 + off + n / 400);
+    if (m & (16 << side)) {
+      // Planes whose z-BC this stage's flux kernel applied (ZBC_IN_FLUX):
+      // the BC values, from the plane's cell k = 2 (or 17).
+      const bool influx = m & (ZBC_IN_FLUX << side);
+      for (int n = threadIdx.x; n < 800; n += blockDim.x) {
+        const int i = (n % 400) / 20, cell = i * 400 + ((n % 400) % 20) * 20 + off + n / 400;
+        if (influx && i >= 2 && i <= 17) {
+          const int src = cell - (off + n / 400) + (side ? 17 : 2);
+#pragma unroll
+          for (int v = 0; v < 5; ++v) {
+            const T x = Uin[base + (size_t)v * PLANE + src];
+            Uout[base + (size_t)v * PLANE + cell] = (v == 3 && !side) ? zbc_neg(x, r3) : x;
+          }
+        } else {
+          cp(cell);
+        }
+      }
+    }
   }
 }
 

@@ -521,6 +521,21 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
   // other planes they are the four corners, read only by the viscous stencil
   // of their own plane: they are formed from the staged plane during its face
   // pass (corner_prims below).
+  // ZBC_IN_FLUX faces: the z-ghost cells (k = 0, 1 or 18, 19) of the
+  // interior planes are read as the z-BC sets them, from the cell k = 2 (w
+  // negated) or 17 of the same row.
+  const int zm = PUSH ? cube_mask[c] : 0;
+  const bool zlo = zm & ZBC_IN_FLUX, zhi = zm & (ZBC_IN_FLUX << 1);
+  auto zsrc = [&](int cell, bool& neg) {
+    const int kk = cell % 20;
+    neg = false;
+    if (zlo && kk < 2 && cell >= 800 && cell < 7200) {
+      neg = true;
+      return cell - kk + 2;
+    }
+    if (zhi && kk > 17 && cell >= 800 && cell < 7200) return cell - kk + 17;
+    return cell;
+  };
   auto fill_prims = [&](int ip) {
     const int slot = ip % 3;
     const bool gplane = ip == 1 || ip == 18;
@@ -531,10 +546,15 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
       if (PUSH) {
         int ea = 0, eb = 0;
         if (!(gplane && edge_sources(ip, jj, kk, ea, eb))) ea = eb = ip * 400 + jj * 20 + kk;
+        bool na, nb;
+        ea = zsrc(ea, na);
+        eb = zsrc(eb, nb);
         const size_t cb = (size_t)c * CUBE_STRIDE;
 #pragma unroll
-        for (int v = 0; v < 5; ++v)
-          q[v] = 0.5 * (U[cb + (size_t)v * PLANE + ea] + U[cb + (size_t)v * PLANE + eb]);
+        for (int v = 0; v < 5; ++v) {
+          const double x = U[cb + (size_t)v * PLANE + ea], y = U[cb + (size_t)v * PLANE + eb];
+          q[v] = 0.5 * ((v == 3 && na ? -x : x) + (v == 3 && nb ? -y : y));
+        }
       } else {
 #pragma unroll
         for (int v = 0; v < 5; ++v) q[v] = U[o + (size_t)v * PLANE];
@@ -635,9 +655,14 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
     fill_prims(i + 1);
     // Plane i (conserved values and flags) into LDS for the y- and z-faces.
     for (int n = t; n < 400; n += WG) {
-      const size_t o = sidx(c, 0, i, 0, 0) + n;
+      bool neg = false;
+      const int src = PUSH ? zsrc(i * 400 + n, neg) - i * 400 : n;
+      const size_t o = sidx(c, 0, i, 0, 0) + src;
 #pragma unroll
-      for (int v = 0; v < 5; ++v) cpl[v * 400 + n] = U[o + (size_t)v * PLANE];
+      for (int v = 0; v < 5; ++v) {
+        const double x = U[o + (size_t)v * PLANE];
+        cpl[v * 400 + n] = PUSH && v == 3 && neg ? -x : x;
+      }
       fpl[n] = fws[fidx(c, i, 0, 0) + n];
     }
     __syncthreads();
@@ -938,7 +963,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
   }
   if constexpr (PUSH) {
     const int bm = cube_mask[c];
-    if (bm) copy_bc_regions(U, Uout, c, bm);
+    if (bm) copy_bc_regions(U, Uout, c, bm, 0.0);
   }
   if (fuse || PUSH) {
     if (!PUSH)
@@ -1114,7 +1139,10 @@ __device__ __forceinline__ void bc_cube(T* __restrict__ U, int c, int m, PhysTab
   __syncthreads();
   for (int w = t; w < 400; w += WG) {
     const int i = w / 20, j = w % 20;
-    if (m & 16)
+    // Interior planes of a face marked ZBC_IN_FLUX get their z-BC from the
+    // flux kernel.
+    const bool inner = i >= 2 && i <= 17;
+    if ((m & 16) && !(inner && (m & ZBC_IN_FLUX)))
       for (int v = 0; v < 5; ++v) {
         if (v == 3) {
           const double y = -F.ld(U, sidx(c, v, i, j, 2), 3);
@@ -1126,7 +1154,7 @@ __device__ __forceinline__ void bc_cube(T* __restrict__ U, int c, int m, PhysTab
           U[sidx(c, v, i, j, 0)] = x;
         }
       }
-    if (m & 32)
+    if ((m & 32) && !(inner && (m & (ZBC_IN_FLUX << 1))))
       for (int v = 0; v < 5; ++v) {
         const double x = U[sidx(c, v, i, j, 17)];
         U[sidx(c, v, i, j, 18)] = x;

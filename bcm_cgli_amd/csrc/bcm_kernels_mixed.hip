@@ -310,6 +310,23 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
   float* __restrict__ Oc = Uout + (size_t)c * CUBE_STRIDE;
   const HaloSrc hs = halo[c];
   const int8_t* __restrict__ Fc = fws + (size_t)c * PLANE;
+  // ZBC_IN_FLUX faces: the z-ghost cells (k = 0, 1 or 18, 19) of the interior
+  // planes are read as the z-BC sets them, from the cell k = 2 (w negated) or
+  // 17 of the same row.  Prefetched border values take the redirected address
+  // at once and the negation when used (nb18, nb20: lanes on k = 0, 1).
+  const int zm = cube_mask[c];
+  const bool zlo = (zm & ZBC_IN_FLUX) != 0, zhi = (zm & (ZBC_IN_FLUX << 1)) != 0;
+  auto zsrc = [&](int cell, bool& neg) {
+    const int kk = cell % 20;
+    neg = false;
+    if (zlo && kk < 2 && cell >= 800 && cell < 7200) {
+      neg = true;
+      return cell - kk + 2;
+    }
+    if (zhi && kk > 17 && cell >= 800 && cell < 7200) return cell - kk + 17;
+    return cell;
+  };
+  const double r3 = uref[3];
   const size_t gcol = (size_t)c * CUBE_STRIDE + col;
   // Own column: w[m] = plane i-3+m, fw[m] = flag of plane i-2+m.
   float w[7][5];
@@ -355,16 +372,23 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
         // Ghost planes: the border cells other than the cube corners are
         // edge ghost cells, the mean of two face ghost cells.
         if (edge_sources(p, cell / 20, cell % 20, ea, eb)) {
+          bool na, nb;
+          ea = zsrc(ea, na);
+          eb = zsrc(eb, nb);
 #pragma unroll
-          for (int m = 0; m < 5; ++m) q[m] = 0.5f * (Uc[m * PLANE + ea] + Uc[m * PLANE + eb]);
+          for (int m = 0; m < 5; ++m) {
+            const float x = Uc[m * PLANE + ea], y = Uc[m * PLANE + eb];
+            q[m] = 0.5f * ((m == 3 && na ? zbc_neg(x, r3) : x) + (m == 3 && nb ? zbc_neg(y, r3) : y));
+          }
         }
       } else {
+        if (zlo && cell % 20 == 1) q[3] = zbc_neg(q[3], r3);
         // Interior planes: the sources of the plane's corners, edge ghost
         // cells formed by corner_prims.
         const int es = corner_source_slot(t);
         if (es >= 0) {
 #pragma unroll
-          for (int m = 0; m < 5; ++m) esrc[p & 1][es][m] = bq[m];
+          for (int m = 0; m < 5; ++m) esrc[p & 1][es][m] = q[m];
         }
       }
       prims(q, u, v, ww, T);
@@ -401,16 +425,19 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
   auto load_b18 = [&](int p, float (&q)[5]) {
     if (t < 68 && p <= 19) {
       const int cell = border18(t);
+      bool neg;
+      const int src = zsrc(p * 400 + cell, neg);
 #pragma unroll
-      for (int v = 0; v < 5; ++v)
-        q[v] = Uc[v * PLANE + p * 400 + cell];
+      for (int v = 0; v < 5; ++v) q[v] = Uc[v * PLANE + src];
     }
   };
   auto load_b20 = [&](int p, float (&q)[5], int& f) {
     if (t < 144 && p <= 17) {
       const int cell = border20(t);
+      bool neg;
+      const int src = zsrc(p * 400 + cell, neg);
 #pragma unroll
-      for (int v = 0; v < 5; ++v) q[v] = Uc[v * PLANE + p * 400 + cell];
+      for (int v = 0; v < 5; ++v) q[v] = Uc[v * PLANE + src];
       f = Fc[p * 400 + cell];
     }
   };
@@ -440,7 +467,8 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
     if (t < 144) {
       const int cell = border20(t);
 #pragma unroll
-      for (int v = 0; v < 5; ++v) cp[b][v][cell] = bcp[v];
+      for (int v = 0; v < 5; ++v)
+        cp[b][v][cell] = v == 3 && zlo && cell % 20 < 2 ? zbc_neg(bcp[v], r3) : bcp[v];
       fl[b][cell] = (int8_t)bfl;
     }
     fill_prims(i + 1, w[4], bpr);
@@ -695,7 +723,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
   // neighbour writes them), so Uout holds them as an in-place update would.
   {
     const int bm = cube_mask[c];
-    if (bm) copy_bc_regions(U, Uout, c, bm);
+    if (bm) copy_bc_regions(U, Uout, c, bm, uref[3]);
   }
   if (RK == 3) {
 #pragma unroll
