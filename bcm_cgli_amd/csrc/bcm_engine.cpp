@@ -58,8 +58,7 @@ void free_graph(Engine* e) {
 // Tables of the resident cases (rebuilt whenever a case comes or goes).
 void free_tables(Engine* e) {
   void* ptrs[] = {e->bc_cubes,     e->bc_masks, e->halo,     e->halo_pairs,
-                  e->case_desc,    e->bc_mask_cube, e->ml, e->edge_list, e->open_list,
-                  e->open_items};
+                  e->case_desc,    e->bc_mask_cube, e->ml, e->edge_list, e->open_items};
   for (void* p : ptrs)
     if (p) hipFree(p);
   ghost_free(e->ghost);
@@ -71,10 +70,9 @@ void free_tables(Engine* e) {
   e->case_desc = nullptr;
   e->ml = nullptr;
   e->edge_list = nullptr;
-  e->open_list = nullptr;
   e->open_items = nullptr;
   e->n_halo_pairs = e->n_bc = e->n_ghost_cubes = e->n_ghost_entries = e->n_ml = 0;
-  e->n_edge_list = e->n_open_list = e->n_open_items = 0;
+  e->n_edge_list = e->n_open_items = 0;
 }
 
 void free_case(Engine* e) {
@@ -576,11 +574,6 @@ int rebuild_tables(Engine* e, std::string& err) {
         return 1;
       }
     }
-    std::vector<int> list;
-    for (int c = 1; c < top; ++c)
-      if (mark[c]) list.push_back(mark[c] == 2 ? c : -c);
-    e->n_open_list = (int)list.size();
-    if (dev_upload(&e->open_list, list, err)) return 1;
     // One item per cube: IB cubes first (their fills are the longest chains),
     // then the other listed cubes.
     std::vector<int> gidx(top, -1);

@@ -118,16 +118,12 @@ struct Engine {
   // the pre-flux edge pass covers only these.
   int* edge_list = nullptr;
   int n_edge_list = 0;
-  // Cubes whose edge ghost cells must be in memory when a stage opens after a
-  // flux kernel that pushed its face halos: cubes whose IB fill touches an
-  // edge cell (entry c: all edge passes, then BCs) and other boundary cubes
-  // (entry -c: the k-edge cells the z-BC copies, then BCs).  Such stages run
-  // the opening edge passes and BCs on these cubes only; the pushing flux
-  // kernels form the edge cells they read themselves.
-  int* open_list = nullptr;
-  int n_open_list = 0;
-  // The same entries merged with the IB fill per cube for one opening launch:
-  // (open entry or 0, ghost-table index or -1), IB cubes first.
+  // Opening of a stage after a flux kernel that pushed its face halos, one
+  // item per cube that needs one: (edge entry, ghost-table index or -1), IB
+  // cubes first.  Edge entry c: all edge passes, then BCs (the cube's IB fill
+  // touches an edge cell); -c: the k-edge cells the z-BC copies, then BCs
+  // (other boundary cubes); 0: none.  Elsewhere only the pushing flux kernels
+  // read edge cells, and they form them themselves.
   int2* open_items = nullptr;
   int n_open_items = 0;
   // Set when an inner iterate was replaced from outside (state upload, new

@@ -210,10 +210,6 @@ void launch_open(hipStream_t s, double* U, const int2* items, int n, const uint8
                  const GhostTables& g, PhysTab pt);
 void launch_open(hipStream_t s, float* U, const int2* items, int n, const uint8_t* cube_mask,
                  const GhostTables& g, PhysTab pt);
-void launch_edge_bc_list(hipStream_t s, double* U, const uint8_t* cube_mask, const int* list,
-                         int n, PhysTab pt);
-void launch_edge_bc_list(hipStream_t s, float* U, const uint8_t* cube_mask, const int* list,
-                         int n, PhysTab pt);
 // Edge passes of every cube.
 void launch_edge(hipStream_t s, double* U, int ncube, const int* cube_case);
 void launch_edge(hipStream_t s, float* U, int ncube, const int* cube_case);

@@ -1507,13 +1507,6 @@ __device__ __forceinline__ void edge_bc_cube(T* __restrict__ U, int entry,
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(WG) void k_edge_bc(T* __restrict__ U,
-                                                const uint8_t* __restrict__ cube_mask,
-                                                PhysTab pt, const int* __restrict__ list) {
-  edge_bc_cube(U, list[blockIdx.x], cube_mask, pt);
-}
-
 // Opening of a pushed stage in one launch: per item, a cube's edge passes and
 // BCs (x: entry as for edge_bc_cube, 0 for none), then its IB fill (y: ghost
 // table index, -1 for none).  Both touch only the cube's own cells, so cubes
@@ -1675,19 +1668,6 @@ void launch_open(hipStream_t s, float* U, const int2* items, int n, const uint8_
   launch_open_t(s, U, items, n, cube_mask, g, pt);
 }
 
-template <typename T>
-void launch_edge_bc_list_t(hipStream_t s, T* U, const uint8_t* cube_mask, const int* list, int n,
-                           PhysTab pt) {
-  if (n > 0) hipLaunchKernelGGL(k_edge_bc<T>, dim3(n), dim3(WG), 0, s, U, cube_mask, pt, list);
-}
-void launch_edge_bc_list(hipStream_t s, double* U, const uint8_t* cube_mask, const int* list,
-                         int n, PhysTab pt) {
-  launch_edge_bc_list_t(s, U, cube_mask, list, n, pt);
-}
-void launch_edge_bc_list(hipStream_t s, float* U, const uint8_t* cube_mask, const int* list,
-                         int n, PhysTab pt) {
-  launch_edge_bc_list_t(s, U, cube_mask, list, n, pt);
-}
 template <typename T>
 void launch_edge_t(hipStream_t s, T* U, int ncube, const int* cube_case) {
   if (ncube > 1) hipLaunchKernelGGL(k_edge<T>, dim3(ncube - 1), dim3(WG), 0, s, U, cube_case);
