@@ -527,8 +527,9 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
   const int zm = PUSH ? cube_mask[c] : 0;
   const bool zlo = zm & ZBC_IN_FLUX, zhi = zm & (ZBC_IN_FLUX << 1);
   auto zsrc = [&](int cell, bool& neg) {
-    const int kk = cell % 20;
     neg = false;
+    if (!(zlo || zhi)) return cell;
+    const int kk = cell % 20;
     if (zlo && kk < 2 && cell >= 800 && cell < 7200) {
       neg = true;
       return cell - kk + 2;
@@ -543,9 +544,9 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
       const int jj = n / 18 + 1, kk = n % 18 + 1;
       const size_t o = sidx(c, 0, ip, jj, kk);
       double q[5];
-      if (PUSH) {
+      if (PUSH && gplane) {
         int ea = 0, eb = 0;
-        if (!(gplane && edge_sources(ip, jj, kk, ea, eb))) ea = eb = ip * 400 + jj * 20 + kk;
+        if (!edge_sources(ip, jj, kk, ea, eb)) ea = eb = ip * 400 + jj * 20 + kk;
         bool na, nb;
         ea = zsrc(ea, na);
         eb = zsrc(eb, nb);
@@ -554,6 +555,15 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
         for (int v = 0; v < 5; ++v) {
           const double x = U[cb + (size_t)v * PLANE + ea], y = U[cb + (size_t)v * PLANE + eb];
           q[v] = 0.5 * ((v == 3 && na ? -x : x) + (v == 3 && nb ? -y : y));
+        }
+      } else if (PUSH) {
+        bool na;
+        const int ea = zsrc(ip * 400 + jj * 20 + kk, na);
+        const size_t cb = (size_t)c * CUBE_STRIDE;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          const double x = U[cb + (size_t)v * PLANE + ea];
+          q[v] = v == 3 && na ? -x : x;
         }
       } else {
 #pragma unroll

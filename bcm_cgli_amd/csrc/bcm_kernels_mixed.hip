@@ -317,8 +317,9 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
   const int zm = cube_mask[c];
   const bool zlo = (zm & ZBC_IN_FLUX) != 0, zhi = (zm & (ZBC_IN_FLUX << 1)) != 0;
   auto zsrc = [&](int cell, bool& neg) {
-    const int kk = cell % 20;
     neg = false;
+    if (!(zlo || zhi)) return cell;
+    const int kk = cell % 20;
     if (zlo && kk < 2 && cell >= 800 && cell < 7200) {
       neg = true;
       return cell - kk + 2;
