@@ -313,7 +313,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
   // ZBC_IN_FLUX faces: the z-ghost cells (k = 0, 1 or 18, 19) of the interior
   // planes are read as the z-BC sets them, from the cell k = 2 (w negated) or
   // 17 of the same row.  Prefetched border values take the redirected address
-  // at once and the negation when used (nb18, nb20: lanes on k = 0, 1).
+  // at once and the negation when they are used.
   const int zm = cube_mask[c];
   const bool zlo = (zm & ZBC_IN_FLUX) != 0, zhi = (zm & (ZBC_IN_FLUX << 1)) != 0;
   auto zsrc = [&](int cell, bool& neg) {
