@@ -494,7 +494,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
   __shared__ double pr[4][3 * PR_PLANE];   // u, v, w, T = P/(rho R) of 3 planes
   __shared__ double fy[5][NFACE];
   __shared__ double fz[6][NFACE];          // flux + face pressure average
-  __shared__ double cpl[5 * 400];          // conserved values of plane i
+  __shared__ __align__(16) double cpl[5 * 400];   // conserved values of plane i
   __shared__ int8_t fpl[400];              // flags of plane i
   const int seg = blockIdx.x % nseg;
   const int c = blockIdx.x / nseg + 1;
@@ -668,17 +668,32 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
     // update safe now.
     if (!PUSH && fuse && i - 4 >= i0) update_plane(i - 4);
     fill_prims(i + 1);
-    // Plane i (conserved values and flags) into LDS for the y- and z-faces.
-    for (int n = t; n < 400; n += WG) {
-      bool neg = false;
-      const int src = PUSH ? zsrc(i * 400 + n, neg) - i * 400 : n;
-      const size_t o = sidx(c, 0, i, 0, 0) + src;
+    // Plane i (conserved values and flags) into LDS for the y- and z-faces:
+    // two k-neighbours per lane in one 16-byte access, or cell by cell where
+    // the z-ghost cells are redirected.
+    if (!(zlo || zhi)) {
+      if (t < 200) {
+        const size_t o = sidx(c, 0, i, 0, 0) + 2 * t;
 #pragma unroll
-      for (int v = 0; v < 5; ++v) {
-        const double x = U[o + (size_t)v * PLANE];
-        cpl[v * 400 + n] = PUSH && v == 3 && neg ? -x : x;
+        for (int v = 0; v < 5; ++v)
+          *reinterpret_cast<double2*>(&cpl[v * 400 + 2 * t]) =
+              *reinterpret_cast<const double2*>(&U[o + (size_t)v * PLANE]);
+        const int8_t* fr = fws + fidx(c, i, 0, 0) + 2 * t;
+        fpl[2 * t] = fr[0];
+        fpl[2 * t + 1] = fr[1];
       }
-      fpl[n] = fws[fidx(c, i, 0, 0) + n];
+    } else {
+      for (int n = t; n < 400; n += WG) {
+        bool neg = false;
+        const int src = zsrc(i * 400 + n, neg) - i * 400;
+        const size_t o = sidx(c, 0, i, 0, 0) + src;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          const double x = U[o + (size_t)v * PLANE];
+          cpl[v * 400 + n] = v == 3 && neg ? -x : x;
+        }
+        fpl[n] = fws[fidx(c, i, 0, 0) + n];
+      }
     }
     __syncthreads();
     // y-faces (f < 272: joins j = p and p+1 at k) and z-faces (joins k = p
