@@ -537,15 +537,15 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
     if (zhi && kk > 17 && cell >= 800 && cell < 7200) return cell - kk + 17;
     return cell;
   };
-  // The 68 cells beyond one per lane go to lanes 144..211, which stage one
-  // cell of the plane fewer than the others.
+  // The 68 cells beyond one per lane go to lanes 188..255, most of which
+  // stage no part of the plane.
   auto fill_prims = [&](int ip) {
     const int slot = ip % 3;
     const bool gplane = ip == 1 || ip == 18;
 #pragma unroll 1
     for (int rd = 0; rd < 2; ++rd) {
-      const int n = rd == 0 ? t : WG + t - 144;
-      if (rd == 1 && (t < 144 || n >= PR_PLANE)) continue;
+      const int n = rd == 0 ? t : WG + t - 188;
+      if (rd == 1 && t < 188) continue;
       const int jj = n / 18 + 1, kk = n % 18 + 1;
       const size_t o = sidx(c, 0, ip, jj, kk);
       double q[5];
