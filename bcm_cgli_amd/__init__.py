@@ -162,7 +162,9 @@ class Case:
         """A case from the reference's input files BCMgrid00000.dat and
         BCM_STL_fine.stl in ``directory``.  ``params`` needs ``mach``,
         ``reynolds`` and ``sphere_diameter`` (the reference length of the
-        viscosity) and may give ``seed`` and ``noise_amp``."""
+        viscosity) and may give ``seed`` and ``noise_amp``.  Rank-decomposition
+        files in ``directory`` must describe one rank; a decomposition over
+        several ranks raises BCMError."""
         return cls(params, device=device, _from_dir=directory)
 
     def array(self, name):
@@ -187,7 +189,9 @@ class Case:
         return a
 
     def write_inputs(self, directory):
-        """BCMgrid00000.dat and BCM_STL_fine.stl, the reference's inputs."""
+        """BCMgrid00000.dat and BCM_STL_fine.stl, the reference's inputs, and
+        its rank-decomposition files MPI_communication_table_in_turn.dat,
+        Communication_irank.00000.dat and rank_map.dat for one rank."""
         _check(_lib.bcm_case_write_inputs(self._h, os.fsencode(directory)))
 
     def write_ibm(self, directory):

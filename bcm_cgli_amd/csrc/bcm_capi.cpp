@@ -319,6 +319,7 @@ int bcm_case_write_inputs(const bcm_case* c, const char* dir) {
   std::string d(dir);
   if (write_bcmgrid(c->cs, d + "/BCMgrid00000.dat")) return fail("cannot write grid file");
   if (write_stl(c->cs, d + "/BCM_STL_fine.stl")) return fail("cannot write STL file");
+  if (write_rank_tables(c->cs.ncube - 1, d)) return fail("cannot write the rank tables");
   return 0;
 }
 

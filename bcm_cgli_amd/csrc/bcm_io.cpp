@@ -1,6 +1,6 @@
 // On-disk formats of a BCM case: readers for the grid file, the ASCII STL
 // surface and the IB tables; the IB-table and PLOT3D writers in the layouts
-// the reference writes; a case built from files; a restart snapshot of the
+// the reference writes; the rank-decomposition files of a one-rank run; a case built from files; a restart snapshot of the
 // two time levels.
 #include <math.h>
 #include <stdio.h>
@@ -248,10 +248,136 @@ int read_ibm(const std::string& path, IBTable& tb, std::string& err) {
   return 0;
 }
 
+// The three rank-decomposition files the reference reads at start-up, written
+// for a decomposition into one rank that holds all mpi_ncube cubes:
+//  - MPI_communication_table_in_turn.dat: free text up to the line
+//    "Number of Cubes for connection  >>", then the count of cross-rank cube
+//    connections (main.cpp:215-221 reads only that count): 0.
+//  - Communication_irank.00000.dat: per transfer kind (equal size, small to
+//    big, big to small) three title lines, the neighbour-rank count and then
+//    one title line before each per-neighbour or per-cube list
+//    (main.cpp:262-554): all counts 0, so every list is empty.
+//  - rank_map.dat: a title line, then per global cube "global rank local"
+//    (main.cpp:584-596); local numbers run from 1, and the last one of a rank
+//    is its cube count (BCM_Output.cpp:68-77).
+int write_rank_tables(int mpi_ncube, const std::string& dir) {
+  {
+    File F(dir + "/MPI_communication_table_in_turn.dat", "w");
+    if (!F.f) return 1;
+    fprintf(F.f, "Cube connections between ranks (one rank: none)\n");
+    fprintf(F.f, "Number of Cubes for connection  >>\n0\n");
+  }
+  {
+    File F(dir + "/Communication_irank.00000.dat", "w");
+    if (!F.f) return 1;
+    static const char* kinds[3] = {"equal size", "small to big", "big to small"};
+    static const int lists[3] = {6, 7, 8};
+    static const char* names[8] = {"neighbour ranks", "cubes per neighbour rank",
+                                   "cubes sent",      "cubes received",
+                                   "send directions", "receive directions",
+                                   "adjacency slot",  "send adjacency slot"};
+    for (int s = 0; s < 3; ++s) {
+      fprintf(F.f, "# ------------------------------\n# %s neighbours of rank 0\n", kinds[s]);
+      fprintf(F.f, "# ------------------------------\n0\n");
+      for (int l = 0; l < lists[s]; ++l) fprintf(F.f, "# %s\n", names[l]);
+    }
+  }
+  File F(dir + "/rank_map.dat", "w");
+  if (!F.f) return 1;
+  fprintf(F.f, "Cube(global)\tRank\tCube(local)\n");
+  for (int g = 0; g < mpi_ncube; ++g) fprintf(F.f, "%d\t%d\t%d\n", g, 0, g + 1);
+  return 0;
+}
+
+// Checks the rank-decomposition files present in dir against a single-rank
+// run of a case of mpi_ncube cubes: no cross-rank connections, no neighbour
+// ranks, and a rank map of exactly mpi_ncube cubes all on rank 0 with local
+// numbers in 1..mpi_ncube.  Absent files are accepted.
+int check_rank_tables(const std::string& dir, int mpi_ncube, std::string& err) {
+  const std::string multi =
+      " describes a decomposition over several ranks; one GPU runs a case decomposed into one "
+      "rank (BCMgrid00000.dat holding every cube)";
+  std::string path = dir + "/MPI_communication_table_in_turn.dat";
+  if (File F(path, "r"); F.f) {
+    char line[1024];
+    bool found = false;
+    while (fscanf(F.f, "%1023[^\n]\n", line) == 1) {
+      size_t len = strlen(line);
+      while (len && (line[len - 1] == ' ' || line[len - 1] == '\r')) line[--len] = 0;
+      if (strcmp(line, "Number of Cubes for connection  >>") == 0) {
+        found = true;
+        break;
+      }
+    }
+    int nadj = -1;
+    if (!found || fscanf(F.f, "%d", &nadj) != 1 || nadj < 0) {
+      err = "connection count unreadable in " + path;
+      return 1;
+    }
+    if (nadj != 0) {
+      err = path + multi;
+      return 1;
+    }
+  }
+  path = dir + "/Communication_irank.00000.dat";
+  if (File F(path, "r"); F.f) {
+    static const int lists[3] = {6, 7, 8};
+    for (int s = 0; s < 3; ++s) {
+      int ncpu = -1;
+      bool ok = skip_line(F.f) && skip_line(F.f) && skip_line(F.f) && fscanf(F.f, "%d\n", &ncpu) == 1;
+      if (!ok || ncpu < 0) {
+        err = "neighbour-rank count unreadable in " + path;
+        return 1;
+      }
+      if (ncpu != 0) {
+        err = path + multi;
+        return 1;
+      }
+      for (int l = 0; l < lists[s]; ++l)
+        if (!skip_line(F.f)) {
+          err = "title line missing in " + path;
+          return 1;
+        }
+    }
+  }
+  path = dir + "/rank_map.dat";
+  if (File F(path, "r"); F.f) {
+    if (!skip_line(F.f)) {
+      err = "title line missing in " + path;
+      return 1;
+    }
+    int g, rank, local, rows = 0;
+    while (fscanf(F.f, "%d %d %d", &g, &rank, &local) == 3) {
+      if (rank != 0 || ++rows > mpi_ncube) {
+        err = path + multi;
+        return 1;
+      }
+      if (local < 1 || local > mpi_ncube) {
+        err = "local cube number outside 1.." + std::to_string(mpi_ncube) + " in " + path;
+        return 1;
+      }
+    }
+    int ch;
+    while ((ch = fgetc(F.f)) != EOF && (ch == ' ' || ch == '\t' || ch == '\r' || ch == '\n')) {
+    }
+    if (ch != EOF) {
+      err = "rank map row unreadable in " + path;
+      return 1;
+    }
+    if (rows != mpi_ncube) {
+      err = path + " maps " + std::to_string(rows) + " cubes; the grid file holds " +
+            std::to_string(mpi_ncube);
+      return 1;
+    }
+  }
+  return 0;
+}
+
 // A case from the reference's input files in dir: BCMgrid00000.dat and
-// BCM_STL_fine.stl.  p supplies what the reference keeps in prm.h and its
-// initial-condition noise: Mach, Reynolds number, reference length
-// (p.sphere_diameter), seed and noise amplitude; the remaining setup
+// BCM_STL_fine.stl, with the rank-decomposition files checked where present.
+// p supplies what the reference keeps in prm.h and its initial-condition
+// noise: Mach, Reynolds number, reference length (p.sphere_diameter), seed
+// and noise amplitude; the remaining setup
 // (boundary lists, interface tables, flags, IB tables, initial state) runs as
 // for a generated case.
 int read_case(const std::string& dir, const CaseParams& p, Case& cs, std::string& err,
@@ -264,6 +390,7 @@ int read_case(const std::string& dir, const CaseParams& p, Case& cs, std::string
   }
   compute_physics(p, cs.phys);
   if (read_bcmgrid(dir + "/BCMgrid00000.dat", cs, err)) return 1;
+  if (check_rank_tables(dir, cs.ncube - 1, err)) return 1;
   if (read_stl(dir + "/BCM_STL_fine.stl", cs.stl, err)) return 1;
   grid_geometry(cs);
   cs.fws_grid = cs.fws;

@@ -189,6 +189,12 @@ int read_stl(const std::string& path, std::vector<Triangle>& tris, std::string& 
 int read_ibm(const std::string& path, IBTable& tb, std::string& err);
 // IBM_minus00000.dat and IBM_plus00000.dat in dir.
 int write_ibm(const Case& cs, const std::string& dir);
+// Rank-decomposition files (MPI_communication_table_in_turn.dat,
+// Communication_irank.00000.dat, rank_map.dat) of a one-rank run of
+// mpi_ncube cubes; the check accepts absent files and refuses a decomposition
+// over several ranks.
+int write_rank_tables(int mpi_ncube, const std::string& dir);
+int check_rank_tables(const std::string& dir, int mpi_ncube, std::string& err);
 // A case from BCMgrid00000.dat and BCM_STL_fine.stl in dir; p supplies the
 // flow parameters (Mach, Reynolds number, reference length, noise).
 int read_case(const std::string& dir, const CaseParams& p, Case& cs, std::string& err,

@@ -168,7 +168,12 @@ int bcm_case_info(const bcm_case* c, int* info, double* phys);
  * "adjX_sb_plus", "adjX_bs_minus", "adjX_sb_minus", then the same for Y and
  * Z), "nadj" (their 15 lengths) (int); "ib_minus_w", "ib_plus_w" (double). */
 void* bcm_case_array(bcm_case* c, const char* name, long long* count);
-/* Writes BCMgrid00000.dat and BCM_STL_fine.stl, the reference's input files. */
+/* Writes BCMgrid00000.dat and BCM_STL_fine.stl, the reference's input files,
+ * and the rank-decomposition files it reads at start-up (main.cpp:213-221,
+ * 262-554, 584-596) for a decomposition into one rank:
+ * MPI_communication_table_in_turn.dat (0 cross-rank connections),
+ * Communication_irank.00000.dat (no neighbour ranks) and rank_map.dat (every
+ * cube on rank 0, local number = global number + 1). */
 int bcm_case_write_inputs(const bcm_case* c, const char* dir);
 /* Builds a case from the reference's input files in dir: the grid file
  * BCMgrid00000.dat as BCM_Grid.cpp:63-261 reads it (cube sizes and corners,
@@ -177,7 +182,10 @@ int bcm_case_write_inputs(const bcm_case* c, const char* dir);
  * setup of main.cpp:108-1139.  p supplies the flow parameters the reference
  * keeps in prm.h: mach, reynolds, sphere_diameter (the reference length of
  * mu_L), seed and noise_amp; its lattice, cube_size, origin, sphere_center
- * and ico_level are not used. */
+ * and ico_level are not used.  Rank-decomposition files present in dir are
+ * checked: a decomposition over several ranks (cross-rank connections,
+ * neighbour ranks, cubes on another rank or a rank map of another cube
+ * count) is refused with an error, since one GPU runs a one-rank case. */
 int bcm_case_read(const char* dir, const bcm_case_params* p, bcm_case** out);
 /* bcm_case_read with the IB search on GPU `device` (as bcm_case_build_on). */
 int bcm_case_read_on(const char* dir, const bcm_case_params* p, int device, bcm_case** out);

@@ -9,6 +9,10 @@ Pins:
   every array (the generated case is pinned to the reference's setup);
 * the reference-interface drop-ins BCM_Grid and BCM_Reading_IBM fill the
   caller's arrays as the reference's readers do;
+* the rank-decomposition files written for one rank read, through a
+  restatement of the reference's fscanf sequence, as no cross-rank
+  connections, no neighbour ranks and every cube on rank 0, and the case
+  reader refuses decompositions over several ranks;
 * the PLOT3D layout follows from the state by an independent numpy
   transposition; the restart snapshot round-trips bit for bit and rejects
   files that do not fit.
@@ -118,6 +122,116 @@ def test_read_case_reports_missing_and_corrupt_files(B, tmp_path):
     stl.write_text(text[: text.index("vertex", 2000) + 8])
     with pytest.raises(B.BCMError, match="STL facet"):
         B.Case.from_files(str(tmp_path), p)
+
+
+class _Scan:
+    """The two fscanf directives the reference reads these files with:
+    "%[^\\n]\\n" (a non-empty rest of line, then white space) and "%d\\n"."""
+
+    def __init__(self, text):
+        self.t, self.i = text, 0
+
+    def _ws(self):
+        while self.i < len(self.t) and self.t[self.i] in " \t\r\n":
+            self.i += 1
+
+    def line(self):
+        end = self.t.find("\n", self.i)
+        end = len(self.t) if end < 0 else end
+        assert end > self.i, "empty title line"
+        s, self.i = self.t[self.i:end], end
+        self._ws()
+        return s
+
+    def int(self):
+        self._ws()
+        j = self.i + (self.t[self.i] in "+-")
+        while j < len(self.t) and self.t[j].isdigit():
+            j += 1
+        v, self.i = int(self.t[self.i:j]), j
+        self._ws()
+        return v
+
+
+def _read_communication(text):
+    """main.cpp:262-554: per transfer kind three title lines, the neighbour
+    count, then a title line before each list (rank ids, cube counts, then
+    cube, direction and adjacency lists sized by the summed cube counts)."""
+    sc, out = _Scan(text), []
+    for n_cube_lists in (4, 5, 6):
+        for _ in range(3):
+            sc.line()
+        ncpu = sc.int()
+        sc.line()
+        cpus = [sc.int() for _ in range(ncpu)]
+        sc.line()
+        counts = [sc.int() for _ in range(ncpu)]
+        lists = []
+        for _ in range(n_cube_lists):
+            sc.line()
+            lists.append([sc.int() for _ in range(sum(counts))])
+        out.append((ncpu, cpus, counts, lists))
+    assert sc.i == len(text)
+    return out
+
+
+def test_rank_tables_read_as_one_rank(B, tmp_path):
+    p = B.case_params("tiny")
+    case = B.Case(p)
+    case.write_inputs(str(tmp_path))
+    n = case.info["ncube"] - 1
+    # main.cpp:215-221: lines up to the marker, then the connection count.
+    sc = _Scan((tmp_path / "MPI_communication_table_in_turn.dat").read_text())
+    while sc.line() != "Number of Cubes for connection  >>":
+        pass
+    assert sc.int() == 0
+    for ncpu, cpus, counts, lists in _read_communication(
+            (tmp_path / "Communication_irank.00000.dat").read_text()):
+        assert ncpu == 0 and cpus == [] and counts == [] and all(x == [] for x in lists)
+    # main.cpp:584-596 and the output offsets of BCM_Output.cpp:68-77: every
+    # cube on rank 0, whose last local number is its cube count.
+    sc = _Scan((tmp_path / "rank_map.dat").read_text())
+    sc.line()
+    rows = [(sc.int(), sc.int(), sc.int()) for _ in range(n)]
+    assert sc.i == len(sc.t)
+    assert [r[1] for r in rows] == [0] * n
+    assert rows[-1][2] == n and sorted(r[2] for r in rows) == list(range(1, n + 1))
+
+
+def test_read_case_refuses_several_ranks(B, tmp_path):
+    p = B.case_params("tiny")
+    case = B.Case(p)
+    n = case.info["ncube"] - 1
+    comm = tmp_path / "Communication_irank.00000.dat"
+    rmap = tmp_path / "rank_map.dat"
+    conn = tmp_path / "MPI_communication_table_in_turn.dat"
+
+    def planted(path, edit, match):
+        case.write_inputs(str(tmp_path))
+        path.write_text(edit(path.read_text()))
+        with pytest.raises(B.BCMError, match=match):
+            B.Case.from_files(str(tmp_path), p)
+
+    several = "several ranks"
+    planted(conn, lambda t: t.replace(">>\n0\n", ">>\n12\n"), several)
+    planted(conn, lambda t: t.replace(">>\n0\n", ">>\n"), "connection count")
+    lines_at = lambda t, i, v: "\n".join(v if k == i else x for k, x in enumerate(t.split("\n")))
+    case.write_inputs(str(tmp_path))
+    zeros = [i for i, x in enumerate(comm.read_text().split("\n")) if x == "0"]
+    assert len(zeros) == 3
+    for at in zeros:  # neighbour count of each transfer kind
+        planted(comm, lambda t: lines_at(t, at, "2"), several)
+    planted(rmap, lambda t: t.replace(f"\n{n - 1}\t0\t{n}\n", f"\n{n - 1}\t1\t1\n"), several)
+    planted(rmap, lambda t: t + f"{n}\t0\t{n + 1}\n", several)
+    planted(rmap, lambda t: t.replace(f"{n - 1}\t0\t{n}\n", ""), f"maps {n - 1} cubes")
+    planted(rmap, lambda t: t.replace("\n0\t0\t1\n", "\n0\t0\t0\n"), "local cube number")
+    planted(rmap, lambda t: t + "x\n", "row unreadable")
+    # Absent decomposition files are accepted; the written ones read back.
+    case.write_inputs(str(tmp_path))
+    B.Case.from_files(str(tmp_path), p)
+    for f in (comm, rmap, conn):
+        f.unlink()
+    assert B.Case.from_files(str(tmp_path), p).info == case.info
 
 
 def _dropin(B):
