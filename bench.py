@@ -175,12 +175,16 @@ def valu_roofline(precision, cells, flux_ms):
     if per_cell is None:
         return None
     achieved = per_cell * cells / (flux_ms * 1e-3) / 1e12
+    # HBM fraction the kernel would reach at the full issue rate: its
+    # instruction count caps the cell-stage rate, and with it the bytes moved.
+    hbm_cap = VALU_PEAK_TLANE * 1e12 / per_cell * ALGO_BYTES[precision] / 1e9 / HBM_PEAK_GBS
     return {
         "instr_per_cell_stage": per_cell,
         "achieved": round(achieved, 3),
         "peak": round(VALU_PEAK_TLANE, 3),
         "unit": "T lane-instr/s",
         "frac": round(achieved / VALU_PEAK_TLANE, 4),
+        "hbm_frac_at_full_issue": round(hbm_cap, 4),
         "source": "profiles/flux_cfg2_counters.txt (SQ_INSTS_VALU)",
     }
 
