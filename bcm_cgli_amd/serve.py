@@ -18,6 +18,10 @@ inner iteration of a step, with scales (P0, U0, U0, U0, P0/rho0); er[1..5] are
 the reference's change norms (BCM_Flux_XYZ_Viscous_Runge_kutta.cpp:2530-2635).
 The reference's own inner loop instead stops on running-max-normalised norms
 below 0.02 (main.cpp:1248-1271); a serving request needs an absolute stop.
+With inner_tol set, each step also runs that inner control: a case's inner
+loop ends once its five running-max normalised norms are below inner_tol (or
+after n_inner iterations), the maxima carried from step to step per request,
+and the residual is taken at the case's last inner iteration of the step.
 """
 import threading
 import time
@@ -40,6 +44,8 @@ class Request:
     converged: bool = False
     residual: float = None
     er: np.ndarray = None        # er[1..7] of the last inner iteration
+    E: np.ndarray = None         # running maxima E1..E5 (inner control)
+    inner_iters: int = 0         # inner iterations run
     slot: int = None
     case: object = None
     scales: tuple = None
@@ -66,11 +72,12 @@ class Server:
 
     def __init__(self, engine, make_case, precision="mixed", budget_bytes=64 << 30,
                  bytes_per_cube=None, max_cases=64, n_inner=10, max_steps=300, tol=1e-8,
-                 prep_threads=4, keep_states=False, clock=time.perf_counter):
+                 prep_threads=4, keep_states=False, clock=time.perf_counter, inner_tol=None):
         self.engine = engine
         self.make_case = make_case
         self.precision = precision
         self.n_inner = int(n_inner)
+        self.inner_tol = None if inner_tol is None else float(inner_tol)
         self.max_steps = int(max_steps)
         self.tol = float(tol)
         self.max_cases = int(max_cases)
@@ -153,21 +160,33 @@ class Server:
                     if info["active"] >= self.max_cases or info["free_range"] < r.cubes:
                         break
                     r.slot = eng.add_case(r.case)
+                    r.E = np.zeros(5)
                     r.admit = self.clock() - t0
                     resident[r.slot] = pending.pop(0)
                 if not resident:
                     continue
                 max_resident = max(max_resident, len(resident))
                 ts = self.clock()
-                er = eng.run(1, self.n_inner)      # (n_inner, max_cases, 7)
+                if self.inner_tol is None:
+                    er = eng.run(1, self.n_inner)      # (n_inner, max_cases, 7)
+                    n_it = {slot: self.n_inner for slot in resident}
+                else:
+                    E = np.zeros((eng.max_cases, 5))
+                    for slot, r in resident.items():
+                        E[slot] = r.E
+                    iters, er, E = eng.run_converge(1, self.n_inner, self.inner_tol, E=E)
+                    n_it = {slot: int(iters[0, slot]) for slot in resident}
+                    for slot, r in resident.items():
+                        r.E = np.array(E[slot])
                 busy += self.clock() - ts
                 steps += 1
-                cell_iters += sum(r.case.cells() for r in resident.values()) * self.n_inner
+                cell_iters += sum(r.case.cells() * n_it[s] for s, r in resident.items())
                 self.trace.append((self.clock() - t0, len(resident),
                                    sum(r.cubes for r in resident.values())))
                 for slot, r in list(resident.items()):
                     r.steps += 1
-                    r.er = np.array(er[-1, slot, :])
+                    r.inner_iters += n_it[slot]
+                    r.er = np.array(er[n_it[slot] - 1, slot, :])
                     r.residual = residual(r.er, r.scales)
                     r.converged = r.residual < self.tol
                     limit = r.max_steps if r.max_steps is not None else self.max_steps
@@ -207,6 +226,10 @@ class Server:
             "admission_wait_p50_s": pct(wait, 50),
             "admission_wait_p99_s": pct(wait, 99),
             "converged": sum(1 for r in self.requests if r.converged),
+            "inner_control": ("fixed" if self.inner_tol is None
+                              else f"reference, tolerance {self.inner_tol:g}"),
+            "inner_iters_per_step": (sum(r.inner_iters for r in ok) /
+                                     max(sum(r.steps for r in ok), 1)),
             "max_resident": max_resident,
             "pool_cubes": self.cap_cubes,
         }
@@ -278,10 +301,14 @@ def bench(args, metric):
         mean_cells = np.mean([np.prod(p["lattice"]) * 4096 for _, p in shape])
         rate = 0.8 * capacity / (mean_cells * n_inner * args.max_steps)
     reqs = [(t / rate, p) for t, p in shape]
+    inner_tol = getattr(args, "inner_tol", None)
+    inner_desc = (f"{n_inner} inner iterations" if inner_tol is None else
+                  f"at most {n_inner} inner iterations (the reference's inner control, tolerance "
+                  f"{inner_tol:g})")
     eng = Engine(0)
     srv = Server(eng, make_case, precision=prec, budget_bytes=64 << 30,
                  bytes_per_cube=bytes_per_cube(prec), max_cases=64, n_inner=n_inner,
-                 max_steps=args.max_steps, tol=1e-8)
+                 max_steps=args.max_steps, tol=1e-8, inner_tol=inner_tol)
     for t, p in reqs:
         srv.submit(p, t)
     print(f"[bench] serving {len(reqs)} requests at {rate:.3f}/s", file=sys.stderr, flush=True)
@@ -304,7 +331,7 @@ def bench(args, metric):
             "workload": f"serving: {len(reqs)} requests, Poisson arrivals at {rate:.3f}/s, cubes "
                         "uniform in [27,1000] (near-cubic lattices), M in [0.1,0.5], Re in "
                         f"[50,1000], icosphere level 3-5, until inner residual < 1e-8 or "
-                        f"{args.max_steps} steps of {n_inner} inner iterations, mixed precision, "
+                        f"{args.max_steps} steps of {inner_desc}, mixed precision, "
                         f"HBM budget 64 GiB, case setup on 4 host threads with the IB search on "
                         f"the GPU",
             "rate_per_s": rate,

@@ -62,6 +62,10 @@ def parse_args():
     ap.add_argument("--requests", type=int, default=200)
     ap.add_argument("--rate", type=float, default=None, help="serve: arrivals per second")
     ap.add_argument("--max-steps", type=int, default=300)
+    ap.add_argument("--inner-tol", type=float, default=None,
+                    help="serve: end each request's inner loop as the reference does, once its "
+                         "running-max normalised norms fall below this tolerance (0.02 in the "
+                         "reference); default: always --inner iterations")
     return ap.parse_args()
 
 

@@ -48,6 +48,52 @@ def test_served_results_match_alone(B, precision):
         assert np.array_equal(r.er, er1[:7]), r.id
 
 
+def run_alone_converge(B, params, n_steps, n_inner, inner_tol, precision):
+    case = B.Case(params)
+    e = B.Engine(0)
+    try:
+        e.alloc_pool(case.ncube - 1, 1, precision)
+        s = e.add_case(case)
+        iters, er, E = e.run_converge(n_steps, n_inner, inner_tol)
+        last = er[(n_steps - 1) * n_inner + iters[-1, s] - 1, s]
+        return ({n: e.case_state(s, n) for n in ("U1_", "U1", "U1q")}, int(iters[:, s].sum()),
+                last, E[s])
+    finally:
+        e.close()
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_served_inner_control_matches_alone(B, precision):
+    # Under the reference's inner control the 64-cube requests end inner
+    # loops early from step 4 on (tolerance 0.97, at most 8 iterations; the
+    # FP64 oracle takes 8, 8, 8, 6, 5, 4 for the first), while the request
+    # admitted later still runs full loops in the same launches.  Each result
+    # is bitwise the request's run alone, with its maxima carried per step.
+    reqs = poisson_requests(3, rate=40.0, seed=11, cubes=(27, 64), ico_levels=(3, 3))
+    steps = [6, 3, 6]
+    e = B.Engine(0)
+    srv = Server(e, B.Case, precision=precision, budget_bytes=150 * B.bytes_per_cube(precision),
+                 max_cases=2, n_inner=8, max_steps=100, tol=1e-30, keep_states=True,
+                 inner_tol=0.97)
+    for (t, p), n in zip(reqs, steps):
+        srv.submit(p, 0.0, max_steps=n)
+    st = srv.run()
+    e.close()
+    assert st["failed"] == 0 and st["requests"] == 3
+    rs = srv.requests
+    assert any(a.admit < b.admit < a.done for a in rs for b in rs if a is not b)
+    assert any(r.inner_iters < 8 * r.steps for r in rs)
+    assert st["cell_iters"] == sum(r.case.cells() * r.inner_iters for r in rs)
+    for r, n in zip(rs, steps):
+        assert r.steps == n
+        st1, it1, er1, E1 = run_alone_converge(B, r.params, n, 8, 0.97, precision)
+        assert r.inner_iters == it1, r.id
+        for name in ("U1_", "U1", "U1q"):
+            assert np.array_equal(r.state[name], st1[name]), (r.id, name)
+        assert np.array_equal(r.er, er1[:7]), r.id
+        assert np.array_equal(r.E, E1), r.id
+
+
 def test_serve_stats(B):
     reqs = poisson_requests(4, rate=20.0, seed=5, cubes=(27, 40), ico_levels=(3, 4))
     e = B.Engine(0)

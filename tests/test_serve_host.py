@@ -76,6 +76,26 @@ class FakeEngine:
         assert used <= self.cap
         return er
 
+    def run_converge(self, n_steps, max_inner, tol, E=None):
+        # The reference's inner control over the same decaying norms: per
+        # inner iteration the running maxima take the norms, and the loop
+        # ends once all five norms are below tol times their maxima.
+        time.sleep(self.step_time)
+        E = np.array(E, dtype=float).reshape(self.max_cases, 5).copy()
+        iters = np.zeros((n_steps, self.max_cases), dtype=np.int32)
+        er = np.zeros((n_steps * max_inner, self.max_cases, 7))
+        for st in range(n_steps):
+            for s, v in self.slots.items():
+                for it in range(max_inner):
+                    v[1] *= v[0].decay
+                    e5 = v[1] * np.arange(1.0, 6.0)
+                    er[st * max_inner + it, s, :5] = e5
+                    E[s] = np.maximum(E[s], e5)
+                    iters[st, s] = it + 1
+                    if np.all(e5 < tol * E[s]):
+                        break
+        return iters, er, E
+
     def case_state(self, slot, name):
         return np.full(3, slot, dtype=float)
 
@@ -147,6 +167,49 @@ def test_slot_limit():
         srv.submit({"cubes": 1}, 0.0)
     srv.run()
     assert max(n for _, n, _ in srv.trace) == 2
+
+
+def alone_inner_control(decay, n_inner, inner_tol, tol, max_steps):
+    """One request by itself under the inner control: (steps, inner
+    iterations, last norms, E), written out from the rule directly."""
+    v, E, steps, total, last = 1.0, np.zeros(5), 0, 0, None
+    while True:
+        for it in range(n_inner):
+            v *= decay
+            last = v * np.arange(1.0, 6.0)
+            E = np.maximum(E, last)
+            total += 1
+            if np.all(last < inner_tol * E):
+                break
+        steps += 1
+        if max(last) < tol or steps >= max_steps:
+            return steps, total, last, E
+
+
+def test_inner_control_per_request():
+    # Four requests with different decay rates share the device; each ends
+    # its inner loops, carries its maxima and retires as it would alone, and
+    # the throughput counts the inner iterations that ran.
+    eng = FakeEngine()
+    srv = make_server(eng, 60, max_cases=3, n_inner=8, max_steps=12, tol=1e-6,
+                      inner_tol=0.05)
+    decays = [0.5, 0.7, 0.9, 0.3]
+    for i, d in enumerate(decays):
+        srv.submit({"cubes": 20, "decay": d}, 0.001 * i)
+    st = srv.run()
+    assert st["failed"] == 0
+    assert st["inner_control"] == "reference, tolerance 0.05"
+    total = 0
+    for r, d in zip(srv.requests, decays):
+        steps, iters, last, E = alone_inner_control(d, 8, 0.05, 1e-6, 12)
+        assert (r.steps, r.inner_iters) == (steps, iters), d
+        assert np.allclose(r.er[:5], last, rtol=1e-12) and np.allclose(r.E, E, rtol=1e-12)
+        total += iters * 20 * 4096
+    # Some inner loops ended before the cap.
+    assert any(r.inner_iters < 8 * r.steps for r in srv.requests)
+    assert st["cell_iters"] == total
+    assert st["inner_iters_per_step"] == pytest.approx(
+        sum(r.inner_iters for r in srv.requests) / sum(r.steps for r in srv.requests))
 
 
 def test_residual_scaling():
