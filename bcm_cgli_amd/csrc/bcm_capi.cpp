@@ -375,7 +375,7 @@ void bcm_engine_destroy(bcm_engine* h) {
   if (!h) return;
   if (g_current == h) g_current = nullptr;
   if (h->di)
-    for (void* p : h->di->owned) hipFree(p);
+    for (void* p : h->di->owned) (void)hipFree(p);
   engine_destroy(h->e);
   delete h;
 }
@@ -391,7 +391,7 @@ int bcm_engine_alloc_pool(bcm_engine* h, int cap_cubes, int max_cases, int preci
   if (!h) return fail("null engine");
   if (g_current == h) g_current = nullptr;
   if (h->di) {
-    for (void* p : h->di->owned) hipFree(p);
+    for (void* p : h->di->owned) (void)hipFree(p);
     h->di.reset();
   }
   std::string err;
@@ -594,7 +594,7 @@ int bcm_engine_time_part(bcm_engine* h, int part, int RK, int reps, double* ms) 
   if (reps < 1) return fail("reps must be positive");
   if (e->prec == 1 && (part == 3 || part == 4))
     return fail("the mixed path fuses flux and update (use part 7)");
-  hipEventRecord(e->ev0, e->stream);
+  if (hipEventRecord(e->ev0, e->stream) != hipSuccess) return fail("event record failed");
   for (int r = 0; r < reps; ++r) {
     switch (part) {
       case 0: engine_bc(e); break;
@@ -610,10 +610,10 @@ int bcm_engine_time_part(bcm_engine* h, int part, int RK, int reps, double* ms) 
       default: return fail("unknown part");
     }
   }
-  hipEventRecord(e->ev1, e->stream);
+  if (hipEventRecord(e->ev1, e->stream) != hipSuccess) return fail("event record failed");
   if (hipEventSynchronize(e->ev1) != hipSuccess) return fail("event wait failed");
   float t = 0;
-  hipEventElapsedTime(&t, e->ev0, e->ev1);
+  if (hipEventElapsedTime(&t, e->ev0, e->ev1) != hipSuccess) return fail("event timing failed");
   *ms = (double)t / reps;
   std::string err;
   if (engine_sync(e, err)) return fail(err);
@@ -668,7 +668,7 @@ int bcm_bind(bcm_engine* h, int ncube, int n_cubes_norm, double* U1_, double* U1
     return fail("null argument");
   std::string err;
   if (h->di)
-    for (void* p : h->di->owned) hipFree(p);
+    for (void* p : h->di->owned) (void)hipFree(p);
   h->di.reset(new DropIn());
   if (engine_alloc(h->e, ncube, 1, err)) return fail(err);
   Engine* e = h->e;
@@ -933,7 +933,7 @@ void BCM_Grid(int myid, int ncube, int n_wallcube, double* cube_size, int* csl, 
 void BCM_Reading_IBM(int myid, int* NNBC, int mp_switch, double* weight, int* GCindex,
                      int* IPsur) {
   const char* where = "BCM_Reading_IBM";
-  if (!NNBC || mp_switch != 1 && mp_switch != -1) fatal(where, "mp_switch must be 1 or -1");
+  if (!NNBC || (mp_switch != 1 && mp_switch != -1)) fatal(where, "mp_switch must be 1 or -1");
   char name[64];
   snprintf(name, sizeof name, "IBM_%s%05d.dat", mp_switch == 1 ? "plus" : "minus", myid);
   IBTable tb;

@@ -48,8 +48,8 @@ double* state_ptr(Engine* e, int which) {
 }
 
 void free_graph(Engine* e) {
-  if (e->graph_exec) hipGraphExecDestroy(e->graph_exec);
-  if (e->graph) hipGraphDestroy(e->graph);
+  if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
   e->graph_exec = nullptr;
   e->graph = nullptr;
   e->graph_valid = false;
@@ -60,7 +60,7 @@ void free_tables(Engine* e) {
   void* ptrs[] = {e->bc_cubes,     e->bc_masks, e->halo,     e->halo_pairs,
                   e->case_desc,    e->bc_mask_cube, e->ml, e->edge_list, e->open_items};
   for (void* p : ptrs)
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   ghost_free(e->ghost);
   e->bc_cubes = nullptr;
   e->bc_masks = nullptr;
@@ -82,11 +82,11 @@ void free_case(Engine* e) {
                   e->Res, e->staging,   e->fws,      e->cube_size, e->partials, e->er_dev,
                   e->phys_dev, e->cube_case, e->U2};
   for (void* p : ptrs)
-    if (p) hipFree(p);
-  if (e->Uf) hipFree(e->Uf);
-  if (e->Uf2) hipFree(e->Uf2);
+    if (p) (void)hipFree(p);
+  if (e->Uf) (void)hipFree(e->Uf);
+  if (e->Uf2) (void)hipFree(e->Uf2);
   e->Uf = e->Uf2 = nullptr;
-  if (e->er_host) hipHostFree(e->er_host);
+  if (e->er_host) (void)hipHostFree(e->er_host);
   e->U = e->U1 = e->U1q = e->U1p1 = e->U1p2 = e->Rku = e->Res = e->staging = e->U2 = nullptr;
   e->fws = nullptr;
   e->cube_size = nullptr;
@@ -243,7 +243,7 @@ int ghost_upload(const GhostSchedule& gs, GhostDevice& d, std::string& err) {
 
 void ghost_free(GhostDevice& d) {
   for (void* p : d.owned)
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   d.owned.clear();
   d.t = GhostTables();
 }
@@ -277,13 +277,13 @@ int engine_create(int device, Engine** out, std::string& err) {
 
 void engine_destroy(Engine* e) {
   if (!e) return;
-  hipSetDevice(e->device);
-  if (e->stream) hipStreamSynchronize(e->stream);
+  (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
   engine_output_free(e);
   free_case(e);
-  if (e->ev0) hipEventDestroy(e->ev0);
-  if (e->ev1) hipEventDestroy(e->ev1);
-  if (e->stream) hipStreamDestroy(e->stream);
+  if (e->ev0) (void)hipEventDestroy(e->ev0);
+  if (e->ev1) (void)hipEventDestroy(e->ev1);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
 }
 
@@ -971,7 +971,7 @@ int engine_autotune(Engine* e, int* variant, int* nseg, std::string& err) {
     }
   }
   HIP_TRY(hipStreamSynchronize(e->stream));
-  for (int a = 0; a < 6; ++a) hipFree(saved[a]);
+  for (int a = 0; a < 6; ++a) (void)hipFree(saved[a]);
   e->flux_push = bp;
   e->flux_variant = bv;
   e->flux_nseg = bs;
@@ -1039,8 +1039,8 @@ void engine_stage(Engine* e, int RK, int er_slot, bool full_pre_halo) {
 
 int ensure_er(Engine* e, int slots, std::string& err) {
   if (e->er_slots >= slots) return 0;
-  if (e->er_dev) hipFree(e->er_dev);
-  if (e->er_host) hipHostFree(e->er_host);
+  if (e->er_dev) (void)hipFree(e->er_dev);
+  if (e->er_host) (void)hipHostFree(e->er_host);
   e->er_dev = nullptr;
   e->er_host = nullptr;
   free_graph(e);

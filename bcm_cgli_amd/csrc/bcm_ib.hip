@@ -137,7 +137,7 @@ template <typename T>
 struct DevBuf {
   T* p = nullptr;
   ~DevBuf() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   }
   int alloc(size_t n, std::string& err) {
     HIP_TRY(hipMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T)));
@@ -149,7 +149,7 @@ class GpuIb : public IbDevice {
  public:
   explicit GpuIb(int device) : device_(device) {}
   ~GpuIb() override {
-    if (stream_) hipStreamDestroy(stream_);
+    if (stream_) (void)hipStreamDestroy(stream_);
   }
 
   int init(std::string& err) {

@@ -297,12 +297,12 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
   const int j = (t >> 4) + 2, k = (t & 15) + 2;
   const int col = j * 20 + k;
   const double dxd = cube_size[c] / 16;
-  const float dx = (float)dxd, inv = (float)(1.0 / dxd), vol = (float)(dxd * dxd * dxd);
+  const float inv = (float)(1.0 / dxd), vol = (float)(dxd * dxd * dxd);
   const float e = (float)e_d;
   const RefF rf = make_ref(ph);
   double uref[5];
   ref_state(ph, uref);
-  const float T0 = (float)ph.T0, iT0 = (float)(1.0 / ph.T0), muL = (float)ph.mu_L;
+  const float iT0 = (float)(1.0 / ph.T0), muL = (float)ph.mu_L;
   const float sut = (float)(ph.T0 + 110.);
   const double i2dT = 1.0 / (2 * deltaT);
   const float dT = (float)deltaT, dtau = (float)deltaTau;

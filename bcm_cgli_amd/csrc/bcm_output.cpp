@@ -50,10 +50,10 @@ struct OutputJob {
   std::string err;
   ~OutputJob() {
     if (th.joinable()) th.join();
-    if (copy_stream) hipStreamSynchronize(copy_stream);
-    if (converted) hipEventDestroy(converted);
-    if (copied) hipEventDestroy(copied);
-    if (host.p) hipHostFree(host.p);
+    if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+    if (converted) (void)hipEventDestroy(converted);
+    if (copied) (void)hipEventDestroy(copied);
+    if (host.p) (void)hipHostFree(host.p);
   }
 };
 
@@ -90,7 +90,7 @@ int take_pinned(OutputQueue* q, size_t bytes, PinnedBuf& out, std::string& err) 
       q->spare.erase(q->spare.begin() + i);
       return 0;
     }
-  for (PinnedBuf& b : q->spare) hipHostFree(b.p);
+  for (PinnedBuf& b : q->spare) (void)hipHostFree(b.p);
   q->spare.clear();
   out.bytes = bytes;
   HIP_TRY(hipHostMalloc((void**)&out.p, bytes, hipHostMallocDefault));
@@ -195,11 +195,11 @@ void engine_output_free(Engine* e) {
   if (!e->out) return;
   OutputQueue* q = e->out;
   while (!q->jobs.empty()) retire(q);
-  if (q->stream) hipStreamSynchronize(q->stream);
-  for (PinnedBuf& b : q->spare) hipHostFree(b.p);
-  if (q->dev) hipFree(q->dev);
-  if (q->dev_free) hipEventDestroy(q->dev_free);
-  if (q->stream) hipStreamDestroy(q->stream);
+  if (q->stream) (void)hipStreamSynchronize(q->stream);
+  for (PinnedBuf& b : q->spare) (void)hipHostFree(b.p);
+  if (q->dev) (void)hipFree(q->dev);
+  if (q->dev_free) (void)hipEventDestroy(q->dev_free);
+  if (q->stream) (void)hipStreamDestroy(q->stream);
   delete q;
   e->out = nullptr;
 }
