@@ -203,6 +203,20 @@ def test_cfg2_one_inner_iteration_bitwise(B, O, engine):
     assert sha(engine.get_state("U1_")) == sha(orc.state["U1_"])
 
 
+def test_cfg2_one_physical_step_bitwise(B, O, engine):
+    # Full-size case, one physical step of 10 inner iterations as the bench
+    # runs it (one captured graph: 30 pushed stages, the run-end edge pass and
+    # the physical-time shift) against the oracle on the host cores: the three
+    # time levels bitwise, the norms of every inner iteration at 1e-12.
+    case = B.Case(B.case_params("cfg2"))
+    engine.load_case(case)
+    h = engine.run(1, 10)
+    orc = O.OracleRun(case, nthreads=16)
+    ho = orc.run(1, 10)
+    compare(engine, orc, ("U1_", "U1", "U1q"))
+    assert er_close(h, ho)
+
+
 def _dropin_run(B, case, n_steps, n_inner, output_dir=None):
     """Runs the reference main-loop call sequence (main.cpp:1146-1357) through
     the drop-in BCM_* entry points on host arrays bound to an engine; with
