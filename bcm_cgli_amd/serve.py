@@ -72,12 +72,14 @@ class Server:
 
     def __init__(self, engine, make_case, precision="mixed", budget_bytes=64 << 30,
                  bytes_per_cube=None, max_cases=64, n_inner=10, max_steps=300, tol=1e-8,
-                 prep_threads=4, keep_states=False, clock=time.perf_counter, inner_tol=None):
+                 prep_threads=4, keep_states=False, clock=time.perf_counter, inner_tol=None,
+                 progress=None):
         self.engine = engine
         self.make_case = make_case
         self.precision = precision
         self.n_inner = int(n_inner)
         self.inner_tol = None if inner_tol is None else float(inner_tol)
+        self.progress = progress     # called with (finished, total) as requests finish
         self.max_steps = int(max_steps)
         self.tol = float(tol)
         self.max_cases = int(max_cases)
@@ -198,6 +200,8 @@ class Server:
                         r.case = r.case if self.keep_states else None
                         del resident[slot]
                         n_done += 1
+                        if self.progress:
+                            self.progress(n_done, n_total)
         finally:
             pool.shutdown(wait=True)
             for f in futures:
@@ -306,9 +310,12 @@ def bench(args, metric):
                   f"at most {n_inner} inner iterations (the reference's inner control, tolerance "
                   f"{inner_tol:g})")
     eng = Engine(0)
+    def progress(done, total):
+        if done % 10 == 0 or done == total:
+            print(f"[bench] {done}/{total} requests finished", file=sys.stderr, flush=True)
     srv = Server(eng, make_case, precision=prec, budget_bytes=64 << 30,
                  bytes_per_cube=bytes_per_cube(prec), max_cases=64, n_inner=n_inner,
-                 max_steps=args.max_steps, tol=1e-8, inner_tol=inner_tol)
+                 max_steps=args.max_steps, tol=1e-8, inner_tol=inner_tol, progress=progress)
     for t, p in reqs:
         srv.submit(p, t)
     print(f"[bench] serving {len(reqs)} requests at {rate:.3f}/s", file=sys.stderr, flush=True)
