@@ -38,7 +38,7 @@ STEPPING = {"tiny": (2, 3), "cfg1": (20, 10), "cfg2": (200, 10), "batch": (50, 1
 # physical steps, the iteration cap (iteration_end_step) and the tolerance on
 # the running-max normalised norms.  The reference's tolerance is 0.02; 0.97
 # makes these short runs end their inner loops at different iterations.
-CONVERGE = {"tiny": (8, 12, 0.97), "ml_tiny": (8, 12, 0.97), "cfg1": (12, 10, 0.97)}
+CONVERGE = {"tiny": (8, 12, 0.97), "ml_tiny": (8, 12, 0.97), "cfg1": (12, 10, 0.97), "ml": (12, 10, 0.97)}
 
 # Cube lattices of the batched workload: n^3 cubes on the unit box.
 BATCH_LATTICES = (2, 3, 4, 5, 6, 8)

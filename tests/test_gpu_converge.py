@@ -33,7 +33,7 @@ def ran_rows(iters, max_inner):
     return np.concatenate([np.arange(s * max_inner, s * max_inner + n) for s, n in enumerate(iters)])
 
 
-@pytest.mark.parametrize("name", ["tiny", "ml_tiny", "cfg1"])
+@pytest.mark.parametrize("name", ["tiny", "ml_tiny", "cfg1", "ml"])
 def test_converge_matches_reference(B, name):
     g = golden_converge(name)
     n_steps, max_inner, tol = CONVERGE[name]

@@ -63,7 +63,7 @@ def test_oracle_two_level_bitwise(B, O, name):
     assert np.array_equal(hist.view(np.int64), g["er"].view(np.int64))
 
 
-@pytest.mark.parametrize("name", ["tiny", "ml_tiny", "cfg1"])
+@pytest.mark.parametrize("name", ["tiny", "ml_tiny", "cfg1", "ml"])
 def test_oracle_convergence_break_bitwise(B, O, name):
     # The reference's inner-iteration control (running-max normalised norms
     # below the tolerance, or the iteration cap) ends each step's inner loop
