@@ -56,6 +56,27 @@ def test_mixed_against_oracle(B, O, name):
     assert cd_err <= CD_TOL
 
 
+def test_mixed_full_size_against_fp64(B):
+    # The full-size case (16x16x8 cubes, 8.4M cells), 3 steps of 10 inner
+    # iterations: the mixed path against the FP64 path on the GPU, which is
+    # bitwise the oracle (one full step of it is checked against the oracle
+    # in the parity tests), held to the same bar.
+    case = B.Case(B.case_params("cfg2"))
+    st, er = run_mixed(B, case, 3, 10)
+    e = B.Engine(0)
+    try:
+        e.load_case(case)
+        ref_er = e.run(3, 10)
+        ref = {"U1": e.get_state("U1")}
+    finally:
+        e.close()
+    l2, cd_err = accuracy(case, st, er, ref, ref_er)
+    print(f"cfg2: density rel L2 {l2:.3e}, Cd rel err {cd_err:.3e}")
+    assert np.all(np.isfinite(st["U1_"]))
+    assert l2 <= RHO_L2_TOL
+    assert cd_err <= CD_TOL
+
+
 def test_mixed_pool_matches_alone(B):
     params = batch_cases(3, seed=77, lattices=(2, 3), ico_level=3)
     alone = [run_mixed(B, B.Case(p), 2, 2) for p in params]
