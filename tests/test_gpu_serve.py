@@ -48,6 +48,38 @@ def test_served_results_match_alone(B, precision):
         assert np.array_equal(r.er, er1[:7]), r.id
 
 
+def test_served_serving_distribution_matches_alone(B):
+    # Ten requests drawn from the serving workload's own distribution (cube
+    # counts uniform in [27, 1000], icosphere level 3-5, M in [0.1, 0.5], Re
+    # in [50, 1000]), mixed precision, 10 inner iterations per step, the 1e-8
+    # residual stop, per-request step limits of 2-7 steps.  A 2,500-cube
+    # budget and 4 slots make later requests wait for retirements and land in
+    # ranges earlier requests freed.  Each result is bitwise its run alone.
+    reqs = poisson_requests(10, rate=50.0, seed=2026)
+    steps = [2 + (i * 3) % 6 for i in range(len(reqs))]
+    cubes = [int(np.prod(p["lattice"])) for _, p in reqs]
+    assert max(cubes) > 600 and min(cubes) < 400
+    assert len({p["ico_level"] for _, p in reqs}) >= 2
+    e = B.Engine(0)
+    srv = Server(e, lambda p: B.Case(p, device=0), precision="mixed",
+                 budget_bytes=2500 * B.bytes_per_cube("mixed"), max_cases=4, n_inner=10,
+                 max_steps=300, tol=1e-8, keep_states=True)
+    for (t, p), n in zip(reqs, steps):
+        srv.submit(p, 0.0, max_steps=n)
+    st = srv.run()
+    e.close()
+    assert st["failed"] == 0 and st["requests"] == len(reqs)
+    assert max(c for _, _, c in srv.trace) <= 2500
+    rs = srv.requests
+    assert any(a.admit < b.admit < a.done for a in rs for b in rs if a is not b)
+    for r in rs:
+        assert 1 <= r.steps
+        st1, er1 = run_alone(B, r.params, r.steps, 10, "mixed")
+        for name in ("U1_", "U1", "U1q"):
+            assert np.array_equal(r.state[name], st1[name]), (r.id, name)
+        assert np.array_equal(r.er, er1[:7]), r.id
+
+
 def run_alone_converge(B, params, n_steps, n_inner, inner_tol, precision):
     case = B.Case(params)
     e = B.Engine(0)
