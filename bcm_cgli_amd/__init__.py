@@ -226,6 +226,19 @@ class Case:
 
 PRECISIONS = {"fp64": 0, "mixed": 1}
 
+# Area of the reference's force normalisation: er[6] and er[7] are the x- and
+# z-forces divided by 0.5 rho0 U0^2 times this constant
+# (BCM_Flux_XYZ_Viscous_Runge_kutta.cpp:2637-2638).
+FORCE_AREA = 2.602
+
+
+def drag_coefficient(er6, diameter):
+    """Drag coefficient of the sphere of diameter D from the normalised
+    x-force er[6]: Cd = -er[6] * 2.602 / (pi D^2 / 4).  The reference prints
+    -er[6] as its force coefficient (main.cpp:1284); its area 2.602 is
+    replaced by the sphere's frontal area."""
+    return -float(er6) * FORCE_AREA / (np.pi * float(diameter) ** 2 / 4.0)
+
 
 def write_restart(path, step, U1_, U1, U1q, E=None):
     """Restart snapshot of the three time levels (AoS states of one case),

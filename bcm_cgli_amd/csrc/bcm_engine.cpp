@@ -618,15 +618,27 @@ int engine_free_range(const Engine* e) {
 }
 
 namespace {
-// First pool cube of a free run of n cubes, or 0.
+// First pool cube of the smallest free run of at least n cubes (the lowest
+// such run on ties), or 0.  Best fit keeps large runs whole for large cases.
 int find_range(const Engine* e, int n) {
-  int run = 0;
-  for (int c = 1; c < e->cap; ++c) {
-    run = e->cube_case_host[c] < 0 ? run + 1 : 0;
-    if (run == n) return c - n + 1;
+  int best = 0, best_len = 0, run = 0;
+  for (int c = 1; c <= e->cap; ++c) {
+    if (c < e->cap && e->cube_case_host[c] < 0) {
+      ++run;
+      continue;
+    }
+    if (run >= n && (best_len == 0 || run < best_len)) {
+      best = c - run;
+      best_len = run;
+    }
+    run = 0;
   }
-  return 0;
+  return best;
 }
+}  // namespace
+
+namespace {
+int place_case(Engine* e, const Case& cs, int slot, std::string& err);
 }  // namespace
 
 int engine_add_case(Engine* e, const Case& cs, int* slot_out, std::string& err) {
@@ -655,6 +667,26 @@ int engine_add_case(Engine* e, const Case& cs, int* slot_out, std::string& err) 
   S.tb = std::move(tb);
   S.active = true;
   const int n = S.tb.n;
+  if (place_case(e, cs, slot, err)) {
+    // Roll back: the slot and its cubes are free again and the tables of the
+    // other cases are rebuilt without it.
+    for (int c = first; c < first + n; ++c) e->cube_case_host[c] = -1;
+    S = CaseSlot();
+    std::string err2;
+    (void)rebuild_tables(e, err2);
+    e->halo_full_next = true;
+    return 1;
+  }
+  if (slot_out) *slot_out = slot;
+  return 0;
+}
+
+namespace {
+// Uploads the flags, sizes and state of a case just given slot `slot` and
+// its range, and rebuilds the pool tables.
+int place_case(Engine* e, const Case& cs, int slot, std::string& err) {
+  CaseSlot& S = e->cases[slot];
+  const int first = S.first, n = S.tb.n;
   S.centres.assign((size_t)3 * (n + 1) * NP, 0.0);
   for (int a = 0; a < 3; ++a) {
     const std::vector<double>& src = a == 0 ? cs.Xcnt : a == 1 ? cs.Ycnt : cs.Zcnt;
@@ -681,9 +713,9 @@ int engine_add_case(Engine* e, const Case& cs, int* slot_out, std::string& err) 
   for (int w = 0; w < 3; ++w)
     if (engine_case_state(e, slot, w, const_cast<double*>(st[w]), true, err)) return 1;
   e->halo_full_next = true;
-  if (slot_out) *slot_out = slot;
   return 0;
 }
+}  // namespace
 
 int engine_remove_case(Engine* e, int slot, std::string& err) {
   if (slot < 0 || slot >= e->max_cases || !e->cases[slot].active) {

@@ -5,13 +5,16 @@ Requests arrive over time (each with its own cube count, M, Re and
 icosphere level).  A pool of host threads builds each case (grid flags,
 IB tables, initial state) as soon as it arrives.  The device loop admits built
 cases in arrival order while a case slot and a contiguous range of pool cubes
-are free (the pool is sized from the HBM budget once), then runs one physical
-step of every resident case per iteration, every kernel launch covering all
-resident cubes.  After each step a case whose inner residual fell below the
-tolerance, or that reached the step limit, is retired and its cubes are
-reused by the next admissions.  So requests arriving while others run are
-admitted as soon as slots and memory free up, and their work overlaps on the
-device.
+are free (the pool is sized from the HBM budget once; the engine places a
+case in the smallest free range that holds it); a request that does not fit
+yet may be passed by later ones that do, at most max_skips times.  It then
+runs one physical step of every resident case per iteration, every kernel
+launch covering all resident cubes.  After each step a case whose inner
+residual fell below the tolerance, or that reached the step limit, is retired
+and its cubes are reused by the next admissions; a case whose norms turned
+NaN or infinite (diverged) is retired as failed.  So requests arriving while
+others run are admitted as soon as slots and memory free up, and their work
+overlaps on the device.
 
 Inner residual of a case: max over m = 1..5 of er[m] / scale[m] at the last
 inner iteration of a step, with scales (P0, U0, U0, U0, P0/rho0); er[1..5] are
@@ -52,6 +55,7 @@ class Request:
     state: dict = field(default_factory=dict)
     error: str = None
     max_steps: int = None        # per-request step limit (None: the server's)
+    skipped: int = 0             # later requests admitted ahead of this one
 
     @property
     def tts(self):
@@ -73,16 +77,20 @@ class Server:
     def __init__(self, engine, make_case, precision="mixed", budget_bytes=64 << 30,
                  bytes_per_cube=None, max_cases=64, n_inner=10, max_steps=300, tol=1e-8,
                  prep_threads=4, keep_states=False, clock=time.perf_counter, inner_tol=None,
-                 progress=None):
+                 progress=None, max_skips=8, on_step=None):
         self.engine = engine
         self.make_case = make_case
         self.precision = precision
         self.n_inner = int(n_inner)
         self.inner_tol = None if inner_tol is None else float(inner_tol)
         self.progress = progress     # called with (finished, total) as requests finish
+        # Called after every step with (steps so far, {slot: request} of the
+        # resident requests), on the device loop's thread.
+        self.on_step = on_step
         self.max_steps = int(max_steps)
         self.tol = float(tol)
         self.max_cases = int(max_cases)
+        self.max_skips = int(max_skips)
         self.keep_states = keep_states
         self.clock = clock
         self.prep_threads = int(prep_threads)
@@ -122,6 +130,45 @@ class Server:
             self._cv.notify()
 
     # -- device loop ---------------------------------------------------------
+    def _admit(self, pending, resident, t0):
+        """Admits pending requests (in arrival order) while a slot and a
+        range are free; returns the number of requests finished here (too
+        large for the pool, or refused by the engine)."""
+        eng = self.engine
+        finished = 0
+        for r in list(pending):
+            if r.cubes > self.cap_cubes:
+                pending.remove(r)
+                r.error = "case larger than the pool"
+                r.done = self.clock() - t0
+                finished += 1
+                continue
+            info = eng.pool_info()
+            if info["active"] >= self.max_cases:
+                break
+            if info["free_range"] < r.cubes:
+                # Does not fit now: later requests may pass it, a bounded
+                # number of times.
+                if r.skipped >= self.max_skips:
+                    break
+                continue
+            try:
+                r.slot = eng.add_case(r.case)
+            except Exception as exc:   # reported per request, the loop goes on
+                pending.remove(r)
+                r.error = f"admission: {exc}"
+                r.done = self.clock() - t0
+                finished += 1
+                continue
+            r.E = np.zeros(5)
+            r.admit = self.clock() - t0
+            pending.remove(r)
+            resident[r.slot] = r
+            for q in pending:
+                if q.arrival < r.arrival or (q.arrival == r.arrival and q.id < r.id):
+                    q.skipped += 1
+        return finished
+
     def run(self):
         eng = self.engine
         eng.alloc_pool(self.cap_cubes, self.max_cases, self.precision)
@@ -150,21 +197,10 @@ class Server:
                     r.done = self.clock() - t0
                     n_done += 1
                 # Admission in arrival order while a slot and a range are free.
-                while pending:
-                    r = pending[0]
-                    if r.cubes > self.cap_cubes:
-                        pending.pop(0)
-                        r.error = "case larger than the pool"
-                        r.done = self.clock() - t0
-                        n_done += 1
-                        continue
-                    info = eng.pool_info()
-                    if info["active"] >= self.max_cases or info["free_range"] < r.cubes:
-                        break
-                    r.slot = eng.add_case(r.case)
-                    r.E = np.zeros(5)
-                    r.admit = self.clock() - t0
-                    resident[r.slot] = pending.pop(0)
+                # A request that does not fit the largest free range is passed
+                # over by later ones that do, at most max_skips times; after
+                # that nothing behind it is admitted until it fits.
+                n_done += self._admit(pending, resident, t0)
                 if not resident:
                     continue
                 max_resident = max(max_resident, len(resident))
@@ -190,9 +226,17 @@ class Server:
                     r.inner_iters += n_it[slot]
                     r.er = np.array(er[n_it[slot] - 1, slot, :])
                     r.residual = residual(r.er, r.scales)
-                    r.converged = r.residual < self.tol
+                    # A case whose norms or force sums of any inner iteration
+                    # of the step are not finite has diverged (a non-positive
+                    # density turns the square roots of the fluxes into NaN
+                    # within one stage): it is retired as failed.  Cases share
+                    # no data, so the others are unaffected.
+                    diverged = not np.all(np.isfinite(er[:n_it[slot], slot, :]))
+                    if diverged:
+                        r.error = f"diverged: non-finite norms in step {r.steps}"
+                    r.converged = not diverged and r.residual < self.tol
                     limit = r.max_steps if r.max_steps is not None else self.max_steps
-                    if r.converged or r.steps >= limit:
+                    if diverged or r.converged or r.steps >= limit:
                         if self.keep_states:
                             r.state = {n: eng.case_state(slot, n) for n in ("U1_", "U1", "U1q")}
                         eng.remove_case(slot)
@@ -202,6 +246,8 @@ class Server:
                         n_done += 1
                         if self.progress:
                             self.progress(n_done, n_total)
+                if self.on_step:
+                    self.on_step(steps, resident)
         finally:
             pool.shutdown(wait=True)
             for f in futures:
@@ -217,6 +263,7 @@ class Server:
         return {
             "requests": n_total,
             "failed": sum(1 for r in self.requests if r.error),
+            "diverged": sum(1 for r in self.requests if r.error and r.error.startswith("diverged")),
             "wall_s": wall,
             "device_busy_s": busy,
             "steps": steps,

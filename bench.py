@@ -300,7 +300,11 @@ def main():
             "value": round(mix["value"], 3), "unit": "Mcell-iter/s",
             "speedup_vs_fp64": round(mix["value"] / ref["value"], 3),
             "density_rel_l2": l2, "cd_rel_err": cd,
-            "cd_fp64": float(ref["er"][0][5]), "cd_mixed": float(mix["er"][0][5]),
+            "cd_fp64": B.drag_coefficient(ref["er"][0][5], p["sphere_diameter"]),
+            "cd_mixed": B.drag_coefficient(mix["er"][0][5], p["sphere_diameter"]),
+            "er6_fp64": float(ref["er"][0][5]), "er6_mixed": float(mix["er"][0][5]),
+            "cd_definition": "Cd = -er[6] * 2.602 / (pi D^2 / 4), er[6] the normalised x-force "
+                             "of the last inner iteration",
             "steps_compared": args.warmup + args.steps,
             "roofline": roofline("mixed", cells, mix["flux_ms"], args.config),
             "kernels_ms": {"flux_update": round(mix["flux_ms"], 4),
@@ -340,6 +344,8 @@ def main():
             "cd_rel_err_max": max(e[1] for e in errs),
             "density_rel_l2_per_case": [float(f"{e[0]:.3e}") for e in errs],
             "cd_rel_err_per_case": [float(f"{e[1]:.3e}") for e in errs],
+            "cd_fp64_per_case": [round(B.drag_coefficient(e[5], q["sphere_diameter"]), 6)
+                                 for e, q in zip(b64["er"], params)],
             "roofline": roofline("mixed", bmx["cells"], bmx["flux_ms"], "batch"),
         }
         out["forms"] = f

@@ -71,9 +71,12 @@ void BCM_Interface_table(int myid, int ncube, int n_wallcube, int* nadjX_eq, int
                          int* adjZ_bs_plus, int* adjZ_sb_plus, int* adjZ_bs_minus,
                          int* adjZ_sb_minus);
 /* Replaces BCM_Interface.cpp:27-5735 (main.h:224-354) on one device: the
- * equal-size halo copies and the edge-ghost averaging.  The MPI arguments
- * must describe a single rank (all neighbour-rank counts zero); cube pairs of
- * different size (any bs/sb count non-zero) are rejected. */
+ * equal-size halo copies, the two-level faces of the bs lists (injection into
+ * the small cubes' ghost layers, 8-cell restriction into the big cube's;
+ * :1790-2600 for x, y and z alike) and the edge-ghost averaging.  The MPI
+ * arguments must describe a single rank (neighbour-rank counts ncpu_* zero,
+ * else the call aborts); the sb lists serve only the reference's exchanges
+ * with other ranks and are ignored. */
 void BCM_Interface(int myid, int ncube, int mPI_Nadj, int ncpu_bs, int ncpu_eq, int ncpu_sb,
                    int max_nei_bs, int max_nei_eq, int max_nei_sb, int nadjX_eq, int nadjY_eq,
                    int nadjZ_eq, int nadjX_bs_plus, int nadjX_sb_plus, int nadjX_bs_minus,

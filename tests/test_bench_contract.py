@@ -41,6 +41,21 @@ def test_profile_figures_parse():
     assert r["frac"] == pytest.approx(r["achieved"] / 8000.0, rel=1e-3)
 
 
+def test_drag_coefficient_rescales_the_reference_force():
+    import math
+
+    import bcm_cgli_amd as B
+    # er[6] = Fx / (0.5 rho0 U0^2 * 2.602) (the reference's normalisation);
+    # Cd = Fx / (0.5 rho0 U0^2 * pi D^2 / 4), with the sign the reference
+    # prints (-er[6]).
+    D = 0.25
+    area = math.pi * D * D / 4
+    assert B.drag_coefficient(-area / 2.602, D) == pytest.approx(1.0, rel=1e-15)
+    assert B.drag_coefficient(0.0318, D) == pytest.approx(-0.0318 * 2.602 / area, rel=1e-15)
+    assert B.drag_coefficient(-0.0318, 2 * D) == pytest.approx(
+        B.drag_coefficient(-0.0318, D) / 4, rel=1e-15)
+
+
 @pytest.mark.gpu
 def test_bench_line_on_a_small_case():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--config", "cfg1", "--steps",

@@ -80,6 +80,48 @@ def test_served_serving_distribution_matches_alone(B):
         assert np.array_equal(r.er, er1[:7]), r.id
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_diverged_request_is_quarantined(B, precision):
+    # After its second step, one resident request's iterate gets a NaN
+    # density in one fluid cell (uploaded from the host).  Its norms turn NaN
+    # in the next step: it is reported failed and its slot freed, while the
+    # requests sharing the launches finish bitwise as they do alone.
+    reqs = poisson_requests(4, rate=40.0, seed=21, cubes=(27, 64), ico_levels=(3, 3))
+    steps = [5, 6, 4, 5]
+    target = 1
+    e = B.Engine(0)
+
+    def poison(n_steps, resident):
+        for slot, r in resident.items():
+            if r.id == target and r.steps == 2:
+                u = e.case_state(slot, "U1_")
+                fluid = np.argwhere(r.case.array("FWS")[1:, 2:18, 2:18, 2:18] == 1)[0]
+                c, i, j, k = fluid + np.array([1, 2, 2, 2])
+                u[c, i, j, k, 0] = np.nan
+                e.set_case_state(slot, "U1_", u)
+
+    srv = Server(e, B.Case, precision=precision, budget_bytes=200 * B.bytes_per_cube(precision),
+                 max_cases=3, n_inner=3, max_steps=100, tol=1e-30, keep_states=True,
+                 on_step=poison)
+    for (t, p), n in zip(reqs, steps):
+        srv.submit(p, 0.0, max_steps=n)
+    st = srv.run()
+    e.close()
+    rs = srv.requests
+    bad = rs[target]
+    assert bad.error is not None and bad.error.startswith("diverged"), bad.error
+    assert bad.steps == 3 and not bad.converged
+    assert st["failed"] == 1 and st["diverged"] == 1
+    for r, n in zip(rs, steps):
+        if r.id == target:
+            continue
+        assert r.error is None and r.steps == n
+        st1, er1 = run_alone(B, r.params, n, 3, precision)
+        for name in ("U1_", "U1", "U1q"):
+            assert np.array_equal(r.state[name], st1[name]), (r.id, name)
+        assert np.array_equal(r.er, er1[:7]), r.id
+
+
 def run_alone_converge(B, params, n_steps, n_inner, inner_tol, precision):
     case = B.Case(params)
     e = B.Engine(0)

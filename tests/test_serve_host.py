@@ -17,6 +17,8 @@ class FakeCase:
     def __init__(self, params):
         self.ncube = params["cubes"] + 1
         self.decay = params.get("decay", 0.5)
+        self.nan_after = params.get("nan_after")   # steps after which the norms turn NaN
+        self.steps = 0
         self.phys = {"P0": 1.0, "U0": 1.0, "rho0": 1.0}
         if params.get("fail"):
             raise RuntimeError("bad case")
@@ -72,6 +74,9 @@ class FakeEngine:
         for s, v in self.slots.items():
             v[1] *= v[0].decay
             er[:, s, :5] = v[1]
+            v[0].steps += 1
+            if v[0].nan_after is not None and v[0].steps > v[0].nan_after:
+                er[-1, s, 2] = np.nan
         used = sum(1 for o in self.owner if o >= 0)
         assert used <= self.cap
         return er
@@ -167,6 +172,44 @@ def test_slot_limit():
         srv.submit({"cubes": 1}, 0.0)
     srv.run()
     assert max(n for _, n, _ in srv.trace) == 2
+
+
+def test_diverged_case_is_quarantined():
+    # The second request's norms turn NaN in its third step: it is retired
+    # at once as failed, its slot and cubes go to the waiting request, and
+    # the others run to their limits.
+    eng = FakeEngine()
+    srv = make_server(eng, 60, max_cases=2, n_inner=2, max_steps=6, tol=1e-30)
+    srv.submit({"cubes": 20}, 0.0)
+    srv.submit({"cubes": 20, "nan_after": 2}, 0.0)
+    srv.submit({"cubes": 20}, 0.0)
+    st = srv.run()
+    a, b, c = srv.requests
+    assert b.error.startswith("diverged") and b.steps == 3 and not b.converged
+    assert a.error is None and a.steps == 6 and c.error is None and c.steps == 6
+    assert c.admit >= b.done and c.admit < a.done
+    assert st["failed"] == 1 and st["diverged"] == 1
+    assert ("remove", b.slot) in eng.log
+
+
+def test_admission_passes_a_request_that_does_not_fit_a_bounded_number_of_times():
+    def run(max_skips):
+        eng = FakeEngine()
+        srv = make_server(eng, 100, max_cases=8, n_inner=1, max_steps=4, tol=1e-30,
+                          max_skips=max_skips)
+        srv.submit({"cubes": 50}, 0.0)      # A
+        srv.submit({"cubes": 80}, 0.0)      # B: waits for A's (and C's) cubes
+        srv.submit({"cubes": 20}, 0.0)      # C
+        srv.submit({"cubes": 20}, 0.0)      # D
+        srv.run()
+        return srv.requests
+    a, b, c, d = run(1)
+    assert c.admit < b.admit and c.admit < a.done     # C passes B once
+    assert d.admit >= b.admit                          # D may not: B's bound is reached
+    assert b.skipped == 1 and b.steps == 4
+    a, b, c, d = run(8)
+    assert c.admit < b.admit and d.admit < b.admit     # both pass B
+    assert b.skipped == 2
 
 
 def alone_inner_control(decay, n_inner, inner_tol, tol, max_steps):
