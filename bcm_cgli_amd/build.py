@@ -34,8 +34,8 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-SOURCES = ["bcm_kernels.hip", "bcm_kernels_mixed.hip", "bcm_ib.hip", "bcm_engine.cpp",
-           "bcm_output.cpp", "bcm_capi.cpp", "bcm_setup.cpp", "bcm_io.cpp"]
+SOURCES = ["bcm_kernels.hip", "bcm_kernels_mixed.hip", "bcm_ib.hip", "bcm_noalloc.hip",
+           "bcm_engine.cpp", "bcm_output.cpp", "bcm_capi.cpp", "bcm_setup.cpp", "bcm_io.cpp"]
 # Sources that include the HIP runtime; the others are plain host C++.
 HIP_HOST = ("bcm_engine.cpp", "bcm_output.cpp", "bcm_capi.cpp", "bcm_setup.cpp")
 

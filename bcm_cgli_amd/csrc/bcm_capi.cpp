@@ -32,6 +32,8 @@ int fail(const std::string& m) {
 }
 }  // namespace
 
+void bcm::set_last_error(const std::string& m) { g_err = m; }
+
 struct bcm_case {
   Case cs;
   std::vector<int> ib_minus_gc, ib_minus_ip, ib_plus_gc, ib_plus_ip;

@@ -15,6 +15,9 @@
 
 namespace bcm {
 
+// Sets the message bcm_last_error() returns on the calling thread.
+void set_last_error(const std::string& m);
+
 struct GhostSchedule {
   std::vector<GhostEntry> entries;   // grouped per cube, level-major within the cube
   std::vector<int> cube_level_off;   // per ghost cube: first level index (+1 sentinel)

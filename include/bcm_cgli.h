@@ -327,6 +327,33 @@ int bcm_engine_save_restart(bcm_engine* e, int slot, const char* path, long long
 int bcm_engine_load_restart(bcm_engine* e, int slot, const char* path, long long* step,
                             double* E5);
 
+/* ------------------------------------------------------------------------ */
+/* 6. Flux/RK stage that allocates nothing.                                   */
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the work of BCM_Flux_XYZ_Viscous_Runge_kutta (same
+ * kernels, bitwise the same results) on arrays the caller owns on the
+ * current device, in the library's device layout: state arrays [ncube][5][20]
+ * [20][20] doubles (variable-major per cube, k fastest), flags FWS [ncube][20]
+ * [20][20] int8, cube_size [ncube]; cube 0 unused.  U1_ is updated in place
+ * and U1p1 (RK 1), U1p2 (RK 2), Rku1 and Residual1 written; at RK 3, er[0..7]
+ * (device, 8 doubles) receives 0 and the norms and force sums er[1..7].
+ * phys7 (host) holds rho0, U0, Uin, V0, W0, P0, mu_L as for bcm_bind, and
+ * n_cubes_norm the norm scaling's cube count (MPI_Ncube).  Work is queued on
+ * `stream` (a hipStream_t; NULL for the default stream) and not waited for.
+ * The norm partials and case tables live in a fixed device array of
+ * BCM_NOALLOC_SCRATCH_BYTES per device, held until the process ends, so at
+ * most BCM_NOALLOC_MAX_CUBES cubes (ncube - 1) per call; larger calls fail.
+ * Calls that overlap on one device share that array and race. */
+#define BCM_NOALLOC_MAX_CUBES 16384
+#define BCM_NOALLOC_SCRATCH_BYTES 1114624
+int bcm_flux_rk_noalloc(int ncube, int n_cubes_norm, int RK, double deltaT, double deltaTau,
+                        double e, const signed char* FWS, const double* cube_size, double* U1_,
+                        const double* U1, const double* U1q, double* U1p1, double* U1p2,
+                        double* Rku1, double* Residual1, double* er, const double* phys7,
+                        void* stream);
+
 #ifdef __cplusplus
 }
 #endif
