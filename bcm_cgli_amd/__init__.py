@@ -38,6 +38,8 @@ class CaseParamsC(ctypes.Structure):
         ("noise_amp", ctypes.c_double),
         ("refine_lo", ctypes.c_int * 3),
         ("refine_hi", ctypes.c_int * 3),
+        ("refine2_lo", ctypes.c_int * 3),
+        ("refine2_hi", ctypes.c_int * 3),
     ]
 
 
@@ -77,6 +79,7 @@ def load_library(path=LIB_PATH):
     lib.bcm_engine_sync.argtypes = [c_void_p]
     lib.bcm_engine_time_part.argtypes = [c_void_p, c_int, c_int, c_int, P(c_double)]
     lib.bcm_engine_ghost_stats.argtypes = [c_void_p, c_void_p]
+    lib.bcm_engine_set_ghost_lds_limit.argtypes = [c_void_p, c_int]
     lib.bcm_engine_set_flux.argtypes = [c_void_p, c_int, c_int, c_int]
     lib.bcm_engine_autotune.argtypes = [c_void_p, c_void_p]
     lib.bcm_bytes_per_cube.restype = ctypes.c_longlong
@@ -136,6 +139,11 @@ class Case:
             for d in range(3):
                 p.refine_lo[d] = int(refine[0][d])
                 p.refine_hi[d] = int(refine[1][d])
+        refine2 = params.get("refine2")
+        if refine2 is not None:
+            for d in range(3):
+                p.refine2_lo[d] = int(refine2[0][d])
+                p.refine2_hi[d] = int(refine2[1][d])
         h = ctypes.c_void_p()
         if _from_dir is None and device is None:
             _check(lib.bcm_case_build(ctypes.byref(p), ctypes.byref(h)))
@@ -420,14 +428,22 @@ class Engine:
         return ms.value
 
     def set_flux(self, variant, nseg=1, fuse=1):
-        """fuse: 0 separate update, 1 update fused in place, 2 update and face
-        halos written to a second copy of the iterate (engine stages)."""
+        """Pins the flux kernel choice (no autotuning).  FP64: fuse 0
+        separate update, 1 update fused in place, 2 update and face halos
+        written to a second copy of the iterate (engine stages).  Mixed: only
+        nseg counts, the runs per cube (1, 2 or 4) of its one kernel."""
         _check(_lib.bcm_engine_set_flux(self._h, int(variant), int(nseg), int(fuse)))
 
     def autotune(self):
         out = (ctypes.c_int * 3)()
         _check(_lib.bcm_engine_autotune(self._h, out))
         return {"variant": out[0], "nseg": out[1], "fuse": int(out[2])}
+
+    def set_ghost_lds_limit(self, slots):
+        """Cubes whose IB fill needs more LDS value slots than this take the
+        in-place schedule on memory (cases loaded or added afterwards); the
+        default 1536 is the kernel's LDS capacity.  A test hook."""
+        _check(_lib.bcm_engine_set_ghost_lds_limit(self._h, int(slots)))
 
     def ghost_stats(self):
         s = (ctypes.c_int * 3)()

@@ -28,11 +28,17 @@ CONFIGS = {
                     refine=((0, 0, 0), (1, 1, 1))),
     "ml": dict(_BASE, lattice=(4, 4, 4), cube_size=0.25, sphere_center=(0.5, 0.5, 0.5),
                ico_level=4, mach=0.2, reynolds=100.0, refine=((1, 1, 1), (3, 3, 3))),
+    # Three-level grid: the ml lattice with the central 2x2x2 of its half-size
+    # cubes (doubled-lattice indices 3..4) split again (176 cubes of edges 0.25,
+    # 0.125 and 0.0625), a sphere of diameter 0.15 inside the finest block.
+    "ml3": dict(_BASE, lattice=(4, 4, 4), cube_size=0.25, sphere_center=(0.5, 0.5, 0.5),
+                sphere_diameter=0.15, ico_level=4, mach=0.2, reynolds=100.0,
+                refine=((1, 1, 1), (3, 3, 3)), refine2=((3, 3, 3), (5, 5, 5))),
 }
 
 # Stepping of each case: (physical steps, inner iterations per step).
 STEPPING = {"tiny": (2, 3), "cfg1": (20, 10), "cfg2": (200, 10), "batch": (50, 10),
-            "ml_tiny": (2, 3), "ml": (4, 5)}
+            "ml_tiny": (2, 3), "ml": (4, 5), "ml3": (3, 4)}
 
 # Runs under the reference's inner-iteration control (main.cpp:1249-1320):
 # physical steps, the iteration cap (iteration_end_step) and the tolerance on

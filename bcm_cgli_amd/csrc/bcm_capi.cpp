@@ -182,6 +182,8 @@ CaseParams to_params(const bcm_case_params* p) {
   for (int d = 0; d < 3; ++d) {
     cp.refine_lo[d] = p->refine_lo[d];
     cp.refine_hi[d] = p->refine_hi[d];
+    cp.refine2_lo[d] = p->refine2_lo[d];
+    cp.refine2_hi[d] = p->refine2_hi[d];
   }
   return cp;
 }
@@ -399,6 +401,7 @@ int bcm_engine_alloc_pool(bcm_engine* h, int cap_cubes, int max_cases, int preci
   std::string err;
   if (engine_alloc(h->e, cap_cubes + 1, max_cases, err, precision)) return fail(err);
   h->e->flux_tuned = false;
+  h->e->flux_pinned = false;
   return 0;
 }
 
@@ -596,6 +599,10 @@ int bcm_engine_time_part(bcm_engine* h, int part, int RK, int reps, double* ms) 
   if (reps < 1) return fail("reps must be positive");
   if (e->prec == 1 && (part == 3 || part == 4))
     return fail("the mixed path fuses flux and update (use part 7)");
+  // The pushing stage kernel is timed as the inner iterations of a step run
+  // it, without the stores of Rku1, U1p1, U1p2 and Residual1 that only a
+  // step's last inner iteration makes.
+  e->push_aux = part != 9;
   if (hipEventRecord(e->ev0, e->stream) != hipSuccess) return fail("event record failed");
   for (int r = 0; r < reps; ++r) {
     switch (part) {
@@ -609,9 +616,10 @@ int bcm_engine_time_part(bcm_engine* h, int part, int RK, int reps, double* ms) 
       case 7: engine_flux_update(e, RK, 0); break;
       case 8: engine_halo_refresh(e); break;
       case 9: engine_flux_push(e, RK, 0); break;
-      default: return fail("unknown part");
+      default: e->push_aux = true; return fail("unknown part");
     }
   }
+  e->push_aux = true;
   if (hipEventRecord(e->ev1, e->stream) != hipSuccess) return fail("event record failed");
   if (hipEventSynchronize(e->ev1) != hipSuccess) return fail("event wait failed");
   float t = 0;
@@ -624,7 +632,15 @@ int bcm_engine_time_part(bcm_engine* h, int part, int RK, int reps, double* ms) 
 
 int bcm_engine_set_flux(bcm_engine* h, int variant, int nseg, int fuse) {
   if (!h) return fail("null engine");
-  if (h->e->prec == 1) return fail("the mixed path has a single flux kernel");
+  if (h->e->prec == 1) {
+    // One kernel; the choice is its runs per cube (variant and fuse ignored).
+    if (nseg != 1 && nseg != 2 && nseg != 4) return fail("mixed path: nseg must be 1, 2 or 4");
+    h->e->mixed_nseg = nseg;
+    h->e->flux_tuned = true;
+    h->e->flux_pinned = true;
+    h->e->graph_valid = false;
+    return 0;
+  }
   if (variant != 0 && variant != 1) return fail("flux variant must be 0 or 1");
   if (nseg != 1 && nseg != 2 && nseg != 4 && nseg != 8 && nseg != 16)
     return fail("nseg must divide 16");
@@ -635,6 +651,7 @@ int bcm_engine_set_flux(bcm_engine* h, int variant, int nseg, int fuse) {
   h->e->flux_variant = variant;
   h->e->flux_nseg = nseg;
   h->e->flux_tuned = true;
+  h->e->flux_pinned = true;
   h->e->graph_valid = false;
   return 0;
 }
@@ -643,6 +660,7 @@ int bcm_engine_autotune(bcm_engine* h, int* out3) {
   if (!h) return fail("null engine");
   std::string err;
   int v = 0, n = 1;
+  h->e->flux_pinned = false;
   if (engine_autotune(h->e, &v, &n, err)) return fail(err);
   if (out3) {
     out3[0] = v;
@@ -657,6 +675,14 @@ int bcm_engine_ghost_stats(bcm_engine* h, int* stats3) {
   stats3[0] = h->e->n_ghost_entries;
   stats3[1] = h->e->n_ghost_cubes;
   stats3[2] = h->e->ghost_levels;
+  return 0;
+}
+
+int bcm_engine_set_ghost_lds_limit(bcm_engine* h, int slots) {
+  if (!h) return fail("null engine");
+  if (slots < 0 || slots > GHOST_LDS_SLOTS)
+    return fail("slot limit must be in 0.." + std::to_string(GHOST_LDS_SLOTS));
+  h->e->ghost_slot_limit = slots;
   return 0;
 }
 

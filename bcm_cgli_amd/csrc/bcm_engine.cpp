@@ -107,7 +107,7 @@ int ensure_er(Engine* e, int slots, std::string& err);
 int rebuild_tables(Engine* e, std::string& err);
 
 // ---------------------------------------------------------------------------
-GhostSchedule build_ghost_schedule(const std::vector<GhostEntry>& ordered) {
+GhostSchedule build_ghost_schedule(const std::vector<GhostEntry>& ordered, int slot_limit) {
   GhostSchedule gs;
   // Group by cube, keeping reference order inside each cube.
   std::map<int, std::vector<int>> by_cube;
@@ -147,7 +147,7 @@ GhostSchedule build_ghost_schedule(const std::vector<GhostEntry>& ordered) {
     std::vector<int> level(ne);
     std::vector<int16_t> fin(nt);
     int nlev = 0;
-    if (nt + ne <= GHOST_LDS_SLOTS) {
+    if (nt + ne <= std::min(slot_limit, GHOST_LDS_SLOTS)) {
       // Renamed: entry a writes slot nt + a and reads the slot holding the
       // value the sequential fill would read; only reads of earlier writes
       // order the levels.
@@ -340,7 +340,7 @@ int engine_alloc(Engine* e, int cap, int max_cases, std::string& err, int prec) 
   return 0;
 }
 
-int build_case_tables(const Case& cs, CaseTables& tb, std::string& err) {
+int build_case_tables(const Case& cs, CaseTables& tb, std::string& err, int ghost_slot_limit) {
   tb = CaseTables();
   tb.n = cs.ncube - 1;
   tb.mpi_ncube = cs.ncube - 1;
@@ -388,7 +388,7 @@ int build_case_tables(const Case& cs, CaseTables& tb, std::string& err) {
         for (int m = 0; m < 8; ++m) E.w[m] = t->weight[(n - 1) * 8 + 1 + m];
         ordered.push_back(E);
       }
-    tb.ghost = build_ghost_schedule(ordered);
+    tb.ghost = build_ghost_schedule(ordered, ghost_slot_limit);
   }
   // Face halo sources from the equal-size interface lists.
   tb.halo.resize(cs.ncube);
@@ -655,7 +655,7 @@ int engine_add_case(Engine* e, const Case& cs, int* slot_out, std::string& err) 
     return 1;
   }
   CaseTables tb;
-  if (build_case_tables(cs, tb, err)) return 1;
+  if (build_case_tables(cs, tb, err, e->ghost_slot_limit)) return 1;
   const int first = find_range(e, tb.n);
   if (first == 0) {
     err = "no free range of " + std::to_string(tb.n) + " cubes in the pool";
@@ -744,6 +744,7 @@ int engine_bind_single(Engine* e, int mpi_ncube, const DevPhys& ph, std::string&
 int engine_load_case(Engine* e, const Case& cs, std::string& err) {
   if (engine_alloc(e, cs.ncube, 1, err)) return 1;
   e->flux_tuned = false;
+  e->flux_pinned = false;
   int slot;
   if (engine_add_case(e, cs, &slot, err)) return 1;
   e->halo_full_next = false;
@@ -927,11 +928,11 @@ void engine_flux_update(Engine* e, int RK, int er_slot) {
   if (e->prec == 1) {
     launch_flux_mixed(e->stream, e->Uf, e->Uf2, e->halo, e->bc_mask_cube, e->U1, e->U1q, e->fws,
                       e->cube_size, e->ncube, e->pt(), e->deltaT, e->deltaTau, e->e, RK,
-                      e->partials);
+                      e->partials, e->mixed_nseg);
     std::swap(e->Uf, e->Uf2);
     if (RK == 3)
-      launch_reduce(e->stream, e->partials, 1, e->case_desc, e->n_active, e->phys_dev,
-                    er_row(e, er_slot));
+      launch_reduce(e->stream, e->partials, MIXED_ROWS_PER_CUBE, e->case_desc, e->n_active,
+                    e->phys_dev, er_row(e, er_slot));
     return;
   }
   if (!engine_fused(e)) {
@@ -947,6 +948,30 @@ void engine_flux_update(Engine* e, int RK, int er_slot) {
                   er_row(e, er_slot));
 }
 
+namespace {
+// Size class of the resident pool: floor(log2(resident cubes)).
+int size_class(const Engine* e) {
+  long n = 0;
+  for (const CaseSlot& S : e->cases)
+    if (S.active) n += S.tb.n;
+  int c = 0;
+  while (n > 1) {
+    n >>= 1;
+    ++c;
+  }
+  return c;
+}
+}  // namespace
+
+// Tunes the flux kernel choice when it has not been chosen yet, or (mixed
+// path, choice not pinned) when the pool's size class changed since.
+int engine_maybe_tune(Engine* e, std::string& err) {
+  if (e->flux_tuned && (e->prec == 0 || e->flux_pinned || size_class(e) == e->tuned_class))
+    return 0;
+  int v, n;
+  return engine_autotune(e, &v, &n, err);
+}
+
 int engine_autotune(Engine* e, int* variant, int* nseg, std::string& err) {
   HIP_TRY(hipSetDevice(e->device));
   if (e->n_active == 0) {
@@ -954,25 +979,79 @@ int engine_autotune(Engine* e, int* variant, int* nseg, std::string& err) {
     return 1;
   }
   if (e->prec == 1) {
-    // The mixed path has one kernel shape (one workgroup per cube, update fused).
+    // The mixed flux kernel in 1, 2 or 4 runs per cube (bitwise the same
+    // results), each timed as one whole RK stage on the resident cases, once
+    // per size class of the pool; the iterate the stages write is restored.
+    const int cls = size_class(e);
+    auto known = e->nseg_by_class.find(cls);
+    if (known == e->nseg_by_class.end()) {
+      const size_t bytes = (size_t)e->ncube * CUBE_STRIDE * sizeof(float);
+      struct Saved {
+        float* p[2] = {nullptr, nullptr};
+        ~Saved() {
+          for (float* q : p)
+            if (q) (void)hipFree(q);
+        }
+      } saved;
+      float* const live[2] = {e->Uf, e->Uf2};
+      for (int a = 0; a < 2; ++a) {
+        HIP_TRY(hipMalloc((void**)&saved.p[a], bytes));
+        HIP_TRY(hipMemcpyAsync(saved.p[a], live[a], bytes, hipMemcpyDeviceToDevice, e->stream));
+      }
+      const bool full = e->halo_full_next;
+      float best = 1e30f;
+      int bn = 1;
+      for (int cand : {1, 2, 4}) {
+        e->mixed_nseg = cand;
+        engine_stage(e, 2, 0, full);   // warm up
+        HIP_TRY(hipEventRecord(e->ev0, e->stream));
+        for (int r = 0; r < 3; ++r) engine_stage(e, 2, 0, full);
+        HIP_TRY(hipEventRecord(e->ev1, e->stream));
+        HIP_TRY(hipEventSynchronize(e->ev1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+        e->Uf = live[0];
+        e->Uf2 = live[1];
+        for (int a = 0; a < 2; ++a)
+          HIP_TRY(hipMemcpyAsync(live[a], saved.p[a], bytes, hipMemcpyDeviceToDevice, e->stream));
+        if (ms < best) {
+          best = ms;
+          bn = cand;
+        }
+      }
+      HIP_TRY(hipStreamSynchronize(e->stream));
+      known = e->nseg_by_class.emplace(cls, bn).first;
+    }
+    if (e->mixed_nseg != known->second) e->graph_valid = false;
+    e->mixed_nseg = known->second;
+    e->tuned_class = cls;
     e->flux_tuned = true;
     if (variant) *variant = 2;
-    if (nseg) *nseg = 1;
+    if (nseg) *nseg = e->mixed_nseg;
     return 0;
   }
-  // Candidates (variant, nseg, fuse): the plane kernel, the marching kernel
-  // with 1..8 segments, the marching kernel with the RK update fused in place,
-  // and the marching kernel writing the update and face halos into the other
-  // copy of the iterate (fuse 2).  Each is timed as one whole RK stage on the
-  // resident cases; every array a stage writes is saved first and restored
-  // afterwards.
-  const int cand[7][3] = {{0, 1, 0}, {1, 1, 1}, {1, 1, 2}, {1, 1, 0},
-                          {1, 2, 0}, {1, 4, 0}, {1, 8, 0}};
+  // Candidates (variant, nseg, fuse): the marching kernel with the RK update
+  // fused in place, and the marching kernel writing the update and face halos
+  // into the other copy of the iterate (fuse 2).  Both reduce the norms over
+  // one partial row per cube, so the norms do not depend on which one wins.
+  // (The plane kernel and the segmented marching kernel, selectable with
+  // set_flux, reduce over per-plane rows and are not candidates.)  Each is
+  // timed as one whole RK stage on the resident cases, as the inner
+  // iterations of a step run it; every array a stage writes is saved first
+  // and restored afterwards.
+  const int cand[2][3] = {{1, 1, 1}, {1, 1, 2}};
   double* const live[6] = {e->U, e->U2, e->U1p1, e->U1p2, e->Rku, e->Res};
-  double* saved[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  struct Saved {
+    double* p[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    ~Saved() {
+      for (double* q : p)
+        if (q) (void)hipFree(q);
+    }
+  } saved;
   for (int a = 0; a < 6; ++a) {
-    HIP_TRY(hipMalloc((void**)&saved[a], pool_bytes(e)));
-    HIP_TRY(hipMemcpyAsync(saved[a], live[a], pool_bytes(e), hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(hipMalloc((void**)&saved.p[a], pool_bytes(e)));
+    HIP_TRY(hipMemcpyAsync(saved.p[a], live[a], pool_bytes(e), hipMemcpyDeviceToDevice,
+                           e->stream));
   }
   const bool full = e->halo_full_next;
   float best = 1e30f;
@@ -983,17 +1062,20 @@ int engine_autotune(Engine* e, int* variant, int* nseg, std::string& err) {
     e->flux_nseg = cv[1];
     e->flux_fuse = cv[2] != 0;
     e->flux_push = cv[2] == 2;
+    e->push_aux = false;
     engine_stage(e, 2, 0, full);  // warm up
     HIP_TRY(hipEventRecord(e->ev0, e->stream));
     for (int r = 0; r < 3; ++r) engine_stage(e, 2, 0, full);
     HIP_TRY(hipEventRecord(e->ev1, e->stream));
+    e->push_aux = true;
     HIP_TRY(hipEventSynchronize(e->ev1));
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
     e->U = live[0];
     e->U2 = live[1];
     for (int a = 0; a < 6; ++a)
-      HIP_TRY(hipMemcpyAsync(live[a], saved[a], pool_bytes(e), hipMemcpyDeviceToDevice, e->stream));
+      HIP_TRY(hipMemcpyAsync(live[a], saved.p[a], pool_bytes(e), hipMemcpyDeviceToDevice,
+                             e->stream));
     if (ms < best) {
       best = ms;
       bv = cv[0];
@@ -1003,7 +1085,6 @@ int engine_autotune(Engine* e, int* variant, int* nseg, std::string& err) {
     }
   }
   HIP_TRY(hipStreamSynchronize(e->stream));
-  for (int a = 0; a < 6; ++a) (void)hipFree(saved[a]);
   e->flux_push = bp;
   e->flux_variant = bv;
   e->flux_nseg = bs;
@@ -1037,7 +1118,7 @@ void engine_flux_push(Engine* e, int RK, int er_slot) {
   }
   launch_flux_push(e->stream, e->U, e->U2, e->halo, e->bc_mask_cube, e->U1, e->U1q, e->fws,
                    e->cube_size, e->Rku, e->Res, e->ncube, e->pt(), e->deltaT, e->deltaTau, e->e,
-                   RK, e->U1p1, e->U1p2, e->partials);
+                   RK, e->U1p1, e->U1p2, e->partials, e->push_aux ? 1 : 0);
   std::swap(e->U, e->U2);
   if (RK == 3)
     launch_reduce(e->stream, e->partials, 1, e->case_desc, e->n_active, e->phys_dev,
@@ -1089,8 +1170,14 @@ namespace {
 void enqueue_step(Engine* e, int n_inner, bool full_first) {
   float* startf = e->Uf;
   double* startd = e->U;
-  for (int it = 0; it < n_inner; ++it)
+  // Only the last inner iteration's stages store Rku1, U1p1, U1p2 and
+  // Residual1, which no stage reads back: the step ends with the arrays the
+  // reference leaves.
+  for (int it = 0; it < n_inner; ++it) {
+    e->push_aux = it == n_inner - 1;
     for (int RK = 1; RK <= 3; ++RK) engine_stage(e, RK, it, full_first && it == 0 && RK == 1);
+  }
+  e->push_aux = true;
   engine_shift(e);
   // An odd number of flux kernels that swap copies: bring the iterate back to
   // the copy the step started from, so that every replay of a captured step
@@ -1115,10 +1202,7 @@ int engine_run(Engine* e, int n_steps, int n_inner, double* er_hist, bool use_gr
     return 1;
   }
   if (ensure_er(e, n_inner, err)) return 1;
-  if (!e->flux_tuned) {
-    int v, n;
-    if (engine_autotune(e, &v, &n, err)) return 1;
-  }
+  if (engine_maybe_tune(e, err)) return 1;
   if (use_graph && (!e->graph_valid || e->graph_inner != n_inner)) {
     free_graph(e);
     HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
@@ -1217,10 +1301,7 @@ int engine_run_converge(Engine* e, int n_steps, int max_inner, double tol, doubl
     return 1;
   }
   if (ensure_er(e, 1, err)) return 1;
-  if (!e->flux_tuned) {
-    int v, n;
-    if (engine_autotune(e, &v, &n, err)) return 1;
-  }
+  if (engine_maybe_tune(e, err)) return 1;
   const int mc = e->max_cases;
   const size_t row = (size_t)mc * 8;
   std::vector<int> done(mc);

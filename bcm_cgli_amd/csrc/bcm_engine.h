@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -40,8 +41,11 @@ void ghost_free(GhostDevice& d);
 
 // Builds the level schedule of the sequential in-place IB fill for entries
 // given in reference order (minus list, then plus list): per cube, levels
-// over renamed LDS slots when the cube fits GHOST_LDS_SLOTS, else in place.
-GhostSchedule build_ghost_schedule(const std::vector<GhostEntry>& ordered);
+// over renamed LDS slots when the cube needs at most slot_limit slots
+// (GHOST_LDS_SLOTS normally; lower only to exercise the other schedule),
+// else in place on memory.
+GhostSchedule build_ghost_schedule(const std::vector<GhostEntry>& ordered,
+                                   int slot_limit = GHOST_LDS_SLOTS);
 
 // Device tables of one case in its own cube numbering (cubes 1..n).
 struct CaseTables {
@@ -55,7 +59,8 @@ struct CaseTables {
   std::vector<int2> halo_pairs;    // (source, destination) flat cells, cube * 8000 + cell
   std::vector<MlTaskDev> ml;       // two-level faces (empty on one-size grids)
 };
-int build_case_tables(const Case& cs, CaseTables& tb, std::string& err);
+int build_case_tables(const Case& cs, CaseTables& tb, std::string& err,
+                      int ghost_slot_limit = GHOST_LDS_SLOTS);
 
 struct CaseSlot {
   bool active = false;
@@ -146,7 +151,21 @@ struct Engine {
   // into the other copy of the iterate (engine stages only; the stage pieces
   // and drop-ins keep the in-place kernels).
   bool flux_push = true;
+  // The pushing FP64 kernel stores Rku1, U1p1, U1p2 and Residual1 (which no
+  // stage reads back) only when set: captured steps clear it for all but
+  // their last inner iteration.
+  bool push_aux = true;
+  // Mixed path: runs per cube of the flux kernel (1, 2 or 4); the autotuner
+  // picks it per size class of the resident pool (log2 of its cubes) and
+  // keeps each class's pick.
+  int mixed_nseg = 1;
+  int tuned_class = -1;
+  std::map<int, int> nseg_by_class;
   bool flux_tuned = false;
+  bool flux_pinned = false;    // chosen with set_flux: no re-tuning
+  // Cubes whose IB fill needs more LDS slots than this take the in-place
+  // schedule on memory (GHOST_LDS_SLOTS unless lowered to test that path).
+  int ghost_slot_limit = GHOST_LDS_SLOTS;
 
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   OutputQueue* out = nullptr;   // asynchronous output (created on first use)
@@ -219,6 +238,9 @@ int engine_sync(Engine* e, std::string& err);
 // restored afterwards) and keeps the fastest.  Returns the chosen
 // (variant, nseg) through the pointers.
 int engine_autotune(Engine* e, int* variant, int* nseg, std::string& err);
+// Tunes only when no choice was made yet, or (mixed path, choice not pinned
+// with set_flux) when the pool's size class has changed since the last tuning.
+int engine_maybe_tune(Engine* e, std::string& err);
 
 // Asynchronous PLOT3D output (bcm_output.cpp): queues the current inner
 // iterate U1_ of a case for writing as BCMdata00001.x and qBCMdata<step>.q in

@@ -230,14 +230,15 @@ void launch_flux_march(hipStream_t s, double* U, const double* U1, const double*
 // Marching kernel with the update written to Uout (one workgroup per cube):
 // every interior cell, and the two layers next to each face with an
 // equal-size neighbour (halo[c]) into that neighbour's ghost layers in Uout.
-// Writes Rku, Res, U1p1 (RK1), U1p2 (RK2) and, at RK 3, one row of norm
-// partials per cube, bitwise as the in-place fused kernel does.
+// With aux != 0 it writes Rku, Res, U1p1 (RK1) and U1p2 (RK2) bitwise as the
+// in-place fused kernel does (nothing reads them back during stages); at RK 3
+// one row of norm partials per cube either way.
 // Ghost regions the domain BCs wrote (cube_mask) are copied from Uin to Uout.
 void launch_flux_push(hipStream_t s, const double* Uin, double* Uout, const HaloSrc* halo,
                       const uint8_t* cube_mask, const double* U1, const double* U1q,
                       const int8_t* fws, const double* cube_size, double* Rku, double* Res,
                       int ncube, PhysTab pt, double deltaT, double deltaTau, double e, int RK,
-                      double* U1p1, double* U1p2, double* partials);
+                      double* U1p1, double* U1p2, double* partials, int aux);
 void launch_copy_d(hipStream_t s, const double* src, double* dst, size_t n);
 void launch_update(hipStream_t s, int RK, double* U, double* U1p1, double* U1p2,
                    const double* Rku, const double* Res, const int8_t* fws, int ncube,
@@ -262,12 +263,15 @@ int update_blocks(int ncube);
 // (updated fluid cells, other cells unchanged).  Cells of the two layers
 // next to a face with an equal-size neighbour (halo[c]) are also written into
 // that neighbour's ghost layers in Uout: the face part of the post-update
-// exchange.  At RK 3 one row of 8 norm partials per cube.
+// exchange.  nseg (1, 2 or 4) runs of 16/nseg planes per cube, one workgroup
+// each; results are bitwise the same for every nseg.  At RK 3,
+// MIXED_ROWS_PER_CUBE rows of 8 norm partials per cube.
+constexpr int MIXED_ROWS_PER_CUBE = 16;
 void launch_flux_mixed(hipStream_t s, const float* Uin, float* Uout, const HaloSrc* halo,
                        const uint8_t* cube_mask, const double* U1, const double* U1q,
                        const int8_t* fws,
                        const double* cube_size, int ncube, PhysTab pt, double deltaT,
-                       double deltaTau, double e, int RK, double* partials);
+                       double deltaTau, double e, int RK, double* partials, int nseg);
 // dst[0..n) <- src[0..n).
 void launch_copy_f(hipStream_t s, const float* src, float* dst, size_t n);
 // U1q <- U1, U1 <- ref_state + U on interior cells.

@@ -490,7 +490,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
     double* __restrict__ Rku, double* __restrict__ Res, PhysTab pt, double deltaT,
     double deltaTau, double e, int nseg, int RK, int fuse, double* __restrict__ U1p1,
     double* __restrict__ U1p2, double* __restrict__ partials, double* __restrict__ Uout,
-    const HaloSrc* __restrict__ halo, const uint8_t* __restrict__ cube_mask) {
+    const HaloSrc* __restrict__ halo, const uint8_t* __restrict__ cube_mask, int aux) {
   __shared__ double pr[4][3 * PR_PLANE];   // u, v, w, T = P/(rho R) of 3 planes
   __shared__ double fy[5][NFACE];
   __shared__ double fz[6][NFACE];          // flux + face pressure average
@@ -696,12 +696,21 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
       }
     }
     __syncthreads();
-    // y-faces (f < 272: joins j = p and p+1 at k) and z-faces (joins k = p
-    // and p+1 at j) of plane i in one pass.
+    // y-faces (face f = (p-1)*16 + kk-2 joins j = p and p+1 at k = kk) and
+    // z-faces (g = (jj-2)*17 + p-1 joins k = p and p+1 at j = jj) of plane i.
+    // Round 0: each lane the y-face above its cell; round 1: the z-face
+    // above it (every wave evaluates one face kind at a time); rounds 2 and
+    // 3: the 16 y-faces and 16 z-faces next to the first ghost layers
+    // (p = 1), on lanes of waves 0 and 1.
     if (PUSH) corner_prims(i);
-    for (int f = t; f < 2 * NFACE; f += WG) {
-      if (f < NFACE) {
-        const int p = f / 16 + 1, kk = f % 16 + 2;
+#pragma unroll 1
+    for (int r = 0; r < 4; ++r) {
+      const bool bnd = r >= 2;
+      const int l0 = r == 2 ? 0 : 64;
+      if (bnd && (t < l0 || t >= l0 + 16)) continue;
+      if ((r & 1) == 0) {
+        const int p = bnd ? 1 : j, kk = bnd ? t - l0 + 2 : k;
+        const int f = (p - 1) * 16 + (kk - 2);
         const int8_t* fb = fpl + p * 20 + kk;
         const int o = p >= 2 ? order_fwd(p, fb[-20], fb[20], fb[40]) : order_bwd(2, fb[-20], fb[0], fb[40]);
         double F[5], dP;
@@ -709,8 +718,8 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
 #pragma unroll
         for (int v = 0; v < 5; ++v) fy[v][f] = F[v];
       } else {
-        const int g = f - NFACE;
-        const int jj = g / 17 + 2, p = g % 17 + 1;
+        const int jj = bnd ? t - l0 + 2 : j, p = bnd ? 1 : k;
+        const int g = (jj - 2) * 17 + (p - 1);
         const int8_t* fb = fpl + jj * 20 + p;
         const int o = p >= 2 ? order_fwd(p, fb[-1], fb[1], fb[2]) : order_bwd(2, fb[-1], fb[0], fb[2]);
         double F[5], dP;
@@ -919,15 +928,22 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
       const double d55 = (2 * deltaT * (2 * H * KM1 * (beta - 1) * deltaT + KG * T * (2 * deltaT + 3 * beta * deltaTau))) / temp;
       out[4] = deltaTau * (d51 * Rk[0] + d52 * Rk[1] + d53 * Rk[2] + d54 * Rk[3] + d55 * Rk[4]);
     }
+    // The pushing kernel reads none of Rku1, U1p1, U1p2 and Residual1 back;
+    // it stores them only when asked (aux), which the engine does in the
+    // last inner iteration of a step, so the arrays end as the reference's.
+    if (!PUSH || aux) {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) Rku[b + (size_t)v * PLANE] = out[v];
+      for (int v = 0; v < 5; ++v) Rku[b + (size_t)v * PLANE] = out[v];
+    }
     if (IF < 1) {
       const double px = pow2 ? -(dPxf - dPxb) * inv : -(dPxf - dPxb) / dx;
       const double pz = pow2 ? -(dPzf - dPzb) * inv : -(dPzf - dPzb) / dz;
       const double r0 = -(px + vF[1]) * dx * dy * dz;
       const double r1 = -(pz + vF[3]) * dx * dy * dz;
-      Res[b] = r0;
-      Res[b + PLANE] = r1;
+      if (!PUSH || aux) {
+        Res[b] = r0;
+        Res[b + PLANE] = r1;
+      }
       if ((fuse || PUSH) && RK == 3) {
         acc[5] += r0;
         acc[6] += r1;
@@ -944,14 +960,14 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
           for (int v = 0; v < 5; ++v) {
             const double x = u[v] + out[v];
             nv[v] = x;
-            U1p1[b + (size_t)v * PLANE] = x;
+            if (aux) U1p1[b + (size_t)v * PLANE] = x;
           }
         } else if (RK == 2) {
 #pragma unroll
           for (int v = 0; v < 5; ++v) {
             const double x = 0.75 * u[v] + 0.25 * u[v] + 0.25 * out[v];
             nv[v] = x;
-            U1p2[b + (size_t)v * PLANE] = x;
+            if (aux) U1p2[b + (size_t)v * PLANE] = x;
           }
         } else {
           const double ro = u[0];
@@ -1772,18 +1788,18 @@ void launch_flux_march(hipStream_t s, double* U, const double* U1, const double*
   if (ncube > 1)
     hipLaunchKernelGGL(k_flux_march<false>, dim3((ncube - 1) * nseg), dim3(WG), 0, s, U, U1, U1q,
                        fws, cube_size, Rku, Res, pt, deltaT, deltaTau, e, nseg, RK,
-                       fuse && nseg == 1, U1p1, U1p2, partials, nullptr, nullptr, nullptr);
+                       fuse && nseg == 1, U1p1, U1p2, partials, nullptr, nullptr, nullptr, 1);
 }
 
 void launch_flux_push(hipStream_t s, const double* Uin, double* Uout, const HaloSrc* halo,
                       const uint8_t* cube_mask, const double* U1, const double* U1q,
                       const int8_t* fws, const double* cube_size, double* Rku, double* Res,
                       int ncube, PhysTab pt, double deltaT, double deltaTau, double e, int RK,
-                      double* U1p1, double* U1p2, double* partials) {
+                      double* U1p1, double* U1p2, double* partials, int aux) {
   if (ncube > 1)
     hipLaunchKernelGGL(k_flux_march<true>, dim3(ncube - 1), dim3(WG), 0, s,
                        const_cast<double*>(Uin), U1, U1q, fws, cube_size, Rku, Res, pt, deltaT,
-                       deltaTau, e, 1, RK, 0, U1p1, U1p2, partials, Uout, halo, cube_mask);
+                       deltaTau, e, 1, RK, 0, U1p1, U1p2, partials, Uout, halo, cube_mask, aux);
 }
 
 __global__ __launch_bounds__(WG) void k_copy_d(const double* __restrict__ src,

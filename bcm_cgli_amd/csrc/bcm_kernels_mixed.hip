@@ -36,12 +36,44 @@ namespace bcm {
 namespace {
 
 constexpr int WG = 256;
+// First lanes of the 16 that evaluate the y- and z-faces next to the ghost
+// layers (waves 1 and 2: wave 0 stages the plane's border cells and forms
+// their primitives, wave 3 forms the corner primitives).
+#ifndef BCM_MIXED_BND_Y
+#define BCM_MIXED_BND_Y 64
+#endif
+#ifndef BCM_MIXED_BND_Z
+#define BCM_MIXED_BND_Z 128
+#endif
+constexpr int BND_Y = BCM_MIXED_BND_Y, BND_Z = BCM_MIXED_BND_Z;
 constexpr float KGf = 1.4f;
 constexpr float KM1f = 0.4f;
 constexpr float IKM1f = 2.5f;          // 1/(K-1)
 constexpr float IRGf = 1.0f / 287.0f;  // 1/R
 constexpr float HEATf = 717.5f * 1.4f / 0.72f;   // Cv K / Pr
 constexpr double KM1d = 1.4 - 1;
+
+// x of the lane CTRL's data-parallel move selects (0 where it selects none),
+// both halves of a double moved alike.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_d(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, ROW_MASK, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+// Sum of x over the 64 lanes of the wave, complete in lane 63, in a fixed
+// order: prefix sums along each row of 16 lanes (row_shr 1, 2, 4, 8), then
+// row 0's total into row 1 and row 2's into row 3 (row_bcast 15), then the
+// first half's total into the second half (row_bcast 31).
+__device__ __forceinline__ double wave_sum(double x) {
+  x += dpp_d<0x111, 0xf>(x);
+  x += dpp_d<0x112, 0xf>(x);
+  x += dpp_d<0x114, 0xf>(x);
+  x += dpp_d<0x118, 0xf>(x);
+  x += dpp_d<0x142, 0xa>(x);
+  x += dpp_d<0x143, 0xc>(x);
+  return x;
+}
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
@@ -150,19 +182,37 @@ __device__ __forceinline__ void roe_f(const float (&L)[5], const float (&R)[5], 
 }
 
 // Face states of order 5, 3 or 1 (Flux.cpp:499-596) from six cells s0..s5 =
-// p-2..p+3 of the face between p and p+1.
-__device__ __forceinline__ void recon(int order, float s0, float s1, float s2, float s3, float s4,
-                                      float s5, float& L, float& R) {
-  if (order == 5) {
-    L = (1.f / 60) * (2 * s0 - 13 * s1 + 47 * s2 + 27 * s3 - 3 * s4);
-    R = (1.f / 60) * (-3 * s1 + 27 * s2 + 47 * s3 - 13 * s4 + 2 * s5);
-  } else if (order == 3) {
-    L = (1.f / 6) * (-s1 + 5 * s2 + 2 * s3);
-    R = (1.f / 6) * (2 * s2 + 5 * s3 - s4);
-  } else {
-    L = s2;
-    R = s3;
-  }
+// p-2..p+3 of the face between p and p+1, as weighted sums with the order's
+// weights (zero for the cells an order does not use), so that lanes of one
+// wave with different orders run the same instructions:
+//   5: L = (2 s0 - 13 s1 + 47 s2 + 27 s3 - 3 s4) / 60,
+//      R = (-3 s1 + 27 s2 + 47 s3 - 13 s4 + 2 s5) / 60
+//   3: L = (-s1 + 5 s2 + 2 s3) / 6,  R = (2 s2 + 5 s3 - s4) / 6
+//   1: L = s2, R = s3.
+// Cells outside an order's stencil must hold finite values (callers pass
+// an in-range cell in their place).
+struct ReconW {
+  float a0, a1, a2, a3, a4, b1, b2, b3, b4, b5;
+};
+__device__ __forceinline__ ReconW recon_weights(int order) {
+  const bool o5 = order == 5, o3 = order == 3;
+  ReconW w;
+  w.a0 = o5 ? 2.f / 60 : 0.f;
+  w.a1 = o5 ? -13.f / 60 : (o3 ? -1.f / 6 : 0.f);
+  w.a2 = o5 ? 47.f / 60 : (o3 ? 5.f / 6 : 1.f);
+  w.a3 = o5 ? 27.f / 60 : (o3 ? 2.f / 6 : 0.f);
+  w.a4 = o5 ? -3.f / 60 : 0.f;
+  w.b1 = o5 ? -3.f / 60 : 0.f;
+  w.b2 = o5 ? 27.f / 60 : (o3 ? 2.f / 6 : 0.f);
+  w.b3 = o5 ? 47.f / 60 : (o3 ? 5.f / 6 : 1.f);
+  w.b4 = o5 ? -13.f / 60 : (o3 ? -1.f / 6 : 0.f);
+  w.b5 = o5 ? 2.f / 60 : 0.f;
+  return w;
+}
+__device__ __forceinline__ void recon(const ReconW& w, float s0, float s1, float s2, float s3,
+                                      float s4, float s5, float& L, float& R) {
+  L = w.a0 * s0 + w.a1 * s1 + w.a2 * s2 + w.a3 * s3 + w.a4 * s4;
+  R = w.b1 * s1 + w.b2 * s2 + w.b3 * s3 + w.b4 * s4 + w.b5 * s5;
 }
 
 __device__ __forceinline__ int order_fwd(int p, int fm1, int fp1, int fp2) {
@@ -174,24 +224,42 @@ __device__ __forceinline__ int order_bwd(int q, int fm2, int fm1, int fp1) {
   return 1;
 }
 
-constexpr int PR = 18 * 18;       // primitive plane, j,k in [1,18]
+// LDS layouts.  Lane t of a workgroup owns cell (j, k) with k = t % 16 + 2
+// and j = 2 + w + 4 q for wave w = t / 64 and q = (t / 16) % 4: the four
+// 16-cell rows a wave reads at once lie 4 rows apart, so row strides of 20
+// (staged plane, primitive ring) put them on distinct LDS banks.
+constexpr int PRW = 20;           // row stride of the primitive ring
+constexpr int PR = 18 * PRW;      // primitive plane, j,k in [1,18]
 constexpr int NF = 17 * 16;       // faces of one axis in one plane
+constexpr int NFY = 4 * 80;       // y-face slots (17 rows of 16 in 4 blocks of 5 rows)
+__device__ __forceinline__ int ring_idx(int jj, int kk) { return (jj - 1) * PRW + (kk - 1); }
+// y-face between rows p and p+1 at k = kk (p = 1..17): block (p-1) % 4, row
+// (p-1) / 4 in it, so a wave's rows are consecutive slots.
+__device__ __forceinline__ int fy_idx(int p, int kk) {
+  return ((p - 1) & 3) * 80 + ((p - 1) >> 2) * 16 + (kk - 2);
+}
+// z-face between columns p and p+1 in row jj (jj = 2..17, p = 1..17).
+__device__ __forceinline__ int fz_idx(int jj, int p) {
+  return ((jj - 2) & 3) * 68 + ((jj - 2) >> 2) * 17 + (p - 1);
+}
 
-// Flux of the face between p and p+1 along axis d (1 or 2) of the staged
-// plane: base is the plane index of cell p, s the index stride of the axis.
-__device__ __forceinline__ void plane_face(const float (*cp)[400], int base, int s, int d,
-                                           int order, float e, const RefF& rf, float (&F)[5],
-                                           float& dP) {
+// Flux of the face between p and p+1 along axis D (1 or 2) of the staged
+// plane: base is the plane index of cell p.  Orders below 5 read cell p in
+// place of the cells p-2 and p+3 they do not use (which may lie beyond the
+// plane).
+template <int D>
+__device__ __forceinline__ void plane_face(const float (*cp)[400], int base, int order, float e,
+                                           const RefF& rf, float (&F)[5], float& dP) {
+  constexpr int s = D == 1 ? 20 : 1;
+  const ReconW rw = recon_weights(order);
+  const int o0 = order == 5 ? -2 * s : 0, o5 = order == 5 ? 3 * s : 0;
   float L[5], R[5];
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
     const float* q = cp[v] + base;
-    const float s1 = q[-s], s2 = q[0], s3 = q[s], s4 = q[2 * s];
-    const float s0 = order == 5 ? q[-2 * s] : 0.f;
-    const float s5 = order == 5 ? q[3 * s] : 0.f;
-    recon(order, s0, s1, s2, s3, s4, s5, L[v], R[v]);
+    recon(rw, q[o0], q[-s], q[0], q[s], q[2 * s], q[o5], L[v], R[v]);
   }
-  roe_f(L, R, d, e, rf, F, dP);
+  roe_f(L, R, D, e, rf, F, dP);
 }
 
 // Viscous flux through the face between cell P = (i+a0, j0, k0) and P + e_D
@@ -200,12 +268,12 @@ __device__ __forceinline__ void plane_face(const float (*cp)[400], int base, int
 // (slot = plane & 3, j,k in [1,18]).  L receives the x, y, z stress and the
 // energy flux.
 template <int D>
-__device__ __forceinline__ void visc_face(const float (*ring)[4][18 * 18], int i, int a0, int j0,
+__device__ __forceinline__ void visc_face(const float (*ring)[4][PR], int i, int a0, int j0,
                                           int k0, const RefF& rf, float inv, float iT0,
                                           float muL, float sut, float (&L)[4]) {
   constexpr int da = D == 0, dj = D == 1, dk = D == 2;
   auto P = [&](int a, int jj, int kk, int m) -> float {
-    return ring[(i + a) & 3][m][(jj - 1) * 18 + (kk - 1)];
+    return ring[(i + a) & 3][m][ring_idx(jj, kk)];
   };
   float me[4], fv[4];
 #pragma unroll
@@ -238,6 +306,36 @@ __device__ __forceinline__ void visc_face(const float (*ring)[4][18 * 18], int i
   for (int m = 0; m < 3; ++m)
     L[m] = (m == D) ? mu * (2 * g[D][D] - (2.0f / 3.0f) * div) : mu * (g[m][D] + g[D][m]);
   L[3] = uf * L[0] + vf * L[1] + wf * L[2] + mu * HEATf * g[D][3];
+}
+
+// y-face (D = 1) between (p, kk) and (p + 1, kk), or z-face (D = 2) between
+// (jj, p) and (jj, p + 1), of staged plane i: inviscid flux into fy / fz (face
+// slot fy_idx(p, kk), or fz_idx(jj, p); fz also keeps the face pressure
+// average) and viscous flux into vy / vz.  Face p = 1 lies between the first
+// ghost layer and the first interior cell.
+template <int D>
+__device__ __forceinline__ void yz_face(const float (*cp)[400], const int8_t* fl,
+                                        const float (*ring)[4][PR], int i, int a, int p,
+                                        float e, const RefF& rf, float inv, float iT0, float muL,
+                                        float sut, float* fo, float* vo) {
+  constexpr int W = D == 1 ? NFY : NF;   // row length of the face arrays
+  constexpr int s = D == 1 ? 20 : 1;
+  const int base = D == 1 ? p * 20 + a : a * 20 + p;
+  const int8_t* fb = fl + base;
+  const int o = p >= 2 ? order_fwd(p, fb[-s], fb[s], fb[2 * s])
+                       : order_bwd(2, fb[-s], fb[0], fb[2 * s]);
+  float F[5], dP, Lv[4];
+  plane_face<D>(cp, base, o, e, rf, F, dP);
+  if (D == 1)
+    visc_face<1>(ring, i, 0, p, a, rf, inv, iT0, muL, sut, Lv);
+  else
+    visc_face<2>(ring, i, 0, a, p, rf, inv, iT0, muL, sut, Lv);
+  const int g = D == 1 ? fy_idx(p, a) : fz_idx(a, p);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) fo[v * W + g] = F[v];
+  if (D == 2) fo[5 * W + g] = dP;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) vo[v * W + g] = Lv[v];
 }
 
 // Border cells of a 20x20 plane (j or k in {0,1,18,19}): 144 of them.
@@ -273,28 +371,44 @@ __device__ __forceinline__ int corner_source_slot(int t) {
   }
 }
 
+// Segmented march: with nseg > 1 each cube's interior planes 2..17 are split
+// into nseg runs of 16/nseg planes, one workgroup each (more workgroups when
+// few cubes are resident).  A run starting at i0 > 2 first executes one
+// prologue iteration at i0 - 1 that only forms the x-face i0 - 1/2 (inviscid
+// and viscous) the run carries into its first plane: the same loop body, and
+// so the same instructions and bits, as the unsegmented march computes it
+// with.  Norm partials are rows per (group of 4 planes, wave), 16 per cube,
+// summed in a fixed order, so every result is bitwise the same for every
+// nseg.
+
 __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
     const float* __restrict__ U, float* __restrict__ Uout, const HaloSrc* __restrict__ halo,
     const uint8_t* __restrict__ cube_mask, const double* __restrict__ U1,
     const double* __restrict__ U1q,
     const int8_t* __restrict__ fws, const double* __restrict__ cube_size, PhysTab pt,
-    double deltaT, double deltaTau, double e_d, int RK, double* __restrict__ partials) {
+    double deltaT, double deltaTau, double e_d, int RK, double* __restrict__ partials,
+    int nseg) {
   __shared__ float cp[2][5][400];     // conserved values of plane i (double-buffered)
   __shared__ int8_t fl[2][400];       // flags of plane i
   __shared__ float pr[4][4][PR];      // u, v, w, T of planes (ring of 4)
-  __shared__ float fy[5][NF];
+  __shared__ float fy[5][NFY];
   __shared__ float fz[6][NF];         // flux + face pressure average
-  __shared__ float vy[4][NF];         // viscous fluxes of the y- and z-faces
+  __shared__ float vy[4][NFY];        // viscous fluxes of the y- and z-faces
   __shared__ float vz[4][NF];
-  __shared__ double red[(WG / 64) * 7];
   __shared__ float esrc[2][8][5];     // corner sources of interior planes (by parity)
 
-  const int c = blockIdx.x + 1;
+  const int seg = blockIdx.x % nseg;
+  const int c = blockIdx.x / nseg + 1;
   const int cc = pt.cube_case[c];
   if (cc < 0) return;
   const DevPhys ph = pt.phys[cc];
+  // Planes i0..i1 of this run; the march starts at `start` (a prologue
+  // iteration at i0 - 1 for runs after the first).
+  const int len = 16 / nseg;
+  const int i0 = 2 + seg * len, i1 = i0 + len - 1;
+  const int start = i0 == 2 ? 2 : i0 - 1;
   const int t = threadIdx.x;
-  const int j = (t >> 4) + 2, k = (t & 15) + 2;
+  const int j = 2 + (t >> 6) + 4 * ((t >> 4) & 3), k = (t & 15) + 2;
   const int col = j * 20 + k;
   const double dxd = cube_size[c] / 16;
   const float inv = (float)(1.0 / dxd), vol = (float)(dxd * dxd * dxd);
@@ -329,17 +443,17 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
   };
   const double r3 = uref[3];
   const size_t gcol = (size_t)c * CUBE_STRIDE + col;
-  // Own column: w[m] = plane i-3+m, fw[m] = flag of plane i-2+m.
+  // Own column: w[m] = plane i-3+m, fw[m] = flag of plane i-2+m (plane -1,
+  // before the first ghost plane, a placeholder no stencil uses).
   float w[7][5];
   int fw[5];
 #pragma unroll
-  for (int v = 0; v < 5; ++v) w[0][v] = 1.f;
+  for (int m = 0; m < 7; ++m)
 #pragma unroll
-  for (int m = 1; m < 7; ++m)
+    for (int v = 0; v < 5; ++v)
+      w[m][v] = start - 3 + m >= 0 ? Uc[v * PLANE + (start - 3 + m) * 400 + col] : 1.f;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) w[m][v] = Uc[v * PLANE + (m - 1) * 400 + col];
-#pragma unroll
-  for (int m = 0; m < 5; ++m) fw[m] = Fc[m * 400 + col];
+  for (int m = 0; m < 5; ++m) fw[m] = Fc[(start - 2 + m) * 400 + col];
 
   // Deviations of the viscous block's primitives u, v, w and T = P/(rho R)
   // from the reference state: dT = (dP - (P0/rho0) drho) / (rho R).
@@ -358,7 +472,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
     float* dst = &pr[p & 3][0][0];
     float u, v, ww, T;
     prims(own, u, v, ww, T);
-    const int o = (j - 1) * 18 + (k - 1);
+    const int o = ring_idx(j, k);
     dst[o] = u;
     dst[PR + o] = v;
     dst[2 * PR + o] = ww;
@@ -393,7 +507,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
         }
       }
       prims(q, u, v, ww, T);
-      const int ob = (cell / 20 - 1) * 18 + (cell % 20 - 1);
+      const int ob = ring_idx(cell / 20, cell % 20);
       dst[ob] = u;
       dst[PR + ob] = v;
       dst[2 * PR + ob] = ww;
@@ -413,7 +527,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
     for (int m = 0; m < 5; ++m) q[m] = 0.5f * (esrc[p & 1][2 * q4][m] + esrc[p & 1][2 * q4 + 1][m]);
     float u, v, ww, T;
     prims(q, u, v, ww, T);
-    const int o = (q4 & 2 ? 17 * 18 : 0) + (q4 & 1 ? 17 : 0);
+    const int o = ring_idx(q4 & 2 ? 18 : 1, q4 & 1 ? 18 : 1);
     float* dst = &pr[p & 3][0][0];
     dst[o] = u;
     dst[PR + o] = v;
@@ -444,14 +558,14 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
   };
   float bcp[5] = {0, 0, 0, 0, 0}, bpr[5] = {1, 0, 0, 0, 0};
   int bfl = 0;
-  load_b18(1, bpr);
-  fill_prims(1, w[2], bpr);
-  load_b18(2, bpr);
-  fill_prims(2, w[3], bpr);
+  load_b18(start - 1, bpr);
+  fill_prims(start - 1, w[2], bpr);
+  load_b18(start, bpr);
+  fill_prims(start, w[3], bpr);
   __syncthreads();
-  corner_prims(2);
-  load_b20(2, bcp, bfl);
-  load_b18(3, bpr);
+  corner_prims(start);
+  load_b20(start, bcp, bfl);
+  load_b18(start + 1, bpr);
 
   float cF[5] = {0, 0, 0, 0, 0}, cdP = 0.f;
   float cL[4] = {0, 0, 0, 0};   // viscous x-face flux carried from the previous plane
@@ -459,26 +573,29 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
 
 #pragma unroll 1
-  for (int i = 2; i <= 17; ++i) {
+  for (int i = start; i <= i1; ++i) {
     const int b = i & 1;
+    const bool pro = i < i0;    // prologue: only the x-face i + 1/2
     // ---- stage plane i (conserved, flags) and primitives of plane i+1 ----
+    if (!pro) {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) cp[b][v][col] = w[3][v];
-    fl[b][col] = (int8_t)fw[2];
-    if (t < 144) {
-      const int cell = border20(t);
+      for (int v = 0; v < 5; ++v) cp[b][v][col] = w[3][v];
+      fl[b][col] = (int8_t)fw[2];
+      if (t < 144) {
+        const int cell = border20(t);
 #pragma unroll
-      for (int v = 0; v < 5; ++v)
-        cp[b][v][cell] = v == 3 && zlo && cell % 20 < 2 ? zbc_neg(bcp[v], r3) : bcp[v];
-      fl[b][cell] = (int8_t)bfl;
+        for (int v = 0; v < 5; ++v)
+          cp[b][v][cell] = v == 3 && zlo && cell % 20 < 2 ? zbc_neg(bcp[v], r3) : bcp[v];
+        fl[b][cell] = (int8_t)bfl;
+      }
     }
     fill_prims(i + 1, w[4], bpr);
     // Loads for later: the next iteration's borders, this cell's time levels
     // folded into the FP64 source constant Bv = 3 Uref - 4 U1 + U1q.
     load_b20(i + 1, bcp, bfl);
     load_b18(i + 2, bpr);
-    double Bv[5];
-    {
+    double Bv[5] = {0, 0, 0, 0, 0};
+    if (!pro) {
       const size_t gb = gcol + (size_t)i * 400;
 #pragma unroll
       for (int v = 0; v < 5; ++v)
@@ -488,31 +605,22 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
 
     // ---- y- and z-faces of plane i: 272 + 272 faces over 256 lanes ----
     if (i + 1 <= 17) corner_prims(i + 1);
-    for (int f = t; f < 2 * NF; f += WG) {
-      const bool yface = f < NF;
-      const int g = yface ? f : f - NF;
-      const int p = yface ? g / 16 + 1 : g % 17 + 1;
-      const int base = yface ? p * 20 + (g % 16 + 2) : (g / 17 + 2) * 20 + p;
-      const int s = yface ? 20 : 1;
-      const int8_t* fb = fl[b] + base;
-      const int o = p >= 2 ? order_fwd(p, fb[-s], fb[s], fb[2 * s])
-                           : order_bwd(2, fb[-s], fb[0], fb[2 * s]);
-      float F[5], dP, Lv[4];
-      plane_face(cp[b], base, s, yface ? 1 : 2, o, e, rf, F, dP);
-      if (yface) {
-        visc_face<1>(pr, i, 0, p, g % 16 + 2, rf, inv, iT0, muL, sut, Lv);
-#pragma unroll
-        for (int v = 0; v < 5; ++v) fy[v][g] = F[v];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) vy[v][g] = Lv[v];
-      } else {
-        visc_face<2>(pr, i, 0, g / 17 + 2, p, rf, inv, iT0, muL, sut, Lv);
-#pragma unroll
-        for (int v = 0; v < 5; ++v) fz[v][g] = F[v];
-        fz[5][g] = dP;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) vz[v][g] = Lv[v];
-      }
+    // Round 0: each lane the y-face above its cell; round 1: the z-face above
+    // it (every wave evaluates one face kind at a time); rounds 2 and 3: the
+    // 16 y-faces and the 16 z-faces between the first ghost layer and the
+    // first interior cells, on lanes of two different waves.  One face per
+    // loop iteration keeps the faces' working sets apart.
+#pragma unroll 1
+    for (int r = pro ? 4 : 0; r < 4; ++r) {
+      const bool bnd = r >= 2;
+      const int l0 = r == 2 ? BND_Y : BND_Z;
+      if (bnd && (t < l0 || t >= l0 + 16)) continue;
+      if ((r & 1) == 0)
+        yz_face<1>(cp[b], fl[b], pr, i, bnd ? t - l0 + 2 : k, bnd ? 1 : j, e, rf, inv, iT0, muL,
+                   sut, &fy[0][0], &vy[0][0]);
+      else
+        yz_face<2>(cp[b], fl[b], pr, i, bnd ? t - l0 + 2 : j, bnd ? 1 : k, e, rf, inv, iT0, muL,
+                   sut, &fz[0][0], &vz[0][0]);
     }
     __syncthreads();
 
@@ -520,22 +628,27 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
     // ---- inviscid x: forward face from the column, backward carried ----
     float Rx[5], Ry[5], Rz[5], dPxb, dPxf, dPzb, dPzf;
     {
+      // Column cells an order does not use hold finite values (placeholder
+      // planes beyond the cube repeat a loaded one).
       const int of = order_fwd(i, fw[1], fw[3], fw[4]);
       float L[5], R[5], Ff[5];
+      const ReconW rwf = recon_weights(of);
 #pragma unroll
       for (int v = 0; v < 5; ++v)
-        recon(of, w[1][v], w[2][v], w[3][v], w[4][v], w[5][v], w[6][v], L[v], R[v]);
+        recon(rwf, w[1][v], w[2][v], w[3][v], w[4][v], w[5][v], w[6][v], L[v], R[v]);
       roe_f(L, R, 0, e, rf, Ff, dPxf);
       const int ob = order_bwd(i, fw[0], fw[1], fw[3]);
-      if (i != 2 && ob == corder) {
+      if (pro) {
+      } else if (i != 2 && ob == corder) {
         dPxb = cdP;
 #pragma unroll
         for (int v = 0; v < 5; ++v) Rx[v] = Ff[v] - cF[v];
       } else {
         float Fb[5];
+        const ReconW rwb = recon_weights(ob);
 #pragma unroll
         for (int v = 0; v < 5; ++v)
-          recon(ob, w[0][v], w[1][v], w[2][v], w[3][v], w[4][v], w[5][v], L[v], R[v]);
+          recon(rwb, w[0][v], w[1][v], w[2][v], w[3][v], w[4][v], w[5][v], L[v], R[v]);
         roe_f(L, R, 0, e, rf, Fb, dPxb);
 #pragma unroll
         for (int v = 0; v < 5; ++v) Rx[v] = Ff[v] - Fb[v];
@@ -547,24 +660,24 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
     }
     const int8_t* fc = fl[b] + col;
     // ---- inviscid y ----
-    {
+    if (!pro) {
       const int fm2 = fc[-40], fm1 = fc[-20], fp1 = fc[20];
-      const int ff = (j - 1) * 16 + (k - 2);
+      const int ff = fy_idx(j, k), fb = fy_idx(j - 1, k);
       const int ob = order_bwd(j, fm2, fm1, fp1);
       if (j == 2 || ob == order_fwd(j - 1, fm2, fc[0], fp1)) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) Ry[v] = fy[v][ff] - fy[v][ff - 16];
+        for (int v = 0; v < 5; ++v) Ry[v] = fy[v][ff] - fy[v][fb];
       } else {
         float Fb[5], dPu;
-        plane_face(cp[b], col - 20, 20, 1, ob, e, rf, Fb, dPu);
+        plane_face<1>(cp[b], col - 20, ob, e, rf, Fb, dPu);
 #pragma unroll
         for (int v = 0; v < 5; ++v) Ry[v] = fy[v][ff] - Fb[v];
       }
     }
     // ---- inviscid z ----
-    {
+    if (!pro) {
       const int fm2 = fc[-2], fm1 = fc[-1], fp1 = fc[1];
-      const int ff = (j - 2) * 17 + (k - 1);
+      const int ff = fz_idx(j, k);   // the backward face is ff - 1
       const int ob = order_bwd(k, fm2, fm1, fp1);
       dPzf = fz[5][ff];
       if (k == 2 || ob == order_fwd(k - 1, fm2, fc[0], fp1)) {
@@ -573,27 +686,29 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
         for (int v = 0; v < 5; ++v) Rz[v] = fz[v][ff] - fz[v][ff - 1];
       } else {
         float Fb[5];
-        plane_face(cp[b], col - 1, 1, 2, ob, e, rf, Fb, dPzb);
+        plane_face<2>(cp[b], col - 1, ob, e, rf, Fb, dPzb);
 #pragma unroll
         for (int v = 0; v < 5; ++v) Rz[v] = fz[v][ff] - Fb[v];
       }
     }
 
     // ---- viscous (Flux.cpp:1722-2279): x-face carried, y/z faces shared ----
-    float vF[5];
+    float vF[5] = {0, 0, 0, 0, 0};
     {
       float Lf[4];
       visc_face<0>(pr, i, 0, j, k, rf, inv, iT0, muL, sut, Lf);
       if (i == 2) visc_face<0>(pr, i, -1, j, k, rf, inv, iT0, muL, sut, cL);
-      const int fyf = (j - 1) * 16 + (k - 2), fzf = (j - 2) * 17 + (k - 1);
-      vF[0] = 0.f;
+      if (!pro) {
+        const int fyf = fy_idx(j, k), fyb = fy_idx(j - 1, k), fzf = fz_idx(j, k);
 #pragma unroll
-      for (int v = 1; v < 5; ++v)
-        vF[v] = ((Lf[v - 1] - cL[v - 1]) + (vy[v - 1][fyf] - vy[v - 1][fyf - 16]) +
-                 (vz[v - 1][fzf] - vz[v - 1][fzf - 1])) * inv;
+        for (int v = 1; v < 5; ++v)
+          vF[v] = ((Lf[v - 1] - cL[v - 1]) + (vy[v - 1][fyf] - vy[v - 1][fyb]) +
+                   (vz[v - 1][fzf] - vz[v - 1][fzf - 1])) * inv;
+      }
 #pragma unroll
       for (int v = 0; v < 4; ++v) cL[v] = Lf[v];
     }
+    if (!pro) {
 
     // ---- residual with the FP64 dual-time source ----
     float u[5];
@@ -705,6 +820,20 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
         if (nz > 0) Uout[(size_t)nz * CUBE_STRIDE + v * PLANE + oz] = nv[v];
       }
     }
+    // Norm partials at the end of each group of 4 planes: one row per wave.
+    if (RK == 3 && ((i - 2) & 3) == 3) {
+#pragma unroll
+      for (int m = 0; m < 7; ++m) acc[m] = wave_sum(acc[m]);
+      if ((t & 63) == 63) {
+        double* row =
+            partials + ((size_t)(c - 1) * MIXED_ROWS_PER_CUBE + ((i - 2) >> 2) * 4 + (t >> 6)) * 8;
+#pragma unroll
+        for (int m = 0; m < 7; ++m) row[m] = acc[m];
+      }
+#pragma unroll
+      for (int m = 0; m < 7; ++m) acc[m] = 0.0;
+    }
+    }  // !pro
 
     // ---- advance the column window ----
 #pragma unroll
@@ -722,26 +851,9 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
 
   // Ghost regions the domain BCs wrote this stage go to Uout as they are (no
   // neighbour writes them), so Uout holds them as an in-place update would.
-  {
+  if (seg == 0) {
     const int bm = cube_mask[c];
     if (bm) copy_bc_regions(U, Uout, c, bm, uref[3]);
-  }
-  if (RK == 3) {
-#pragma unroll
-    for (int m = 0; m < 7; ++m)
-      for (int off = 32; off > 0; off >>= 1) acc[m] += __shfl_down(acc[m], off, 64);
-    const int lane = t & 63, wave = t >> 6;
-    if (lane == 0)
-#pragma unroll
-      for (int m = 0; m < 7; ++m) red[wave * 7 + m] = acc[m];
-    __syncthreads();
-    if (t == 0)
-#pragma unroll
-      for (int m = 0; m < 7; ++m) {
-        double s = red[m];
-        for (int wv = 1; wv < WG / 64; ++wv) s += red[wv * 7 + m];
-        partials[(size_t)blockIdx.x * 8 + m] = s;
-      }
   }
 }
 
@@ -804,10 +916,12 @@ __global__ __launch_bounds__(WG) void k_soa_to_aos_f(const float* __restrict__ s
 void launch_flux_mixed(hipStream_t s, const float* Uin, float* Uout, const HaloSrc* halo,
                        const uint8_t* cube_mask, const double* U1, const double* U1q,
                        const int8_t* fws, const double* cube_size, int ncube, PhysTab pt,
-                       double deltaT, double deltaTau, double e, int RK, double* partials) {
+                       double deltaT, double deltaTau, double e, int RK, double* partials,
+                       int nseg) {
   if (ncube > 1)
-    hipLaunchKernelGGL(k_flux_mixed, dim3(ncube - 1), dim3(WG), 0, s, Uin, Uout, halo, cube_mask,
-                       U1, U1q, fws, cube_size, pt, deltaT, deltaTau, e, RK, partials);
+    hipLaunchKernelGGL(k_flux_mixed, dim3((ncube - 1) * nseg), dim3(WG), 0, s, Uin, Uout, halo,
+                       cube_mask, U1, U1q, fws, cube_size, pt, deltaT, deltaTau, e, RK, partials,
+                       nseg);
 }
 
 void launch_copy_f(hipStream_t s, const float* src, float* dst, size_t n) {

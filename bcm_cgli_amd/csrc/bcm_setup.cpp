@@ -753,28 +753,48 @@ void uniform_lattice(const CaseParams& p, Case& cs) {
 // face with four smaller neighbours lists them in slots 1..4 with slot 2 on
 // the upper half of the first tangential axis and slot 3 on the upper half of
 // the second (tangential axes in x, y, z order), e.g. :1825-1883 for +x.
+// With a second refinement box (on the doubled lattice) the split cubes'
+// children inside it are split again, recursively in place: a three-level
+// grid.  Positions count in units of the smallest cube.
 int two_level_lattice(const CaseParams& p, Case& cs) {
+  bool second = true;
+  for (int d = 0; d < 3; ++d)
+    if (p.refine2_hi[d] <= p.refine2_lo[d]) second = false;
+  const int S = second ? 4 : 2;                       // lattice cube edge in smallest cubes
   const int L[3] = {p.lattice[0], p.lattice[1], p.lattice[2]};
-  const int F[3] = {2 * L[0], 2 * L[1], 2 * L[2]};   // fine positions per axis
-  const double h = p.cube_size / 2;
+  const int F[3] = {S * L[0], S * L[1], S * L[2]};   // finest positions per axis
+  const double h = p.cube_size / S;
   std::vector<int> owner((size_t)F[0] * F[1] * F[2], 0);
   auto own = [&](int x, int y, int z) -> int& { return owner[((size_t)z * F[1] + y) * F[0] + x]; };
   struct Cube { int x0[3]; int s; };
   std::vector<Cube> cubes(1);
+  auto in_box = [](const int* ii, const int* lo, const int* hi) {
+    bool in = true;
+    for (int d = 0; d < 3; ++d) in = in && ii[d] >= lo[d] && ii[d] < hi[d];
+    return in;
+  };
   for (int iz = 0; iz < L[2]; ++iz)
     for (int iy = 0; iy < L[1]; ++iy)
       for (int ix = 0; ix < L[0]; ++ix) {
         const int ii[3] = {ix, iy, iz};
-        bool in = true;
-        for (int d = 0; d < 3; ++d) in = in && ii[d] >= p.refine_lo[d] && ii[d] < p.refine_hi[d];
-        if (in) {
-          for (int dz = 0; dz < 2; ++dz)
-            for (int dy = 0; dy < 2; ++dy)
-              for (int dx = 0; dx < 2; ++dx)
-                cubes.push_back({{2 * ix + dx, 2 * iy + dy, 2 * iz + dz}, 1});
-        } else {
-          cubes.push_back({{2 * ix, 2 * iy, 2 * iz}, 2});
+        if (!in_box(ii, p.refine_lo, p.refine_hi)) {
+          cubes.push_back({{S * ix, S * iy, S * iz}, S});
+          continue;
         }
+        for (int dz = 0; dz < 2; ++dz)
+          for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+              const int jj[3] = {2 * ix + dx, 2 * iy + dy, 2 * iz + dz};   // doubled lattice
+              const int s = S / 2;
+              if (!second || !in_box(jj, p.refine2_lo, p.refine2_hi)) {
+                cubes.push_back({{s * jj[0], s * jj[1], s * jj[2]}, s});
+                continue;
+              }
+              for (int ez = 0; ez < 2; ++ez)
+                for (int ey = 0; ey < 2; ++ey)
+                  for (int ex = 0; ex < 2; ++ex)
+                    cubes.push_back({{s * jj[0] + ex, s * jj[1] + ey, s * jj[2] + ez}, 1});
+            }
       }
   const int ncube = (int)cubes.size();
   for (int c = 1; c < ncube; ++c) {
@@ -801,9 +821,13 @@ int two_level_lattice(const CaseParams& p, Case& cs) {
       const int t0 = A == 0 ? 1 : 0, t1 = A == 2 ? 1 : 2;
       const int pa = plus ? q.x0[A] + q.s : q.x0[A] - 1;
       if (pa < 0 || pa >= F[A]) continue;
+      // The neighbour at each quadrant of the face (u0 on the first
+      // tangential axis fastest): one cube of the same or twice the size
+      // covers them all, or four cubes of half the size cover one each.
       int ids[4], n = 0;
-      for (int u1 = 0; u1 < q.s; ++u1)
-        for (int u0 = 0; u0 < q.s; ++u0) {
+      const int hs = q.s > 1 ? q.s / 2 : 1;
+      for (int u1 = 0; u1 < q.s; u1 += hs)
+        for (int u0 = 0; u0 < q.s; u0 += hs) {
           int pos[3];
           pos[A] = pa;
           pos[t0] = q.x0[t0] + u0;
@@ -811,11 +835,13 @@ int two_level_lattice(const CaseParams& p, Case& cs) {
           ids[n++] = own(pos[0], pos[1], pos[2]);
         }
       int* a = &cs.adj_number[(size_t)c * 35];
+      const int s0 = cubes[ids[0]].s;
       if (n == 1 || (ids[0] == ids[1] && ids[0] == ids[2] && ids[0] == ids[3])) {
+        if (s0 != q.s && s0 != 2 * q.s) return 1;   // more than one level apart
         a[1 * 7 + dir] = ids[0];
       } else {
         for (int m = 0; m < 4; ++m) {
-          if (cubes[ids[m]].s != 1) return 1;   // neither one neighbour nor four small ones
+          if (cubes[ids[m]].s != hs) return 1;   // neither one neighbour nor four half-size ones
           a[(1 + m) * 7 + dir] = ids[m];
         }
       }
@@ -848,9 +874,21 @@ int build_case(const CaseParams& p, Case& cs, std::string& err, IbDevice* dev) {
       return 1;
     }
   }
+  bool refined2 = true;
+  for (int d = 0; d < 3; ++d) {
+    if (p.refine2_hi[d] <= p.refine2_lo[d]) refined2 = false;
+    if (p.refine2_lo[d] < 0 || p.refine2_hi[d] > 2 * p.lattice[d]) {
+      err = "second refinement box outside the doubled lattice";
+      return 1;
+    }
+  }
+  if (refined2 && !refined) {
+    err = "a second refinement box needs a first one";
+    return 1;
+  }
   if (refined) {
     if (two_level_lattice(p, cs)) {
-      err = "refined lattice could not be built";
+      err = "refined lattice could not be built (neighbouring cubes more than one level apart)";
       return 1;
     }
   } else {

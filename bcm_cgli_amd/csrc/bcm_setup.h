@@ -29,6 +29,10 @@ struct CaseParams {
   // along every axis are each split into 8 cubes of half the edge (a two-level
   // grid); no refinement when the box is empty on some axis.
   int refine_lo[3], refine_hi[3];
+  // Optional second refinement inside the first: the half-edge cubes whose
+  // index on the doubled lattice lies in [refine2_lo, refine2_hi) are split
+  // again (a three-level grid).  Neighbouring cubes may differ by one level.
+  int refine2_lo[3], refine2_hi[3];
 };
 
 // Immersed-boundary table of one list (minus or plus) in the 1-based flat

@@ -282,6 +282,11 @@ class Server:
             "inner_iters_per_step": (sum(r.inner_iters for r in ok) /
                                      max(sum(r.steps for r in ok), 1)),
             "max_resident": max_resident,
+            "mean_resident": (float(np.mean([n for _, n, _ in self.trace]))
+                              if self.trace else 0.0),
+            "mean_resident_cubes": (float(np.mean([c for _, _, c in self.trace]))
+                                    if self.trace else 0.0),
+            "busy_fraction": busy / wall if wall > 0 else 0.0,
             "pool_cubes": self.cap_cubes,
         }
 
@@ -333,41 +338,68 @@ def measure_capacity(Engine, Case, reqs, precision, n_inner, n_probe=16, steps=2
     return sum(c.cells() for c in cases) * n_inner * steps / dt
 
 
-def bench(args, metric):
-    """The serving workload of bench.py: one JSON record.  Cases are built on
-    host threads with the IB surface search on the GPU."""
-    import sys
+def serve_run(n_requests, max_steps, n_inner, rate=None, capacity=None, inner_tol=None,
+              seed=20260503, log=None):
+    """Serves a seeded Poisson stream of the serving workload's requests
+    (mixed precision, 64 GiB HBM budget, case setup on 4 host threads with
+    the IB search on the GPU).  The arrival rate, unless given, offers 0.8 of
+    the batched capacity (cell-iterations per second: given, or measured on
+    the stream's first 16 requests) to requests running the full step limit.
+    Returns (statistics, configuration)."""
     from . import Case, Engine, bytes_per_cube
 
     def make_case(p):
         return Case(p, device=0)
     prec = "mixed"
-    n_inner = args.inner
-    shape = poisson_requests(args.requests, 1.0)
-    if args.rate:
-        rate, capacity = args.rate, None
-    else:
-        # Offered load 0.8 of the measured batched capacity.
-        capacity = measure_capacity(Engine, make_case, shape, prec, n_inner)
-        mean_cells = np.mean([np.prod(p["lattice"]) * 4096 for _, p in shape])
-        rate = 0.8 * capacity / (mean_cells * n_inner * args.max_steps)
+    shape = poisson_requests(n_requests, 1.0, seed=seed)
+    mean_cells = float(np.mean([np.prod(p["lattice"]) * 4096 for _, p in shape]))
+    if rate is None:
+        if capacity is None:
+            capacity = measure_capacity(Engine, make_case, shape, prec, n_inner)
+        rate = 0.8 * capacity / (mean_cells * n_inner * max_steps)
     reqs = [(t / rate, p) for t, p in shape]
-    inner_tol = getattr(args, "inner_tol", None)
+    eng = Engine(0)
+
+    def progress(done, total):
+        if log and (done % 8 == 0 or done == total):
+            log(f"{done}/{total} requests finished")
+    srv = Server(eng, make_case, precision=prec, budget_bytes=64 << 30,
+                 bytes_per_cube=bytes_per_cube(prec), max_cases=64, n_inner=n_inner,
+                 max_steps=max_steps, tol=1e-8, inner_tol=inner_tol, progress=progress)
+    for t, p in reqs:
+        srv.submit(p, t)
+    if log:
+        log(f"serving {len(reqs)} requests at {rate:.3f}/s")
+    try:
+        st = srv.run()
+    finally:
+        eng.close()
     inner_desc = (f"{n_inner} inner iterations" if inner_tol is None else
                   f"at most {n_inner} inner iterations (the reference's inner control, tolerance "
                   f"{inner_tol:g})")
-    eng = Engine(0)
-    def progress(done, total):
-        if done % 10 == 0 or done == total:
-            print(f"[bench] {done}/{total} requests finished", file=sys.stderr, flush=True)
-    srv = Server(eng, make_case, precision=prec, budget_bytes=64 << 30,
-                 bytes_per_cube=bytes_per_cube(prec), max_cases=64, n_inner=n_inner,
-                 max_steps=args.max_steps, tol=1e-8, inner_tol=inner_tol, progress=progress)
-    for t, p in reqs:
-        srv.submit(p, t)
-    print(f"[bench] serving {len(reqs)} requests at {rate:.3f}/s", file=sys.stderr, flush=True)
-    st = srv.run()
-    eng.close()
+    cfg = {
+        "workload": f"serving: {len(reqs)} requests, Poisson arrivals at {rate:.3f}/s, cubes "
+                    "uniform in [27,1000] (near-cubic lattices), M in [0.1,0.5], Re in "
+                    f"[50,1000], icosphere level 3-5, until inner residual < 1e-8 or "
+                    f"{max_steps} steps of {inner_desc}, mixed precision, HBM budget 64 GiB, "
+                    "case setup on 4 host threads with the IB search on the GPU",
+        "requests": len(reqs),
+        "rate_per_s": rate,
+        "capacity_mcell_iter_s": None if capacity is None else capacity / 1e6,
+        "offered_load_at_step_limit": 0.8 if capacity is not None else None,
+        "mean_cells_per_request": mean_cells,
+    }
+    return st, cfg
+
+
+def bench(args, metric):
+    """The serving workload of bench.py: one JSON record."""
+    import sys
+
+    def log(m):
+        print(f"[bench] {m}", file=sys.stderr, flush=True)
+    st, cfg = serve_run(args.requests, args.max_steps, args.inner, rate=args.rate,
+                        inner_tol=getattr(args, "inner_tol", None), log=log)
     return {
         "metric": metric,
         "value": round(st["mcell_iter_s"], 3),
@@ -381,15 +413,6 @@ def bench(args, metric):
         "vs_baseline": None,
         "dtype": "f32 stages + f64 time levels",
         "data": "synthetic (generated sphere cases, seeded Poisson request stream)",
-        "config": {
-            "workload": f"serving: {len(reqs)} requests, Poisson arrivals at {rate:.3f}/s, cubes "
-                        "uniform in [27,1000] (near-cubic lattices), M in [0.1,0.5], Re in "
-                        f"[50,1000], icosphere level 3-5, until inner residual < 1e-8 or "
-                        f"{args.max_steps} steps of {inner_desc}, mixed precision, "
-                        f"HBM budget 64 GiB, case setup on 4 host threads with the IB search on "
-                        f"the GPU",
-            "rate_per_s": rate,
-            "measured_capacity_mcell_iter_s": None if capacity is None else capacity / 1e6,
-        },
+        "config": cfg,
         "serving": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()},
     }

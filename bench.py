@@ -21,8 +21,14 @@ The same line also reports, under "forms":
                {2,3,4,5,6,8}; M, Re and sphere position drawn from a seed) in
                one case pool, 50 steps of 10 inner iterations, every launch
                covering all cases; the mixed form with per-case errors
-               against the FP64 form.
---workload serve runs the serving loop instead (see bcm_cgli_amd/serve.py).
+               against the FP64 form;
+  serve_cfg5   the serving workload on a bounded sample: 48 Poisson requests
+               (cube counts uniform in [27, 1000], icosphere level 3-5, M and
+               Re as above, mixed precision) until the inner residual is below
+               1e-8 or 300 steps, arriving at a rate that offers 0.8 of the
+               batch_mixed throughput to requests running all 300 steps;
+               p50/p99 time-to-solution, sustained and busy Mcell-iter/s.
+--workload serve runs the serving loop alone (see bcm_cgli_amd/serve.py).
 """
 import argparse
 import json
@@ -59,6 +65,9 @@ def parse_args():
     ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--flux-reps", type=int, default=30)
+    ap.add_argument("--serve-requests", type=int, default=48,
+                    help="requests of the serving form (0: skip it)")
+    ap.add_argument("--serve-max-steps", type=int, default=300)
     ap.add_argument("--requests", type=int, default=200)
     ap.add_argument("--rate", type=float, default=None, help="serve: arrivals per second")
     ap.add_argument("--max-steps", type=int, default=300)
@@ -347,7 +356,21 @@ def main():
             "cd_fp64_per_case": [round(B.drag_coefficient(e[5], q["sphere_diameter"]), 6)
                                  for e, q in zip(b64["er"], params)],
             "roofline": roofline("mixed", bmx["cells"], bmx["flux_ms"], "batch"),
+            "flux_choice": bmx["choice"],
         }
+        if args.serve_requests > 0:
+            from bcm_cgli_amd import serve
+            st, scfg = serve.serve_run(args.serve_requests, args.serve_max_steps, n_inner,
+                                       capacity=bmx["value"] * 1e6, log=log)
+            rnd = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()}
+            f["serve_cfg5"] = {
+                "value": round(st["mcell_iter_s"], 3), "unit": "Mcell-iter/s",
+                "value_busy": round(st["mcell_iter_s_busy"], 3),
+                "busy_speedup_vs_plain_fp64_cfg2": round(st["mcell_iter_s_busy"] / ref["value"], 3),
+                "tts_p50_s": rnd["tts_p50_s"], "tts_p99_s": rnd["tts_p99_s"],
+                "config": {**scfg, "capacity_source": "batch_mixed value (64-case pool)"},
+                "serving": rnd,
+            }
         out["forms"] = f
     if not args.no_cpu_baseline:
         log("CPU baseline sample")

@@ -147,6 +147,8 @@ typedef struct bcm_case_params {
   double noise_amp;           /* relative density noise amplitude */
   int refine_lo[3];           /* lattice cubes with index in [refine_lo, refine_hi) */
   int refine_hi[3];           /* on every axis are split into 8 (a two-level grid) */
+  int refine2_lo[3];          /* their children with index on the doubled lattice in */
+  int refine2_hi[3];          /* [refine2_lo, refine2_hi) are split again (three levels) */
 } bcm_case_params;
 
 typedef struct bcm_case bcm_case;
@@ -272,6 +274,12 @@ int bcm_engine_set_flux(bcm_engine* e, int variant, int nseg, int fuse);
 int bcm_engine_autotune(bcm_engine* e, int* out3);
 /* Ghost schedule statistics: entries, cubes with entries, maximum levels. */
 int bcm_engine_ghost_stats(bcm_engine* e, int* stats3);
+/* The IB fill of a cube runs out of LDS when its value slots fit the limit
+ * (at most 1536, the default), else in place on memory; a lower limit,
+ * applied to cases loaded or added afterwards, puts more cubes on the memory
+ * schedule (bitwise the same results).  Not a reference interface: a test
+ * and diagnostic hook. */
+int bcm_engine_set_ghost_lds_limit(bcm_engine* e, int slots);
 
 /* ------------------------------------------------------------------------ */
 /* 4. Case pool: several independent cases resident on one device.           */

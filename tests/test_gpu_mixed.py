@@ -41,7 +41,7 @@ def accuracy(case, st, er, ref_state, ref_er):
     return l2, cd_err
 
 
-@pytest.mark.parametrize("name", ["tiny", "cfg1", "ml_tiny"])
+@pytest.mark.parametrize("name", ["tiny", "cfg1", "ml_tiny", "ml3"])
 def test_mixed_against_oracle(B, O, name):
     case = B.Case(B.case_params(name))
     n_steps, n_inner = STEPPING[name]
@@ -90,6 +90,44 @@ def test_mixed_pool_matches_alone(B):
             for n, a in st.items():
                 assert np.array_equal(e.case_state(s, n), a), (s, n)
             assert np.array_equal(er[:, s, :], er1), s
+    finally:
+        e.close()
+
+
+@pytest.mark.parametrize("name", ["cfg1", "ml_tiny"])
+def test_mixed_runs_per_cube_are_bitwise(B, name):
+    # The mixed flux kernel marching each cube in 1, 2 or 4 runs of planes
+    # (the autotuner's choices) gives the same bits: a run that starts inside
+    # the cube forms its first x-face with the same instructions the
+    # unsegmented march uses, and the norm partials do not depend on the runs.
+    case = B.Case(B.case_params(name))
+    out = []
+    for nseg in (1, 2, 4):
+        e = B.Engine(0)
+        try:
+            e.alloc_pool(case.ncube - 1, 1, "mixed")
+            s = e.add_case(case)
+            e.set_flux(2, nseg, 0)
+            er = e.run(2, 3)[:, s, :]
+            out.append(({n: e.case_state(s, n) for n in ("U1_", "U1", "U1q")}, er))
+        finally:
+            e.close()
+    for nseg, (st, er) in zip((2, 4), out[1:]):
+        for n, a in st.items():
+            assert np.array_equal(a.view(np.int64), out[0][0][n].view(np.int64)), (nseg, n)
+        assert np.array_equal(er.view(np.int64), out[0][1].view(np.int64)), nseg
+
+
+def test_mixed_autotune_picks_runs_per_cube(B):
+    case = B.Case(B.case_params("tiny"))
+    e = B.Engine(0)
+    try:
+        e.alloc_pool(case.ncube - 1, 1, "mixed")
+        e.add_case(case)
+        choice = e.autotune()
+        assert choice["nseg"] in (1, 2, 4)
+        with pytest.raises(B.BCMError, match="nseg"):
+            e.set_flux(2, 8, 0)
     finally:
         e.close()
 

@@ -67,7 +67,7 @@ def test_each_kernel_bitwise(B, O, engine, name):
     engine.part("shift"); orc.shift(); compare(engine, orc, STATE)
 
 
-@pytest.mark.parametrize("name", ["tiny", "cfg1", "ml_tiny", "ml"])
+@pytest.mark.parametrize("name", ["tiny", "cfg1", "ml_tiny", "ml", "ml3"])
 def test_run_matches_reference_fixture(B, engine, name):
     g = golden(name)
     case = B.Case(B.case_params(name))
@@ -99,7 +99,7 @@ def test_flux_variants_bitwise(B, O, engine, variant, nseg, fuse, name):
     assert er_close(h, ho)
 
 
-@pytest.mark.parametrize("name", ["tiny", "cfg1", "ml_tiny", "ml"])
+@pytest.mark.parametrize("name", ["tiny", "cfg1", "ml_tiny", "ml", "ml3"])
 @pytest.mark.parametrize("use_graph", [True, False])
 def test_push_stages_match_reference_fixture(B, engine, name, use_graph):
     # Stages whose flux kernel writes the face halos into the other copy of
@@ -330,3 +330,37 @@ def test_dropin_main_loop_matches_reference_fixture(B, tmp_path):
     for f in ("BCMdata00001.x", f"qBCMdata{n_steps:05d}.q"):
         with open(tmp_path / f, "rb") as fh:
             assert hashlib.sha256(fh.read()).hexdigest() == g["file_sha256"][f], f
+
+
+@pytest.mark.parametrize("limit", [0, 40])
+def test_ib_fill_memory_schedule_bitwise(B, O, engine, limit):
+    # A cube whose IB fill needs more value slots than fit in LDS takes the
+    # in-place schedule on memory (levels ordered by writes after reads as
+    # well).  Lowering the slot limit puts every IB cube (limit 0), or the
+    # larger ones (limit 40), on that schedule: the run is still bitwise the
+    # oracle, and the mixed path gives the same bits as with its LDS fill.
+    case = B.Case(B.case_params("cfg1"))
+    engine.load_case(case)
+    levels_lds = engine.ghost_stats()["levels"]
+    try:
+        engine.set_ghost_lds_limit(limit)
+        engine.load_case(case)
+        stats = engine.ghost_stats()
+        if limit == 0:
+            assert stats["levels"] > levels_lds
+        orc = O.OracleRun(case, nthreads=16)
+        h = engine.run(2, 3)
+        ho = orc.run(2, 3)
+        compare(engine, orc, STATE)
+        assert er_close(h, ho)
+        mixed = []
+        for lim in (1536, limit):
+            engine.set_ghost_lds_limit(lim)
+            engine.alloc_pool(case.ncube - 1, 1, "mixed")
+            s = engine.add_case(case)
+            er = engine.run(2, 3)[:, s, :]
+            mixed.append((engine.case_state(s, "U1_"), er))
+        assert np.array_equal(mixed[0][0].view(np.int64), mixed[1][0].view(np.int64))
+        assert np.array_equal(mixed[0][1].view(np.int64), mixed[1][1].view(np.int64))
+    finally:
+        engine.set_ghost_lds_limit(1536)

@@ -50,10 +50,11 @@ def test_oracle_cfg1_bitwise(B, O):
     assert np.array_equal(hist.view(np.int64), g["er"].view(np.int64))
 
 
-@pytest.mark.parametrize("name", ["ml_tiny", "ml"])
+@pytest.mark.parametrize("name", ["ml_tiny", "ml", "ml3"])
 def test_oracle_two_level_bitwise(B, O, name):
-    # Two-level grids: injection into small cubes' ghosts and 8-cell means
-    # into big cubes' ghosts across every size change.
+    # Grids of two and three cube sizes (ml3: edges 0.25, 0.125, 0.0625, the
+    # finest block inside the middle one): injection into small cubes' ghosts
+    # and 8-cell means into big cubes' ghosts across every size change.
     g = golden(name)
     orc, hist, stages = run_oracle(B, O, name, stage_hashes=True)
     for k, v in stages.items():

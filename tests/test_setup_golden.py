@@ -10,7 +10,7 @@ import pytest
 
 from helpers import golden, golden_ib, sha
 
-CASES = ["tiny", "cfg1", "ml_tiny", "ml"]
+CASES = ["tiny", "cfg1", "ml_tiny", "ml", "ml3"]
 
 
 @pytest.fixture(scope="module", params=CASES)
