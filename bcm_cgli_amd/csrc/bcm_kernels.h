@@ -244,10 +244,11 @@ void launch_update(hipStream_t s, int RK, double* U, double* U1p1, double* U1p2,
                    const double* Rku, const double* Res, const int8_t* fws, int ncube,
                    const int* cube_case, double* partials);
 // Norms er[0..7] of each case in cases[0..ncases): rows_per_cube partial rows
-// per cube (1 after the fused flux kernel, 16 after k_update); case d writes
-// er[d.slot*8 .. d.slot*8+7].
-void launch_reduce(hipStream_t s, const double* partials, int rows_per_cube,
-                   const CaseDesc* cases, int ncases, const DevPhys* phys, double* er);
+// per cube (1 after the fused FP64 flux kernels, 16 after k_update and the
+// mixed flux kernel; several rows are first summed in row order into each
+// cube's first row); case d writes er[d.slot*8 .. d.slot*8+7].
+void launch_reduce(hipStream_t s, double* partials, int rows_per_cube, const CaseDesc* cases,
+                   int ncases, const DevPhys* phys, double* er);
 void launch_shift(hipStream_t s, const double* U, double* U1, double* U1q, int ncube,
                   const int* cube_case);
 // AoS <-> SoA transposes of ncube whole cubes (cube 0 of both arrays included).

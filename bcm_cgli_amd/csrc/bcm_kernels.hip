@@ -1101,23 +1101,42 @@ __global__ __launch_bounds__(WG) void k_update(int RK, double* __restrict__ U,
   }
 }
 
+// With several partial rows per cube, each cube's rows are first summed in
+// row order into its first row, one thread per cube (blockIdx.y: case).
+__global__ __launch_bounds__(WG) void k_fold_rows(double* __restrict__ partials,
+                                                  int rows_per_cube,
+                                                  const CaseDesc* __restrict__ cases) {
+  const CaseDesc cd = cases[blockIdx.y];
+  for (int q = blockIdx.x * WG + threadIdx.x; q < cd.n; q += gridDim.x * WG) {
+    double* r = partials + (size_t)(cd.first - 1 + q) * rows_per_cube * 8;
+    double acc[7];
+#pragma unroll
+    for (int m = 0; m < 7; ++m) acc[m] = r[m];
+    for (int b = 1; b < rows_per_cube; ++b)
+#pragma unroll
+      for (int m = 0; m < 7; ++m) acc[m] += r[b * 8 + m];
+#pragma unroll
+    for (int m = 0; m < 7; ++m) r[m] = acc[m];
+  }
+}
+
 // Final norm reduction (Flux.cpp:2627-2652 at one rank): er[1..7] of one
-// case per block.  The partial rows of a case are summed in an order fixed
-// relative to the case's first row, so a case gives the same bits alone and
-// next to other cases.
+// case per block from one partial row per cube (every stride-th row).  The
+// rows of a case are summed in an order fixed relative to the case's first
+// cube, so a case gives the same bits alone and next to other cases.
 __global__ __launch_bounds__(WG) void k_reduce(const double* __restrict__ partials,
-                                               int rows_per_cube,
+                                               int stride,
                                                const CaseDesc* __restrict__ cases,
                                                const DevPhys* __restrict__ phys,
                                                double* __restrict__ er_all) {
   const CaseDesc cd = cases[blockIdx.x];
   const DevPhys ph = phys[cd.slot];
-  const double* rows = partials + (size_t)(cd.first - 1) * rows_per_cube * 8;
-  const int nrows = cd.n * rows_per_cube;
+  const double* rows = partials + (size_t)(cd.first - 1) * stride * 8;
+  const int nrows = cd.n;
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   for (int bi = threadIdx.x; bi < nrows; bi += WG)
 #pragma unroll
-    for (int m = 0; m < 7; ++m) acc[m] += rows[(size_t)bi * 8 + m];
+    for (int m = 0; m < 7; ++m) acc[m] += rows[(size_t)bi * stride * 8 + m];
   __shared__ double sh[(WG / 64) * 7];
   block_sum<7>(acc, sh);
   if (threadIdx.x == 0) {
@@ -1823,11 +1842,14 @@ void launch_update(hipStream_t s, int RK, double* U, double* U1p1, double* U1p2,
                        Res, fws, cube_case, partials);
 }
 
-void launch_reduce(hipStream_t s, const double* partials, int rows_per_cube,
-                   const CaseDesc* cases, int ncases, const DevPhys* phys, double* er) {
-  if (ncases > 0)
-    hipLaunchKernelGGL(k_reduce, dim3(ncases), dim3(WG), 0, s, partials, rows_per_cube, cases,
-                       phys, er);
+void launch_reduce(hipStream_t s, double* partials, int rows_per_cube, const CaseDesc* cases,
+                   int ncases, const DevPhys* phys, double* er) {
+  if (ncases <= 0) return;
+  if (rows_per_cube > 1)
+    hipLaunchKernelGGL(k_fold_rows, dim3(8, ncases), dim3(WG), 0, s, partials, rows_per_cube,
+                       cases);
+  hipLaunchKernelGGL(k_reduce, dim3(ncases), dim3(WG), 0, s, partials, rows_per_cube, cases, phys,
+                     er);
 }
 
 void launch_shift(hipStream_t s, const double* U, double* U1, double* U1q, int ncube,

@@ -65,6 +65,8 @@ def parse_args():
     ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--flux-reps", type=int, default=30)
+    ap.add_argument("--mixed-nseg", type=int, default=None, choices=(1, 2, 4),
+                    help="pin the mixed flux kernel's runs per cube (default: autotuned)")
     ap.add_argument("--serve-requests", type=int, default=48,
                     help="requests of the serving form (0: skip it)")
     ap.add_argument("--serve-max-steps", type=int, default=300)
@@ -110,6 +112,10 @@ def read_valu(kernel):
     return int(m.group(1)) if m else None
 
 
+# Runs per cube of the mixed flux kernel when pinned (--mixed-nseg); None: autotuned.
+MIXED_NSEG = None
+
+
 def run_pool(B, cases, precision, steps, warmup, n_inner, flux_reps, keep_state):
     """Loads cases into one pool, autotunes, warms up, times `steps` steps.
     Returns timing, the last inner iteration's er per case and (keep_state)
@@ -118,6 +124,9 @@ def run_pool(B, cases, precision, steps, warmup, n_inner, flux_reps, keep_state)
     eng.alloc_pool(sum(c.ncube - 1 for c in cases), len(cases), precision)
     slots = [eng.add_case(c) for c in cases]
     choice = eng.autotune()
+    if precision == "mixed" and MIXED_NSEG is not None:
+        eng.set_flux(2, MIXED_NSEG, 0)
+        choice = {"variant": 2, "nseg": MIXED_NSEG, "fuse": 2, "pinned": True}
     if warmup > 0:
         eng.run(warmup, n_inner, record_er=False)
     eng.sync()
@@ -244,6 +253,8 @@ def cpu_baseline_reference(case, params, n_threads, config):
 
 def main():
     args = parse_args()
+    global MIXED_NSEG
+    MIXED_NSEG = args.mixed_nseg
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
               "cases are independent and would shard by cube or by case",
@@ -316,6 +327,7 @@ def main():
                              "of the last inner iteration",
             "steps_compared": args.warmup + args.steps,
             "roofline": roofline("mixed", cells, mix["flux_ms"], args.config),
+            "flux_choice": mix["choice"],
             "kernels_ms": {"flux_update": round(mix["flux_ms"], 4),
                            "stage": round(mix["stage_ms"], 4), **mix["parts_ms"]},
         }
