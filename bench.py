@@ -100,7 +100,7 @@ VALU_PEAK_TLANE = 256 * 4 * 16 * 2.4e9 / 1e12
 
 def read_valu(kernel):
     """VALU instructions per cell-stage of a flux kernel from the committed
-    SQ counter summary (SQ_INSTS_VALU per wave / 1,024 cell-stages)."""
+    SQ counter summary (SQ_INSTS_VALU per wave x 64 lanes / 1,024 cell-stages)."""
     path = os.path.join(ROOT, "profiles", "flux_cfg2_counters.txt")
     if not os.path.exists(path):
         return None
