@@ -80,7 +80,7 @@ void free_case(Engine* e) {
   free_tables(e);
   void* ptrs[] = {e->U,   e->U1,        e->U1q,      e->U1p1,     e->U1p2,   e->Rku,
                   e->Res, e->staging,   e->fws,      e->cube_size, e->partials, e->er_dev,
-                  e->phys_dev, e->cube_case, e->U2};
+                  e->phys_dev, e->cube_case, e->U2, e->Bsrc};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (e->Uf) (void)hipFree(e->Uf);
@@ -88,6 +88,7 @@ void free_case(Engine* e) {
   e->Uf = e->Uf2 = nullptr;
   if (e->er_host) (void)hipHostFree(e->er_host);
   e->U = e->U1 = e->U1q = e->U1p1 = e->U1p2 = e->Rku = e->Res = e->staging = e->U2 = nullptr;
+  e->Bsrc = nullptr;
   e->fws = nullptr;
   e->cube_size = nullptr;
   e->partials = e->er_dev = e->er_host = nullptr;
@@ -249,7 +250,7 @@ void ghost_free(GhostDevice& d) {
 }
 
 size_t engine_bytes_per_cube(int prec) {
-  const size_t state = prec == 1 ? (size_t)CUBE_STRIDE * (2 * sizeof(float) + 3 * sizeof(double))
+  const size_t state = prec == 1 ? (size_t)CUBE_STRIDE * (2 * sizeof(float) + 4 * sizeof(double))
                                  : (size_t)9 * CUBE_STRIDE * sizeof(double);
   return state + PLANE + sizeof(double) + sizeof(int) + (size_t)16 * 8 * sizeof(double);
 }
@@ -316,7 +317,7 @@ int engine_alloc(Engine* e, int cap, int max_cases, std::string& err, int prec) 
     if (dev_alloc(&e->Uf, ncells * 5, err) || dev_alloc(&e->Uf2, ncells * 5, err)) return 1;
     HIP_TRY(hipMemsetAsync(e->Uf, 0, ncells * 5 * sizeof(float), e->stream));
     HIP_TRY(hipMemsetAsync(e->Uf2, 0, ncells * 5 * sizeof(float), e->stream));
-    for (double** p : {&e->U1, &e->U1q, &e->staging}) {
+    for (double** p : {&e->U1, &e->U1q, &e->staging, &e->Bsrc}) {
       if (dev_alloc(p, ncells * 5, err)) return 1;
       HIP_TRY(hipMemsetAsync(*p, 0, bytes, e->stream));
     }
@@ -785,6 +786,9 @@ int engine_case_state(Engine* e, int slot, int which, double* aos, bool upload,
       launch_copy_f(e->stream, dstf, e->Uf2 + (size_t)S.first * CUBE_STRIDE, n * CUBE_STRIDE);
     if (which == 0 && !f32 && e->U2)
       launch_copy_d(e->stream, dst, e->U2 + (size_t)S.first * CUBE_STRIDE, n * CUBE_STRIDE);
+    // The mixed stages read the dual-time source constant formed from U1, U1q.
+    if (e->prec == 1 && which != 0)
+      launch_bsrc_mixed(e->stream, e->U1, e->U1q, e->Bsrc, S.first, (int)n, e->pt());
   } else {
     double ref[5];
     ref_state(S.tb.ph, ref);
@@ -926,7 +930,7 @@ bool engine_fused(const Engine* e) {
 
 void engine_flux_update(Engine* e, int RK, int er_slot) {
   if (e->prec == 1) {
-    launch_flux_mixed(e->stream, e->Uf, e->Uf2, e->halo, e->bc_mask_cube, e->U1, e->U1q, e->fws,
+    launch_flux_mixed(e->stream, e->Uf, e->Uf2, e->halo, e->bc_mask_cube, e->Bsrc, e->fws,
                       e->cube_size, e->ncube, e->pt(), e->deltaT, e->deltaTau, e->e, RK,
                       e->partials, e->mixed_nseg);
     std::swap(e->Uf, e->Uf2);
@@ -1106,7 +1110,7 @@ void engine_update(Engine* e, int RK, int er_slot) {
 
 void engine_shift(Engine* e) {
   if (e->prec == 1)
-    launch_shift_mixed(e->stream, e->Uf, e->U1, e->U1q, e->ncube, e->pt());
+    launch_shift_mixed(e->stream, e->Uf, e->U1, e->U1q, e->Bsrc, e->ncube, e->pt());
   else
     launch_shift(e->stream, e->U, e->U1, e->U1q, e->ncube, e->cube_case);
 }
@@ -1262,7 +1266,7 @@ void shift_case(Engine* e, const CaseSlot& S) {
   const size_t off = (size_t)(S.first - 1) * CUBE_STRIDE;
   int* cc = e->cube_case + (S.first - 1);
   if (e->prec == 1)
-    launch_shift_mixed(e->stream, e->Uf + off, e->U1 + off, e->U1q + off, S.tb.n + 1,
+    launch_shift_mixed(e->stream, e->Uf + off, e->U1 + off, e->U1q + off, e->Bsrc + off, S.tb.n + 1,
                        PhysTab{e->phys_dev, cc});
   else
     launch_shift(e->stream, e->U + off, e->U1 + off, e->U1q + off, S.tb.n + 1, cc);

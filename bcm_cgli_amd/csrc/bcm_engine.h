@@ -101,6 +101,10 @@ struct Engine {
   // U1q, staging, flags and sizes; U, U1p1, U1p2, Rku and Res stay null.
   double *U = nullptr, *U1 = nullptr, *U1q = nullptr, *U1p1 = nullptr, *U1p2 = nullptr;
   double *Rku = nullptr, *Res = nullptr, *staging = nullptr;
+  // Mixed path: the dual-time source constant 3 Uref - 4 U1 + U1q of every
+  // interior cell (FP64), formed when U1 and U1q change (the physical-time
+  // shift, uploads) instead of in every stage.
+  double* Bsrc = nullptr;
   double* U2 = nullptr;         // second copy of the FP64 iterate (flux with halo push)
   int8_t* fws = nullptr;
   double* cube_size = nullptr;

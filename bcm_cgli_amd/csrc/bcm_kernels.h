@@ -258,7 +258,9 @@ int update_blocks(int ncube);
 
 // ---- Mixed-precision path (bcm_kernels_mixed.hip) ----
 // Inner iterate U in FP32 SoA as the deviation from ref_state of the cube's
-// case; U1, U1q (physical time levels) in FP64 as full states.
+// case; U1, U1q (physical time levels) in FP64 as full states; Bsrc, the
+// dual-time source constant 3 Uref - 4 U1 + U1q of every interior cell (FP64),
+// which the stages read in place of U1 and U1q.
 // One workgroup per cube: flux, residual, preconditioned increment and the RK
 // update of stage RK, reading Uin and writing every interior cell to Uout
 // (updated fluid cells, other cells unchanged).  Cells of the two layers
@@ -269,15 +271,19 @@ int update_blocks(int ncube);
 // MIXED_ROWS_PER_CUBE rows of 8 norm partials per cube.
 constexpr int MIXED_ROWS_PER_CUBE = 16;
 void launch_flux_mixed(hipStream_t s, const float* Uin, float* Uout, const HaloSrc* halo,
-                       const uint8_t* cube_mask, const double* U1, const double* U1q,
-                       const int8_t* fws,
+                       const uint8_t* cube_mask, const double* Bsrc, const int8_t* fws,
                        const double* cube_size, int ncube, PhysTab pt, double deltaT,
                        double deltaTau, double e, int RK, double* partials, int nseg);
 // dst[0..n) <- src[0..n).
 void launch_copy_f(hipStream_t s, const float* src, float* dst, size_t n);
-// U1q <- U1, U1 <- ref_state + U on interior cells.
-void launch_shift_mixed(hipStream_t s, const float* U, double* U1, double* U1q, int ncube,
-                        PhysTab pt);
+// U1q <- U1, U1 <- ref_state + U and Bsrc <- 3 ref_state - 4 U1 + U1q on
+// interior cells.
+void launch_shift_mixed(hipStream_t s, const float* U, double* U1, double* U1q, double* Bsrc,
+                        int ncube, PhysTab pt);
+// Bsrc <- 3 ref_state - 4 U1 + U1q on the interior cells of cubes
+// first..first+n-1 (after U1 or U1q were set from outside).
+void launch_bsrc_mixed(hipStream_t s, const double* U1, const double* U1q, double* Bsrc, int first,
+                       int n, PhysTab pt);
 // FP64 AoS full states <-> FP32 SoA deviations from the reference state ref
 // (5 doubles) of ncube whole cubes of one case.
 void launch_aos_to_soa_f(hipStream_t s, const double* aos, float* soa, int ncube,

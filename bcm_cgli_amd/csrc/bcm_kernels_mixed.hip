@@ -8,9 +8,10 @@
 // rather than to the full state, whose pressure and energy carry a large
 // constant offset at these Mach numbers.
 // FP64: the physical time levels U1 and U1q, the dual-time source
-// -(3u - 4U1 + U1q)/(2 dT) (a difference of nearly equal states), the norms
-// and force sums, and the IB interpolation (bcm_kernels.hip, FP64 arithmetic on
-// the FP32 iterate).  The algorithm is the reference's
+// -(3u - 4U1 + U1q)/(2 dT) (a difference of nearly equal states; its constant
+// part 3 Uref - 4 U1 + U1q is formed once per physical step, at the shift), the
+// norms and force sums, and the IB interpolation (bcm_kernels.hip, FP64
+// arithmetic on the FP32 iterate).  The algorithm is the reference's
 // (BCM_Flux_XYZ_Viscous_Runge_kutta.cpp:326-2655, cited as Flux.cpp); the
 // operation order is free here, so divisions become reciprocal multiplies and
 // square roots the hardware approximations.
@@ -383,8 +384,7 @@ __device__ __forceinline__ int corner_source_slot(int t) {
 
 __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
     const float* __restrict__ U, float* __restrict__ Uout, const HaloSrc* __restrict__ halo,
-    const uint8_t* __restrict__ cube_mask, const double* __restrict__ U1,
-    const double* __restrict__ U1q,
+    const uint8_t* __restrict__ cube_mask, const double* __restrict__ Bsrc,
     const int8_t* __restrict__ fws, const double* __restrict__ cube_size, PhysTab pt,
     double deltaT, double deltaTau, double e_d, int RK, double* __restrict__ partials,
     int nseg) {
@@ -590,17 +590,17 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
       }
     }
     fill_prims(i + 1, w[4], bpr);
-    // Loads for later: the next iteration's borders, this cell's time levels
-    // folded into the FP64 source constant Bv = 3 Uref - 4 U1 + U1q.
-    load_b20(i + 1, bcp, bfl);
-    load_b18(i + 2, bpr);
+    // Loads for later: this cell's FP64 source constant Bv = 3 Uref - 4 U1 +
+    // U1q (first use in the residual, after the face pass), the next
+    // iteration's borders.
     double Bv[5] = {0, 0, 0, 0, 0};
     if (!pro) {
       const size_t gb = gcol + (size_t)i * 400;
 #pragma unroll
-      for (int v = 0; v < 5; ++v)
-        Bv[v] = 3.0 * uref[v] - 4.0 * U1[gb + (size_t)v * PLANE] + U1q[gb + (size_t)v * PLANE];
+      for (int v = 0; v < 5; ++v) Bv[v] = Bsrc[gb + (size_t)v * PLANE];
     }
+    load_b20(i + 1, bcp, bfl);
+    load_b18(i + 2, bpr);
     __syncthreads();
 
     // ---- y- and z-faces of plane i: 272 + 272 faces over 256 lanes ----
@@ -859,7 +859,8 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
 
 __global__ __launch_bounds__(WG) void k_shift_mixed(const float* __restrict__ U,
                                                     double* __restrict__ U1,
-                                                    double* __restrict__ U1q, PhysTab pt) {
+                                                    double* __restrict__ U1q,
+                                                    double* __restrict__ Bsrc, PhysTab pt) {
   const int c = blockIdx.y + 1;
   const int cc = pt.cube_case[c];
   if (cc < 0) return;
@@ -872,8 +873,31 @@ __global__ __launch_bounds__(WG) void k_shift_mixed(const float* __restrict__ U,
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
     const size_t o = b + (size_t)v * PLANE;
-    U1q[o] = U1[o];
-    U1[o] = (double)U[o] + ref[v];
+    const double q = U1[o], n = (double)U[o] + ref[v];
+    U1q[o] = q;
+    U1[o] = n;
+    Bsrc[o] = 3.0 * ref[v] - 4.0 * n + q;
+  }
+}
+
+// Bsrc <- 3 Uref - 4 U1 + U1q on the interior cells of cubes first + blockIdx.y.
+__global__ __launch_bounds__(WG) void k_bsrc_mixed(const double* __restrict__ U1,
+                                                   const double* __restrict__ U1q,
+                                                   double* __restrict__ Bsrc, int first,
+                                                   PhysTab pt) {
+  const int c = first + blockIdx.y;
+  const int cc = pt.cube_case[c];
+  if (cc < 0) return;
+  double ref[5];
+  ref_state(pt.phys[cc], ref);
+  const int i = blockIdx.x + 2;
+  const int j = (threadIdx.x >> 4) + 2;
+  const int k = (threadIdx.x & 15) + 2;
+  const size_t b = (size_t)c * CUBE_STRIDE + i * 400 + j * 20 + k;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const size_t o = b + (size_t)v * PLANE;
+    Bsrc[o] = 3.0 * ref[v] - 4.0 * U1[o] + U1q[o];
   }
 }
 
@@ -914,13 +938,12 @@ __global__ __launch_bounds__(WG) void k_soa_to_aos_f(const float* __restrict__ s
 }  // namespace
 
 void launch_flux_mixed(hipStream_t s, const float* Uin, float* Uout, const HaloSrc* halo,
-                       const uint8_t* cube_mask, const double* U1, const double* U1q,
-                       const int8_t* fws, const double* cube_size, int ncube, PhysTab pt,
-                       double deltaT, double deltaTau, double e, int RK, double* partials,
-                       int nseg) {
+                       const uint8_t* cube_mask, const double* Bsrc, const int8_t* fws,
+                       const double* cube_size, int ncube, PhysTab pt, double deltaT,
+                       double deltaTau, double e, int RK, double* partials, int nseg) {
   if (ncube > 1)
     hipLaunchKernelGGL(k_flux_mixed, dim3((ncube - 1) * nseg), dim3(WG), 0, s, Uin, Uout, halo,
-                       cube_mask, U1, U1q, fws, cube_size, pt, deltaT, deltaTau, e, RK, partials,
+                       cube_mask, Bsrc, fws, cube_size, pt, deltaT, deltaTau, e, RK, partials,
                        nseg);
 }
 
@@ -931,10 +954,16 @@ void launch_copy_f(hipStream_t s, const float* src, float* dst, size_t n) {
                      src, dst, n);
 }
 
-void launch_shift_mixed(hipStream_t s, const float* U, double* U1, double* U1q, int ncube,
-                        PhysTab pt) {
+void launch_shift_mixed(hipStream_t s, const float* U, double* U1, double* U1q, double* Bsrc,
+                        int ncube, PhysTab pt) {
   if (ncube > 1)
-    hipLaunchKernelGGL(k_shift_mixed, dim3(16, ncube - 1), dim3(WG), 0, s, U, U1, U1q, pt);
+    hipLaunchKernelGGL(k_shift_mixed, dim3(16, ncube - 1), dim3(WG), 0, s, U, U1, U1q, Bsrc, pt);
+}
+
+void launch_bsrc_mixed(hipStream_t s, const double* U1, const double* U1q, double* Bsrc, int first,
+                       int n, PhysTab pt) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_bsrc_mixed, dim3(16, n), dim3(WG), 0, s, U1, U1q, Bsrc, first, pt);
 }
 
 void launch_aos_to_soa_f(hipStream_t s, const double* aos, float* soa, int ncube,

@@ -111,9 +111,11 @@ def make_server(eng, cap_cubes, **kw):
 
 
 def test_all_requests_finish_and_fifo_admission():
-    eng = FakeEngine()
+    # Arrivals far enough apart that each request is prepared (on the host
+    # threads) before the next one arrives, however loaded the host is.
+    eng = FakeEngine(step_time=0.01)
     srv = make_server(eng, 100, max_cases=8, n_inner=2, max_steps=5, tol=1e-30)
-    arrivals = [0.0, 0.001, 0.002, 0.003, 0.004]
+    arrivals = [0.0, 0.02, 0.04, 0.06, 0.08]
     for i, t in enumerate(arrivals):
         srv.submit({"cubes": 30}, t)
     st = srv.run()
@@ -194,7 +196,9 @@ def test_diverged_case_is_quarantined():
 
 def test_admission_passes_a_request_that_does_not_fit_a_bounded_number_of_times():
     def run(max_skips):
-        eng = FakeEngine()
+        # Steps long enough that every request is prepared (on the host
+        # threads) well before A's four steps end.
+        eng = FakeEngine(step_time=0.01)
         srv = make_server(eng, 100, max_cases=8, n_inner=1, max_steps=4, tol=1e-30,
                           max_skips=max_skips)
         srv.submit({"cubes": 50}, 0.0)      # A
