@@ -339,12 +339,14 @@ __device__ __forceinline__ void yz_face(const float (*cp)[400], const int8_t* fl
   for (int v = 0; v < 4; ++v) vo[v * W + g] = Lv[v];
 }
 
-// Border cells of a 20x20 plane (j or k in {0,1,18,19}): 144 of them.
+// Border cells of a 20x20 plane (j or k in {0,1,18,19}): 144 of them.  (The
+// edge rows are formed arithmetically: a table indexed by lane would live in
+// memory.)
+__device__ __forceinline__ int edge4(int e) { return e < 2 ? e : e + 16; }
 __device__ __forceinline__ int border20(int t) {
-  const int edge[4] = {0, 1, 18, 19};
-  if (t < 80) return edge[t / 20] * 20 + t % 20;
+  if (t < 80) return edge4(t / 20) * 20 + t % 20;
   const int u = t - 80;
-  return (2 + u / 4) * 20 + edge[u % 4];
+  return (2 + u / 4) * 20 + edge4(u % 4);
 }
 // Border cells of the 18x18 primitive plane (j or k in {1,18}): 68 of them,
 // as (j,k) in padded-plane numbering.
@@ -410,6 +412,9 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
   const int t = threadIdx.x;
   const int j = 2 + (t >> 6) + 4 * ((t >> 4) & 3), k = (t & 15) + 2;
   const int col = j * 20 + k;
+  // This lane's cell among the staged plane's 144 border cells (lanes t <
+  // 144) and the primitive plane's 68 border cells (lanes t < 68).
+  const int c20 = border20(t < 144 ? t : 0), c18 = border18(t < 68 ? t : 0);
   const double dxd = cube_size[c] / 16;
   const float inv = (float)(1.0 / dxd), vol = (float)(dxd * dxd * dxd);
   const float e = (float)e_d;
@@ -444,16 +449,18 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
   const double r3 = uref[3];
   const size_t gcol = (size_t)c * CUBE_STRIDE + col;
   // Own column: w[m] = plane i-3+m, fw[m] = flag of plane i-2+m (plane -1,
-  // before the first ghost plane, a placeholder no stencil uses).
+  // before the first ghost plane, is a placeholder no stencil uses).
+  // The leading plane of each (w[6], fw[4]) is loaded at the top of the
+  // iteration; its first use, the x-face, comes after the face pass.
   float w[7][5];
   int fw[5];
 #pragma unroll
-  for (int m = 0; m < 7; ++m)
+  for (int m = 0; m < 6; ++m)
 #pragma unroll
     for (int v = 0; v < 5; ++v)
       w[m][v] = start - 3 + m >= 0 ? Uc[v * PLANE + (start - 3 + m) * 400 + col] : 1.f;
 #pragma unroll
-  for (int m = 0; m < 5; ++m) fw[m] = Fc[(start - 2 + m) * 400 + col];
+  for (int m = 0; m < 4; ++m) fw[m] = Fc[(start - 2 + m) * 400 + col];
 
   // Deviations of the viscous block's primitives u, v, w and T = P/(rho R)
   // from the reference state: dT = (dP - (P0/rho0) drho) / (rho R).
@@ -478,7 +485,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
     dst[2 * PR + o] = ww;
     dst[3 * PR + o] = T;
     if (t < 68) {
-      const int cell = border18(t);
+      const int cell = c18;
       float q[5];
 #pragma unroll
       for (int m = 0; m < 5; ++m) q[m] = bq[m];
@@ -539,7 +546,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
   // border cells of the plane after it.
   auto load_b18 = [&](int p, float (&q)[5]) {
     if (t < 68 && p <= 19) {
-      const int cell = border18(t);
+      const int cell = c18;
       bool neg;
       const int src = zsrc(p * 400 + cell, neg);
 #pragma unroll
@@ -548,7 +555,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
   };
   auto load_b20 = [&](int p, float (&q)[5], int& f) {
     if (t < 144 && p <= 17) {
-      const int cell = border20(t);
+      const int cell = c20;
       bool neg;
       const int src = zsrc(p * 400 + cell, neg);
 #pragma unroll
@@ -576,13 +583,18 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
   for (int i = start; i <= i1; ++i) {
     const int b = i & 1;
     const bool pro = i < i0;    // prologue: only the x-face i + 1/2
+    // Planes beyond the last ghost plane repeat it (a finite placeholder no
+    // stencil uses).
+#pragma unroll
+    for (int v = 0; v < 5; ++v) w[6][v] = Uc[v * PLANE + min(i + 3, 19) * 400 + col];
+    fw[4] = Fc[(i + 2) * 400 + col];
     // ---- stage plane i (conserved, flags) and primitives of plane i+1 ----
     if (!pro) {
 #pragma unroll
       for (int v = 0; v < 5; ++v) cp[b][v][col] = w[3][v];
       fl[b][col] = (int8_t)fw[2];
       if (t < 144) {
-        const int cell = border20(t);
+        const int cell = c20;
 #pragma unroll
         for (int v = 0; v < 5; ++v)
           cp[b][v][cell] = v == 3 && zlo && cell % 20 < 2 ? zbc_neg(bcp[v], r3) : bcp[v];
@@ -842,11 +854,6 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
       for (int v = 0; v < 5; ++v) w[m][v] = w[m + 1][v];
 #pragma unroll
     for (int m = 0; m < 4; ++m) fw[m] = fw[m + 1];
-    if (i + 4 <= 19) {
-#pragma unroll
-      for (int v = 0; v < 5; ++v) w[6][v] = Uc[v * PLANE + (i + 4) * 400 + col];
-    }
-    if (i + 3 <= 19) fw[4] = Fc[(i + 3) * 400 + col];
   }
 
   // Ghost regions the domain BCs wrote this stage go to Uout as they are (no
