@@ -757,6 +757,14 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
       cdP = dPxf;
       corder = of;
     }
+    // The cell's time levels, loaded here so that the y-, z- and viscous
+    // terms run while they arrive.
+    double tu1[5], tuq[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      tu1[v] = U1[b + (size_t)v * PLANE];
+      tuq[v] = U1q[b + (size_t)v * PLANE];
+    }
     // ---- inviscid, y ----
     {
       const int8_t* fl = fpl + j * 20 + k;
@@ -803,9 +811,11 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
     }
 
     // ---- viscous (Flux.cpp:1722-2279) from the primitive ring ----
+    // The cell's conserved values from the staged plane (interior cells are
+    // staged as they are).
     double u[5];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) u[v] = U[b + (size_t)v * PLANE];
+    for (int v = 0; v < 5; ++v) u[v] = cpl[v * 400 + j * 20 + k];
     const double rho = u[0];
     auto Q = [&](int a, int bb, int cc, int m) -> double {
       return pr[m][((i + a) % 3) * PR_PLANE + (j + bb - 1) * 18 + (k + cc - 1)];
@@ -879,12 +889,12 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
     // ---- residual with the dual-time source, preconditioner, increment ----
     double Rk[5];
     {
-      const double uu0 = U1[b], uq0 = U1q[b];
+      const double uu0 = tu1[0], uq0 = tuq[0];
       Rk[0] = -(3 * u[0] - 4 * uu0 + uq0) / (2 * deltaT) + Rf[0];
     }
 #pragma unroll
     for (int v = 1; v < 5; ++v) {
-      const double uu = U1[b + (size_t)v * PLANE], uq = U1q[b + (size_t)v * PLANE];
+      const double uu = tu1[v], uq = tuq[v];
       const double Rp = -(3 * u[v] - 4 * uu + uq) / (2 * deltaT);
       Rk[v] = Rp + Rf[v] + vF[v];
     }

@@ -412,9 +412,16 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
   const int t = threadIdx.x;
   const int j = 2 + (t >> 6) + 4 * ((t >> 4) & 3), k = (t & 15) + 2;
   const int col = j * 20 + k;
-  // This lane's cell among the staged plane's 144 border cells (lanes t <
-  // 144) and the primitive plane's 68 border cells (lanes t < 68).
-  const int c20 = border20(t < 144 ? t : 0), c18 = border18(t < 68 ? t : 0);
+  // Border work goes to waves 0 and 3, whose face pass has no boundary
+  // round: of a staged plane's 144 border cells, s20a (wave 0: 0..63, wave 3:
+  // 64..127) and s20b (wave 3: 128..143); of a primitive plane's 68 border
+  // cells, s18 (wave 0: 0..63, wave 3: 64..67); -1 for none.  c20a, c20b and
+  // c18 are the cells.
+  const int s20a = t < 64 ? t : (t >= 192 ? t - 128 : -1);
+  const int s20b = t >= 196 && t < 212 ? t - 68 : -1;
+  const int s18 = t < 64 ? t : (t >= 192 && t < 196 ? t - 128 : -1);
+  const int c20a = border20(s20a < 0 ? 0 : s20a), c20b = border20(s20b < 0 ? 0 : s20b);
+  const int c18 = border18(s18 < 0 ? 0 : s18);
   const double dxd = cube_size[c] / 16;
   const float inv = (float)(1.0 / dxd), vol = (float)(dxd * dxd * dxd);
   const float e = (float)e_d;
@@ -473,9 +480,8 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
     T = (sd.dP - rf.PoR * q[0]) * sd.ir * IRGf;
   };
   // Primitive plane p into ring slot p & 3: the own cell from the column
-  // window (own holds plane p), the 68 border cells (lanes t < 68) from bq,
-  // their conserved values loaded beforehand.
-  auto fill_prims = [&](int p, const float (&own)[5], const float (&bq)[5]) {
+  // window (own holds plane p).
+  auto fill_own = [&](int p, const float (&own)[5]) {
     float* dst = &pr[p & 3][0][0];
     float u, v, ww, T;
     prims(own, u, v, ww, T);
@@ -484,7 +490,13 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
     dst[PR + o] = v;
     dst[2 * PR + o] = ww;
     dst[3 * PR + o] = T;
-    if (t < 68) {
+  };
+  // The border cells of primitive plane p from bq, their conserved values
+  // loaded beforehand.
+  auto fill_border = [&](int p, const float (&bq)[5]) {
+    float* dst = &pr[p & 3][0][0];
+    float u, v, ww, T;
+    if (s18 >= 0) {
       const int cell = c18;
       float q[5];
 #pragma unroll
@@ -507,7 +519,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
         if (zlo && cell % 20 == 1) q[3] = zbc_neg(q[3], r3);
         // Interior planes: the sources of the plane's corners, edge ghost
         // cells formed by corner_prims.
-        const int es = corner_source_slot(t);
+        const int es = corner_source_slot(s18);
         if (es >= 0) {
 #pragma unroll
           for (int m = 0; m < 5; ++m) esrc[p & 1][es][m] = q[m];
@@ -525,7 +537,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
   // two face ghost cells each, as the pre-flux exchange's edge passes would
   // leave them (no edge pass runs between the stage's face updates and this
   // kernel).  Only the plane's own face pass reads them; four lanes with one
-  // face fewer to evaluate form them from the sources fill_prims kept.
+  // face fewer to evaluate form them from the sources fill_border kept.
   auto corner_prims = [&](int p) {
     const int q4 = t - (WG - 4);
     if (q4 < 0) return;
@@ -541,38 +553,71 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
     dst[2 * PR + o] = ww;
     dst[3 * PR + o] = T;
   };
-  // Border values are loaded one iteration ahead of their use: bcp/bfl the
-  // 144 border cells of the next plane to stage, bpr the 68 primitive-ring
-  // border cells of the plane after it.
+  // Border values are loaded one iteration ahead of their use: bcp/bfl (and
+  // bcq/bfq) the border cells of a plane to stage, bpr those of a primitive
+  // plane.
   auto load_b18 = [&](int p, float (&q)[5]) {
-    if (t < 68 && p <= 19) {
-      const int cell = c18;
+    if (s18 >= 0 && p <= 19) {
       bool neg;
-      const int src = zsrc(p * 400 + cell, neg);
+      const int src = zsrc(p * 400 + c18, neg);
 #pragma unroll
       for (int v = 0; v < 5; ++v) q[v] = Uc[v * PLANE + src];
     }
   };
-  auto load_b20 = [&](int p, float (&q)[5], int& f) {
-    if (t < 144 && p <= 17) {
-      const int cell = c20;
-      bool neg;
-      const int src = zsrc(p * 400 + cell, neg);
+  auto load_b20 = [&](int p, float (&qa)[5], int& fa, float (&qb)[5], int& fb) {
+    if (p > 17) return;
+    bool neg;
+    if (s20a >= 0) {
+      const int src = zsrc(p * 400 + c20a, neg);
 #pragma unroll
-      for (int v = 0; v < 5; ++v) q[v] = Uc[v * PLANE + src];
-      f = Fc[p * 400 + cell];
+      for (int v = 0; v < 5; ++v) qa[v] = Uc[v * PLANE + src];
+      fa = Fc[p * 400 + c20a];
+    }
+    if (s20b >= 0) {
+      const int src = zsrc(p * 400 + c20b, neg);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) qb[v] = Uc[v * PLANE + src];
+      fb = Fc[p * 400 + c20b];
     }
   };
-  float bcp[5] = {0, 0, 0, 0, 0}, bpr[5] = {1, 0, 0, 0, 0};
-  int bfl = 0;
+  // The border cells of plane p into its staging buffer.
+  auto stage_border = [&](int p, const float (&qa)[5], int fa, const float (&qb)[5], int fb) {
+    if (s20a >= 0) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v)
+        cp[p & 1][v][c20a] = v == 3 && zlo && c20a % 20 < 2 ? zbc_neg(qa[v], r3) : qa[v];
+      fl[p & 1][c20a] = (int8_t)fa;
+    }
+    if (s20b >= 0) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v)
+        cp[p & 1][v][c20b] = v == 3 && zlo && c20b % 20 < 2 ? zbc_neg(qb[v], r3) : qb[v];
+      fl[p & 1][c20b] = (int8_t)fb;
+    }
+  };
+  // Before the march: primitive planes start-1, start and the border of
+  // start+1; the border of the first staged plane.  In the march, iteration
+  // i stages plane i's own cells and forms plane i+1's own primitives before
+  // its face pass; during the face pass waves 0 and 3 stage the border of
+  // plane i+1 and form the border primitives of plane i+2.
+  float bcp[5] = {0, 0, 0, 0, 0}, bcq[5] = {0, 0, 0, 0, 0}, bpr[5] = {1, 0, 0, 0, 0};
+  int bfl = 0, bfq = 0;
   load_b18(start - 1, bpr);
-  fill_prims(start - 1, w[2], bpr);
+  fill_own(start - 1, w[2]);
+  fill_border(start - 1, bpr);
   load_b18(start, bpr);
-  fill_prims(start, w[3], bpr);
+  fill_own(start, w[3]);
+  fill_border(start, bpr);
+  load_b18(start + 1, bpr);
+  fill_border(start + 1, bpr);
+  if (start >= i0) {
+    load_b20(start, bcp, bfl, bcq, bfq);
+    stage_border(start, bcp, bfl, bcq, bfq);
+  }
   __syncthreads();
   corner_prims(start);
-  load_b20(start, bcp, bfl);
-  load_b18(start + 1, bpr);
+  load_b20(start + 1, bcp, bfl, bcq, bfq);
+  load_b18(start + 2, bpr);
 
   float cF[5] = {0, 0, 0, 0, 0}, cdP = 0.f;
   float cL[4] = {0, 0, 0, 0};   // viscous x-face flux carried from the previous plane
@@ -588,35 +633,33 @@ __global__ __launch_bounds__(WG, 2) void k_flux_mixed(
 #pragma unroll
     for (int v = 0; v < 5; ++v) w[6][v] = Uc[v * PLANE + min(i + 3, 19) * 400 + col];
     fw[4] = Fc[(i + 2) * 400 + col];
-    // ---- stage plane i (conserved, flags) and primitives of plane i+1 ----
+    // ---- stage plane i's own cells, primitives of plane i+1's own cells ----
     if (!pro) {
 #pragma unroll
       for (int v = 0; v < 5; ++v) cp[b][v][col] = w[3][v];
       fl[b][col] = (int8_t)fw[2];
-      if (t < 144) {
-        const int cell = c20;
-#pragma unroll
-        for (int v = 0; v < 5; ++v)
-          cp[b][v][cell] = v == 3 && zlo && cell % 20 < 2 ? zbc_neg(bcp[v], r3) : bcp[v];
-        fl[b][cell] = (int8_t)bfl;
-      }
     }
-    fill_prims(i + 1, w[4], bpr);
-    // Loads for later: this cell's FP64 source constant Bv = 3 Uref - 4 U1 +
-    // U1q (first use in the residual, after the face pass), the next
-    // iteration's borders.
+    fill_own(i + 1, w[4]);
+    // This cell's FP64 source constant Bv = 3 Uref - 4 U1 + U1q, first used
+    // in the residual after the face pass.
     double Bv[5] = {0, 0, 0, 0, 0};
     if (!pro) {
       const size_t gb = gcol + (size_t)i * 400;
 #pragma unroll
       for (int v = 0; v < 5; ++v) Bv[v] = Bsrc[gb + (size_t)v * PLANE];
     }
-    load_b20(i + 1, bcp, bfl);
-    load_b18(i + 2, bpr);
     __syncthreads();
 
     // ---- y- and z-faces of plane i: 272 + 272 faces over 256 lanes ----
     if (i + 1 <= 17) corner_prims(i + 1);
+    // Border work for later planes (waves 0 and 3), and its loads for the
+    // next iteration.  Plane i+1's staging buffer last held plane i-1, whose
+    // readers all passed this iteration's first barrier; ring slot i+2 last
+    // held plane i-2.
+    if (i + 1 <= i1) stage_border(i + 1, bcp, bfl, bcq, bfq);
+    if (i + 2 <= i1 + 1) fill_border(i + 2, bpr);
+    if (i + 2 <= i1) load_b20(i + 2, bcp, bfl, bcq, bfq);
+    if (i + 3 <= i1 + 1) load_b18(i + 3, bpr);
     // Round 0: each lane the y-face above its cell; round 1: the z-face above
     // it (every wave evaluates one face kind at a time); rounds 2 and 3: the
     // 16 y-faces and the 16 z-faces between the first ghost layer and the
