@@ -46,10 +46,10 @@ METRIC = "Mcell-iter/s + %HBM roofline, mixed vs FP64 at ≤1e-4 Cd err; p99 lat
 HBM_PEAK_GBS = 8000.0
 # Compulsory HBM traffic of one cell and one RK stage of the flux/update pass:
 # FP64: read the inner iterate (40 B), U1 and U1q (40 B each) and the flag
-# (4 B), write the stage result (40 B).  Mixed: read the FP32 iterate (20 B),
+# (1 B), write the stage result (40 B).  Mixed: read the FP32 iterate (20 B),
 # the FP64 dual-time source constant 3 Uref - 4 U1 + U1q formed once per step
-# (40 B) and the flag (4 B), write the FP32 stage result (20 B).
-ALGO_BYTES = {"fp64": 164, "mixed": 84}
+# (40 B) and the flag (1 B), write the FP32 stage result (20 B).
+ALGO_BYTES = {"fp64": 161, "mixed": 81}
 FLUX_KERNEL = {"fp64": "k_flux_march", "mixed": "k_flux_mixed"}   # traffic file stems
 
 

@@ -32,12 +32,12 @@ def test_profile_figures_parse():
     assert v["achieved"] == pytest.approx(per * 8388608 / 2e-3 / 1e12, rel=1e-3)
     assert v["peak"] == pytest.approx(256 * 4 * 16 * 2.4e9 / 1e12, rel=1e-3)
     # At the full issue rate: cell-stages/s = issue rate / instructions, each
-    # moving 164 algorithmic bytes, against 8 TB/s.
+    # moving 161 algorithmic bytes, against 8 TB/s.
     assert v["hbm_frac_at_full_issue"] == pytest.approx(
-        256 * 4 * 16 * 2.4e9 / per * 164 / 8e12, rel=1e-3)
+        256 * 4 * 16 * 2.4e9 / per * 161 / 8e12, rel=1e-3)
     r = bench.roofline("fp64", 8388608, 2.0, "cfg2")
     assert r["bound"] == "hbm" and r["unit"] == "GB/s" and r["peak"] == 8000.0
-    assert r["achieved"] == pytest.approx(164 * 8388608 / 2e-3 / 1e9, rel=1e-3)
+    assert r["achieved"] == pytest.approx(161 * 8388608 / 2e-3 / 1e9, rel=1e-3)
     assert r["frac"] == pytest.approx(r["achieved"] / 8000.0, rel=1e-3)
 
 
