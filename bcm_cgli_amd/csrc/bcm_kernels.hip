@@ -462,6 +462,35 @@ __device__ __forceinline__ void face_flux(const double* __restrict__ U, size_t b
   roe_flux<D>(L, R, e, F, dP);
 }
 
+// face_flux<D> on cells already loaded: s[m][v] is variable v of cell
+// p-2+m.  The same expressions, so the same bits.
+template <int D>
+__device__ __forceinline__ void face_flux_v(const double (&s)[6][5], int order, double e,
+                                            double (&F)[5], double& dP) {
+  const double c6 = 1. / 6, c60 = 1. / 60;
+  double L[5], R[5];
+  if (order == 5) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      L[v] = c60 * (2 * s[0][v] - 13 * s[1][v] + 47 * s[2][v] + 27 * s[3][v] - 3 * s[4][v]);
+      R[v] = c60 * (-3 * s[1][v] + 27 * s[2][v] + 47 * s[3][v] - 13 * s[4][v] + 2 * s[5][v]);
+    }
+  } else if (order == 3) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      L[v] = c6 * (-s[1][v] + 5 * s[2][v] + 2 * s[3][v]);
+      R[v] = c6 * (2 * s[2][v] + 5 * s[3][v] - s[4][v]);
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      L[v] = s[2][v];
+      R[v] = s[3][v];
+    }
+  }
+  roe_flux<D>(L, R, e, F, dP);
+}
+
 // LDS layout of k_flux_march.
 constexpr int PR_PLANE = 18 * 18;   // primitives of j,k in [1,18]
 constexpr int NFACE = 17 * 16;      // faces of one axis in one i-plane
@@ -729,18 +758,28 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
         fz[5][g] = dP;
       }
     }
+    // The x-stencil of the cell's forward face, planes i-2..i+3, and the
+    // flags it needs, loaded before the barrier so that they arrive while the
+    // workgroup gathers.  (Plane i+3 = 20 would lie beyond the cube; the last
+    // interior plane's forward face has order 3 or 1, which does not read
+    // it, so plane 19 stands in.)
+    const int8_t* fc = fws + fidx(c, i, j, k);
+    const int IF = fc[0];
+    const int fm2 = fc[-800], fm1 = fc[-400], fp1 = fc[400], fp2 = fc[800];
+    double xs[6][5];
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+#pragma unroll
+      for (int v = 0; v < 5; ++v) xs[m][v] = U[sidx(c, v, min(i - 2 + m, 19), j, k)];
     __syncthreads();
 
     const size_t b = sidx(c, 0, i, j, k);
-    const int8_t* fc = fws + fidx(c, i, j, k);
-    const int IF = fc[0];
     // ---- inviscid, x: forward face computed, backward face carried ----
     double Rx[5], Ry[5], Rz[5], dPxb, dPxf, dPzb, dPzf;
     {
-      const int fm2 = fc[-800], fm1 = fc[-400], fp1 = fc[400], fp2 = fc[800];
       const int of = order_fwd(i, fm1, fp1, fp2);
       double Ff[5];
-      face_flux<0>(U, b, of, e, Ff, dPxf);
+      face_flux_v<0>(xs, of, e, Ff, dPxf);
       const int ob = order_bwd(i, fm2, fm1, fp1);
       if (i != i0 && ob == corder) {
         dPxb = cdP;
