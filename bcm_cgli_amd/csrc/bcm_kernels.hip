@@ -32,12 +32,14 @@ __device__ __forceinline__ double rmax(double a, double b) { return (a > b) ? a 
 // Full-state access to an inner iterate.  FP64 storage holds the state
 // itself; FP32 storage holds its deviation from the case's reference state.
 struct NoOff {
+  static constexpr bool exact = true;   // the reference's operations, bitwise
   __device__ explicit NoOff(const DevPhys&) {}
   __device__ double ld(const double* U, size_t o, int) const { return U[o]; }
   __device__ void st(double* U, size_t o, int, double x) const { U[o] = x; }
   __device__ double rt(double x, int) const { return x; }
 };
 struct RefOff {
+  static constexpr bool exact = false;  // mixed path: operation order free
   double r[5];
   __device__ explicit RefOff(const DevPhys& ph) { ref_state(ph, r); }
   __device__ double ld(const float* U, size_t o, int v) const { return (double)U[o] + r[v]; }
@@ -1341,9 +1343,17 @@ __device__ __forceinline__ void ghost_cube(T* __restrict__ U, const GhostTables&
           for (int v = 0; v < 5; ++v) q[v] = F.ld(U, o + (size_t)v * PLANE, v);
         }
         rc[m] = q[0];
-        uc[m] = q[1] / rc[m];
-        vc[m] = q[2] / rc[m];
-        wc[m] = q[3] / rc[m];
+        if constexpr (OffOf<T>::type::exact) {
+          uc[m] = q[1] / rc[m];
+          vc[m] = q[2] / rc[m];
+          wc[m] = q[3] / rc[m];
+        } else {
+          // One reciprocal per corner instead of three divisions.
+          const double ir = 1.0 / rc[m];
+          uc[m] = q[1] * ir;
+          vc[m] = q[2] * ir;
+          wc[m] = q[3] * ir;
+        }
         const double VV = uc[m] * uc[m] + vc[m] * vc[m] + wc[m] * wc[m];
         pc[m] = (q[4] - 0.5 * rc[m] * VV) * KM1;
       }
