@@ -1302,6 +1302,30 @@ __global__ __launch_bounds__(WG) void k_bc(T* __restrict__ U, const int* __restr
 
 // IB fill of ghost-table cube g by the calling workgroup; gsm holds
 // [5][lds_cap] doubles.
+// The fields of a GhostEntry the fill reads, loaded one by one so that a copy
+// held across a level stays in registers.
+struct GhostEntryR {
+  int16_t gi, gj, gk, pi, pj, pk, lg;
+  int16_t lc[8];
+  double w[8];
+};
+__device__ __forceinline__ GhostEntryR load_entry(const GhostEntry* p) {
+  GhostEntryR r;
+  r.gi = p->gi;
+  r.gj = p->gj;
+  r.gk = p->gk;
+  r.pi = p->pi;
+  r.pj = p->pj;
+  r.pk = p->pk;
+  r.lg = p->lg;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    r.lc[m] = p->lc[m];
+    r.w[m] = p->w[m];
+  }
+  return r;
+}
+
 template <typename T>
 __device__ __forceinline__ void ghost_cube(T* __restrict__ U, const GhostTables& gt, int g,
                                            int lds_cap, PhysTab pt, double* gsm) {
@@ -1316,125 +1340,163 @@ __device__ __forceinline__ void ghost_cube(T* __restrict__ U, const GhostTables&
   const int ne = gt.level_off[l1] - gt.level_off[l0];
   const bool lds = nt + ne <= lds_cap;
   if (lds) {
-    for (int n = threadIdx.x; n < nt; n += WG) {
+    // Up to six cells per lane: the loads of two cells are issued together.
+    for (int n = threadIdx.x; n < nt; n += 2 * WG) {
+      const int n2 = n + WG;
       const size_t o = (size_t)c * CUBE_STRIDE + gt.touched[t0 + n];
+      const size_t o2 = (size_t)c * CUBE_STRIDE + gt.touched[t0 + (n2 < nt ? n2 : n)];
+      double a[5], a2[5];
 #pragma unroll
-      for (int v = 0; v < 5; ++v) gsm[v * lds_cap + n] = F.ld(U, o + (size_t)v * PLANE, v);
+      for (int v = 0; v < 5; ++v) {
+        a[v] = F.ld(U, o + (size_t)v * PLANE, v);
+        a2[v] = F.ld(U, o2 + (size_t)v * PLANE, v);
+      }
+#pragma unroll
+      for (int v = 0; v < 5; ++v) gsm[v * lds_cap + n] = a[v];
+      if (n2 < nt) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) gsm[v * lds_cap + n2] = a2[v];
+      }
     }
     __syncthreads();
   }
-  for (int lv = l0; lv < l1; ++lv) {
-    const int e0 = gt.level_off[lv], e1 = gt.level_off[lv + 1];
-    for (int n = e0 + threadIdx.x; n < e1; n += WG) {
-      const GhostEntry& E = gt.ent[n];
-      double rc[8], uc[8], vc[8], wc[8], pc[8];
+  // One entry: interpolation from the eight corners, into its slot (LDS) or
+  // its ghost cell (memory).
+  auto fill_entry = [&](const GhostEntryR& E) {
+    double rc[8], uc[8], vc[8], wc[8], pc[8];
 #pragma unroll
-      for (int m = 0; m < 8; ++m) {
-        const int a = (m == 1 || m == 4 || m == 5 || m == 7);
-        const int bq = (m == 2 || m == 4 || m == 6 || m == 7);
-        const int cq = (m == 3 || m == 5 || m == 6 || m == 7);
-        double q[5];
-        if (lds) {
+    for (int m = 0; m < 8; ++m) {
+      const int a = (m == 1 || m == 4 || m == 5 || m == 7);
+      const int bq = (m == 2 || m == 4 || m == 6 || m == 7);
+      const int cq = (m == 3 || m == 5 || m == 6 || m == 7);
+      double q[5];
+      if (lds) {
 #pragma unroll
-          for (int v = 0; v < 5; ++v) q[v] = gsm[v * lds_cap + E.lc[m]];
-        } else {
-          const size_t o = sidx(c, 0, E.pi + a, E.pj + bq, E.pk + cq);
+        for (int v = 0; v < 5; ++v) q[v] = gsm[v * lds_cap + E.lc[m]];
+      } else {
+        const size_t o = sidx(c, 0, E.pi + a, E.pj + bq, E.pk + cq);
 #pragma unroll
-          for (int v = 0; v < 5; ++v) q[v] = F.ld(U, o + (size_t)v * PLANE, v);
-        }
-        rc[m] = q[0];
-        if constexpr (OffOf<T>::type::exact) {
-          uc[m] = q[1] / rc[m];
-          vc[m] = q[2] / rc[m];
-          wc[m] = q[3] / rc[m];
-        } else {
-          // One reciprocal per corner instead of three divisions.
-          const double ir = 1.0 / rc[m];
-          uc[m] = q[1] * ir;
-          vc[m] = q[2] * ir;
-          wc[m] = q[3] * ir;
-        }
-        const double VV = uc[m] * uc[m] + vc[m] * vc[m] + wc[m] * wc[m];
-        pc[m] = (q[4] - 0.5 * rc[m] * VV) * KM1;
+        for (int v = 0; v < 5; ++v) q[v] = F.ld(U, o + (size_t)v * PLANE, v);
       }
-      double rho = E.w[0] * (rc[0] - rho0), P = E.w[0] * (pc[0] - P0);
-      double uu = E.w[0] * uc[0], vv = E.w[0] * vc[0], ww = E.w[0] * wc[0];
-#pragma unroll
-      for (int m = 1; m < 8; ++m) {
-        rho = rho + E.w[m] * (rc[m] - rho0);
-        P = P + E.w[m] * (pc[m] - P0);
-        uu = uu + E.w[m] * uc[m];
-        vv = vv + E.w[m] * vc[m];
-        ww = ww + E.w[m] * wc[m];
+      rc[m] = q[0];
+      if constexpr (OffOf<T>::type::exact) {
+        uc[m] = q[1] / rc[m];
+        vc[m] = q[2] / rc[m];
+        wc[m] = q[3] / rc[m];
+      } else {
+        // One reciprocal per corner instead of three divisions.
+        const double ir = 1.0 / rc[m];
+        uc[m] = q[1] * ir;
+        vc[m] = q[2] * ir;
+        wc[m] = q[3] * ir;
       }
-      rho = rho + rho0;
-      P = P + P0;
-      double VV = uu * uu + vv * vv + ww * ww;
-      double o0 = rho, o1 = 0.5 * rho * uu, o2 = 0.5 * rho * vv, o3 = 0.5 * rho * ww;
-      double o4 = P / KM1 + 0.5 * rho * 0.25 * VV;
-      // The ghost cell may itself be an interpolation corner (:192-449).
-      const int di = E.gi - E.pi, dj = E.gj - E.pj, dk = E.gk - E.pk;
-      if (di >= 0 && di <= 1 && dj >= 0 && dj <= 1 && dk >= 0 && dk <= 1) {
-        const int s = di && dj && dk ? 7
-                      : (dj && dk ? 6 : (di && dk ? 5 : (di && dj ? 4 : (dk ? 3 : (dj ? 2 : (di ? 1 : 0))))));
-        const double ws = E.w[s];
-        if (ws > 0.9999) {
-          rho = rc[s];
-          P = pc[s];
-        } else {
-          double sr = 0, sp = 0;
-          bool first = true;
+      const double VV = uc[m] * uc[m] + vc[m] * vc[m] + wc[m] * wc[m];
+      pc[m] = (q[4] - 0.5 * rc[m] * VV) * KM1;
+    }
+    double rho = E.w[0] * (rc[0] - rho0), P = E.w[0] * (pc[0] - P0);
+    double uu = E.w[0] * uc[0], vv = E.w[0] * vc[0], ww = E.w[0] * wc[0];
 #pragma unroll
-          for (int m = 0; m < 8; ++m) {
-            if (m == s) continue;
-            if (first) {
-              sr = E.w[m] * (rc[m] - rho0);
-              sp = E.w[m] * (pc[m] - P0);
-              first = false;
-            } else {
-              sr = sr + E.w[m] * (rc[m] - rho0);
-              sp = sp + E.w[m] * (pc[m] - P0);
-            }
-          }
-          rho = sr / (1 - ws) + rho0;
-          P = sp / (1 - ws) + P0;
+    for (int m = 1; m < 8; ++m) {
+      rho = rho + E.w[m] * (rc[m] - rho0);
+      P = P + E.w[m] * (pc[m] - P0);
+      uu = uu + E.w[m] * uc[m];
+      vv = vv + E.w[m] * vc[m];
+      ww = ww + E.w[m] * wc[m];
+    }
+    rho = rho + rho0;
+    P = P + P0;
+    double VV = uu * uu + vv * vv + ww * ww;
+    double o0 = rho, o1 = 0.5 * rho * uu, o2 = 0.5 * rho * vv, o3 = 0.5 * rho * ww;
+    double o4 = P / KM1 + 0.5 * rho * 0.25 * VV;
+    // The ghost cell may itself be an interpolation corner (:192-449).
+    const int di = E.gi - E.pi, dj = E.gj - E.pj, dk = E.gk - E.pk;
+    if (di >= 0 && di <= 1 && dj >= 0 && dj <= 1 && dk >= 0 && dk <= 1) {
+      const int s = di && dj && dk ? 7
+                    : (dj && dk ? 6 : (di && dk ? 5 : (di && dj ? 4 : (dk ? 3 : (dj ? 2 : (di ? 1 : 0))))));
+      // Corner s's weight and values, selected without indexing by s (which
+      // would put the arrays in memory).
+      double ws = 0, rs = 0, ps = 0;
+#pragma unroll
+      for (int m = 0; m < 8; ++m)
+        if (m == s) {
+          ws = E.w[m];
+          rs = rc[m];
+          ps = pc[m];
         }
-        double su = 0, sv = 0, sw = 0;
+      if (ws > 0.9999) {
+        rho = rs;
+        P = ps;
+      } else {
+        double sr = 0, sp = 0;
         bool first = true;
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
           if (m == s) continue;
           if (first) {
-            su = E.w[m] * uc[m];
-            sv = E.w[m] * vc[m];
-            sw = E.w[m] * wc[m];
+            sr = E.w[m] * (rc[m] - rho0);
+            sp = E.w[m] * (pc[m] - P0);
             first = false;
           } else {
-            su = su + E.w[m] * uc[m];
-            sv = sv + E.w[m] * vc[m];
-            sw = sw + E.w[m] * wc[m];
+            sr = sr + E.w[m] * (rc[m] - rho0);
+            sp = sp + E.w[m] * (pc[m] - P0);
           }
         }
-        uu = su / (2 - ws);
-        vv = sv / (2 - ws);
-        ww = sw / (2 - ws);
-        VV = uu * uu + vv * vv + ww * ww;
-        o0 = rho;
-        o1 = rho * uu;
-        o2 = rho * vv;
-        o3 = rho * ww;
-        o4 = P / KM1 + 0.5 * rho * VV;
+        rho = sr / (1 - ws) + rho0;
+        P = sp / (1 - ws) + P0;
       }
-      const double out[5] = {o0, o1, o2, o3, o4};
-      if (lds) {
+      double su = 0, sv = 0, sw = 0;
+      bool first = true;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) gsm[v * lds_cap + E.lg] = F.rt(out[v], v);
-      } else {
-        const size_t go = sidx(c, 0, E.gi, E.gj, E.gk);
-#pragma unroll
-        for (int v = 0; v < 5; ++v) F.st(U, go + (size_t)v * PLANE, v, out[v]);
+      for (int m = 0; m < 8; ++m) {
+        if (m == s) continue;
+        if (first) {
+          su = E.w[m] * uc[m];
+          sv = E.w[m] * vc[m];
+          sw = E.w[m] * wc[m];
+          first = false;
+        } else {
+          su = su + E.w[m] * uc[m];
+          sv = sv + E.w[m] * vc[m];
+          sw = sw + E.w[m] * wc[m];
+        }
       }
+      uu = su / (2 - ws);
+      vv = sv / (2 - ws);
+      ww = sw / (2 - ws);
+      VV = uu * uu + vv * vv + ww * ww;
+      o0 = rho;
+      o1 = rho * uu;
+      o2 = rho * vv;
+      o3 = rho * ww;
+      o4 = P / KM1 + 0.5 * rho * VV;
     }
+    const double out[5] = {o0, o1, o2, o3, o4};
+    if (lds) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) gsm[v * lds_cap + E.lg] = F.rt(out[v], v);
+    } else {
+      const size_t go = sidx(c, 0, E.gi, E.gj, E.gk);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) F.st(U, go + (size_t)v * PLANE, v, out[v]);
+    }
+  };
+  // A level holds at most a few hundred entries, so each is one entry's
+  // latency: the lane's entry of the next level is loaded while the current
+  // one is computed (entries beyond the first WG of a level load in place).
+  // (A lane without an entry in a level loads entry 0 of the schedule, which
+  // it does not use.)
+  int nx = gt.level_off[l0] + threadIdx.x;
+  GhostEntryR Enext = load_entry(gt.ent + (nx < gt.level_off[l0 + 1] ? nx : 0));
+  for (int lv = l0; lv < l1; ++lv) {
+    const int e0 = gt.level_off[lv], e1 = gt.level_off[lv + 1];
+    const GhostEntryR Efirst = Enext;
+    if (lv + 1 < l1) {
+      nx = e1 + threadIdx.x;
+      Enext = load_entry(gt.ent + (nx < gt.level_off[lv + 2] ? nx : 0));
+    }
+    const int n0 = e0 + threadIdx.x;
+    if (n0 < e1) fill_entry(Efirst);
+    for (int n = n0 + WG; n < e1; n += WG) fill_entry(load_entry(gt.ent + n));
     __syncthreads();
   }
   if (lds) {
