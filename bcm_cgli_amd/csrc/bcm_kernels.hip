@@ -1360,39 +1360,28 @@ __device__ __forceinline__ void ghost_cube(T* __restrict__ U, const GhostTables&
     }
     __syncthreads();
   }
-  // One entry: interpolation from the eight corners, into its slot (LDS) or
-  // its ghost cell (memory).
-  auto fill_entry = [&](const GhostEntryR& E) {
-    double rc[8], uc[8], vc[8], wc[8], pc[8];
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const int a = (m == 1 || m == 4 || m == 5 || m == 7);
-      const int bq = (m == 2 || m == 4 || m == 6 || m == 7);
-      const int cq = (m == 3 || m == 5 || m == 6 || m == 7);
-      double q[5];
-      if (lds) {
-#pragma unroll
-        for (int v = 0; v < 5; ++v) q[v] = gsm[v * lds_cap + E.lc[m]];
-      } else {
-        const size_t o = sidx(c, 0, E.pi + a, E.pj + bq, E.pk + cq);
-#pragma unroll
-        for (int v = 0; v < 5; ++v) q[v] = F.ld(U, o + (size_t)v * PLANE, v);
-      }
-      rc[m] = q[0];
-      if constexpr (OffOf<T>::type::exact) {
-        uc[m] = q[1] / rc[m];
-        vc[m] = q[2] / rc[m];
-        wc[m] = q[3] / rc[m];
-      } else {
-        // One reciprocal per corner instead of three divisions.
-        const double ir = 1.0 / rc[m];
-        uc[m] = q[1] * ir;
-        vc[m] = q[2] * ir;
-        wc[m] = q[3] * ir;
-      }
-      const double VV = uc[m] * uc[m] + vc[m] * vc[m] + wc[m] * wc[m];
-      pc[m] = (q[4] - 0.5 * rc[m] * VV) * KM1;
+  // Density, velocity and pressure of one interpolation corner.
+  auto corner_prim = [&](const double (&q)[5], double& r, double& u, double& v, double& w,
+                         double& p) {
+    r = q[0];
+    if constexpr (OffOf<T>::type::exact) {
+      u = q[1] / r;
+      v = q[2] / r;
+      w = q[3] / r;
+    } else {
+      // One reciprocal per corner instead of three divisions.
+      const double ir = 1.0 / r;
+      u = q[1] * ir;
+      v = q[2] * ir;
+      w = q[3] * ir;
     }
+    const double VV = u * u + v * v + w * w;
+    p = (q[4] - 0.5 * r * VV) * KM1;
+  };
+  // One entry from its corners' primitives: into its slot (LDS) or its ghost
+  // cell (memory).
+  auto combine = [&](const GhostEntryR& E, const double (&rc)[8], const double (&uc)[8],
+                     const double (&vc)[8], const double (&wc)[8], const double (&pc)[8]) {
     double rho = E.w[0] * (rc[0] - rho0), P = E.w[0] * (pc[0] - P0);
     double uu = E.w[0] * uc[0], vv = E.w[0] * vc[0], ww = E.w[0] * wc[0];
 #pragma unroll
@@ -1480,23 +1469,103 @@ __device__ __forceinline__ void ghost_cube(T* __restrict__ U, const GhostTables&
       for (int v = 0; v < 5; ++v) F.st(U, go + (size_t)v * PLANE, v, out[v]);
     }
   };
-  // A level holds at most a few hundred entries, so each is one entry's
-  // latency: the lane's entry of the next level is loaded while the current
-  // one is computed (entries beyond the first WG of a level load in place).
-  // (A lane without an entry in a level loads entry 0 of the schedule, which
-  // it does not use.)
-  int nx = gt.level_off[l0] + threadIdx.x;
-  GhostEntryR Enext = load_entry(gt.ent + (nx < gt.level_off[l0 + 1] ? nx : 0));
+  // One entry by one lane: interpolation from the eight corners.
+  auto fill_entry = [&](const GhostEntryR& E) {
+    double rc[8], uc[8], vc[8], wc[8], pc[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int a = (m == 1 || m == 4 || m == 5 || m == 7);
+      const int bq = (m == 2 || m == 4 || m == 6 || m == 7);
+      const int cq = (m == 3 || m == 5 || m == 6 || m == 7);
+      double q[5];
+      if (lds) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) q[v] = gsm[v * lds_cap + E.lc[m]];
+      } else {
+        const size_t o = sidx(c, 0, E.pi + a, E.pj + bq, E.pk + cq);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) q[v] = F.ld(U, o + (size_t)v * PLANE, v);
+      }
+      corner_prim(q, rc[m], uc[m], vc[m], wc[m], pc[m]);
+    }
+    combine(E, rc, uc, vc, wc, pc);
+  };
+  // Levels run one after the other, each taking one entry's latency.  A level
+  // of at most 64 (32) entries spreads each entry over 4 (8) lanes: they form
+  // its corners' primitives into LDS, and the entry's first lane combines them
+  // as one lane would.  The lane's entry (and corner slots) of the next level
+  // are loaded while the current one is computed; entries beyond the first
+  // 256 of a level load in place.  (A lane without an entry in a level loads
+  // entry 0 of the schedule, which it does not use.)
+  __shared__ double cpr[64 * 8 * 5];   // corner primitives [entry][corner][5]
+  const int t = threadIdx.x;
+  auto lanes_of = [&](int nl) { return !lds || nl > 64 ? 1 : (nl > 32 ? 4 : 8); };
+  // Entry of lane t in a level starting at e with nl entries.
+  auto entry_of = [&](int e, int nl) {
+    const int L = lanes_of(nl);
+    const int q = t / L;
+    return q < nl ? e + q : 0;
+  };
+  // Slots of the corners lane t forms in a level starting at e with nl
+  // entries (spread levels only; corners r*K and r*K+1 of entry q).
+  auto slots_of = [&](int e, int nl, int& s0, int& s1) {
+    const int L = lanes_of(nl);
+    s0 = s1 = 0;
+    if (L == 1) return;
+    const int q = t / L, r = t % L, K = 8 / L;
+    if (q >= nl) return;
+    s0 = gt.ent[e + q].lc[r * K];
+    if (K == 2) s1 = gt.ent[e + q].lc[r * K + 1];
+  };
+  int nl = gt.level_off[l0 + 1] - gt.level_off[l0];
+  GhostEntryR Enext = load_entry(gt.ent + entry_of(gt.level_off[l0], nl));
+  int sn0, sn1;
+  slots_of(gt.level_off[l0], nl, sn0, sn1);
   for (int lv = l0; lv < l1; ++lv) {
     const int e0 = gt.level_off[lv], e1 = gt.level_off[lv + 1];
-    const GhostEntryR Efirst = Enext;
+    nl = e1 - e0;
+    const int L = lanes_of(nl);
+    const GhostEntryR Ecur = Enext;
+    const int sc0 = sn0, sc1 = sn1;
     if (lv + 1 < l1) {
-      nx = e1 + threadIdx.x;
-      Enext = load_entry(gt.ent + (nx < gt.level_off[lv + 2] ? nx : 0));
+      const int nn = gt.level_off[lv + 2] - e1;
+      Enext = load_entry(gt.ent + entry_of(e1, nn));
+      slots_of(e1, nn, sn0, sn1);
     }
-    const int n0 = e0 + threadIdx.x;
-    if (n0 < e1) fill_entry(Efirst);
-    for (int n = n0 + WG; n < e1; n += WG) fill_entry(load_entry(gt.ent + n));
+    if (L == 1) {
+      const int n0 = e0 + t;
+      if (n0 < e1) fill_entry(Ecur);
+      for (int n = n0 + WG; n < e1; n += WG) fill_entry(load_entry(gt.ent + n));
+    } else {
+      // Lane r of entry q forms corners r*K .. r*K+K-1 (the corner index only
+      // addresses memory, so the lanes of a wave run one instruction stream).
+      const int q = t / L, r = t % L, K = 8 / L;
+      if (q < nl) {
+        for (int kk = 0; kk < K; ++kk) {
+          const int m = r * K + kk, sl = kk == 0 ? sc0 : sc1;
+          double qv[5], pv[5];
+#pragma unroll
+          for (int v = 0; v < 5; ++v) qv[v] = gsm[v * lds_cap + sl];
+          corner_prim(qv, pv[0], pv[1], pv[2], pv[3], pv[4]);
+#pragma unroll
+          for (int v = 0; v < 5; ++v) cpr[(q * 8 + m) * 5 + v] = pv[v];
+        }
+      }
+      __syncthreads();
+      if (r == 0 && q < nl) {
+        double rc[8], uc[8], vc[8], wc[8], pc[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+          const double* a = cpr + (q * 8 + m) * 5;
+          rc[m] = a[0];
+          uc[m] = a[1];
+          vc[m] = a[2];
+          wc[m] = a[3];
+          pc[m] = a[4];
+        }
+        combine(Ecur, rc, uc, vc, wc, pc);
+      }
+    }
     __syncthreads();
   }
   if (lds) {
