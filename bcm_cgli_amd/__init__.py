@@ -80,6 +80,7 @@ def load_library(path=LIB_PATH):
     lib.bcm_engine_time_part.argtypes = [c_void_p, c_int, c_int, c_int, P(c_double)]
     lib.bcm_engine_ghost_stats.argtypes = [c_void_p, c_void_p]
     lib.bcm_engine_set_ghost_lds_limit.argtypes = [c_void_p, c_int]
+    lib.bcm_selftest_division.argtypes = [ctypes.c_ulonglong, c_int, c_void_p]
     lib.bcm_engine_set_flux.argtypes = [c_void_p, c_int, c_int, c_int]
     lib.bcm_engine_autotune.argtypes = [c_void_p, c_void_p]
     lib.bcm_bytes_per_cube.restype = ctypes.c_longlong
@@ -238,6 +239,14 @@ PRECISIONS = {"fp64": 0, "mixed": 1}
 # z-forces divided by 0.5 rho0 U0^2 times this constant
 # (BCM_Flux_XYZ_Viscous_Runge_kutta.cpp:2637-2638).
 FORCE_AREA = 2.602
+
+
+def selftest_division(n, seed=1):
+    """Quotients (of 4 * n on random operands) whose bits differ between the
+    FP64 kernels' shared-denominator division and plain division."""
+    out = ctypes.c_ulonglong(0)
+    _check(load_library().bcm_selftest_division(int(seed), int(n), ctypes.byref(out)))
+    return int(out.value)
 
 
 def drag_coefficient(er6, diameter):

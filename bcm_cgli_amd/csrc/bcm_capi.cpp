@@ -678,6 +678,12 @@ int bcm_engine_ghost_stats(bcm_engine* h, int* stats3) {
   return 0;
 }
 
+int bcm_selftest_division(unsigned long long seed, int n, unsigned long long* mismatches) {
+  if (!mismatches || n < 0) return fail("bad argument");
+  if (div_selftest(seed, n, mismatches)) return fail("HIP error in the division self-test");
+  return 0;
+}
+
 int bcm_engine_set_ghost_lds_limit(bcm_engine* h, int slots) {
   if (!h) return fail("null engine");
   if (slots < 0 || slots > GHOST_LDS_SLOTS)

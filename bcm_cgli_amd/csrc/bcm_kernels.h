@@ -240,6 +240,10 @@ void launch_flux_push(hipStream_t s, const double* Uin, double* Uout, const Halo
                       int ncube, PhysTab pt, double deltaT, double deltaTau, double e, int RK,
                       double* U1p1, double* U1p2, double* partials, int aux);
 void launch_copy_d(hipStream_t s, const double* src, double* dst, size_t n);
+// Runs n threads of the shared-denominator division against a / b (four
+// quotients each); *mismatches receives the count of differing bit patterns.
+// Returns nonzero on a HIP error.
+int div_selftest(unsigned long long seed, int n, unsigned long long* mismatches);
 void launch_update(hipStream_t s, int RK, double* U, double* U1p1, double* U1p2,
                    const double* Rku, const double* Res, const int8_t* fws, int ncube,
                    const int* cube_case, double* partials);

@@ -29,6 +29,40 @@ __device__ __forceinline__ int fidx(int c, int i, int j, int k) {
 // max(a,b) of the reference: (((a)>(b))?(a):(b)), NaN falls to b.
 __device__ __forceinline__ double rmax(double a, double b) { return (a > b) ? a : b; }
 
+// Several correctly rounded divisions by one denominator.  a / b compiles for
+// gfx950 to: the denominator scaled (div_scale), its reciprocal refined by
+// two Newton steps, then per quotient the numerator scaled, one product, one
+// correction and the special-case fixup (div_fmas, div_fixup).  DivBy keeps
+// the denominator's part once, scaled as for the numerator 1 (a zero
+// numerator there would give NaN; the quotient's fixup handles zeros);
+// div_sh runs the quotient's part.  The bits are those of a / b whenever the
+// denominator's scaling does not depend on the numerator: for finite
+// operands, unless an exponent difference reaches the scaling thresholds
+// (768) or an operand is subnormal, far outside the solver's values; the
+// self-test below compares against a / b.
+struct DivBy {
+  double b, ds, r;   // denominator, as scaled, refined reciprocal of ds
+};
+__device__ __forceinline__ DivBy div_by(double b) {
+  bool f;
+  DivBy d;
+  d.b = b;
+  d.ds = __builtin_amdgcn_div_scale(1.0, b, false, &f);
+  const double r0 = __builtin_amdgcn_rcp(d.ds);
+  const double e0 = __builtin_fma(-d.ds, r0, 1.0);
+  const double r1 = __builtin_fma(r0, e0, r0);
+  const double e1 = __builtin_fma(-d.ds, r1, 1.0);
+  d.r = __builtin_fma(r1, e1, r1);
+  return d;
+}
+__device__ __forceinline__ double div_sh(const DivBy& d, double a) {
+  bool f;
+  const double as = __builtin_amdgcn_div_scale(a, d.b, true, &f);
+  const double q = as * d.r;
+  const double rem = __builtin_fma(-d.ds, q, as);
+  return __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(rem, d.r, q, f), d.b, a);
+}
+
 // Full-state access to an inner iterate.  FP64 storage holds the state
 // itself; FP32 storage holds its deviation from the case's reference state.
 struct NoOff {
@@ -57,37 +91,44 @@ template <> struct OffOf<float> { using type = RefOff; };
 template <int D>
 __device__ __forceinline__ void roe_flux(const double (&L)[5], const double (&R)[5], double e,
                                          double (&F)[5], double& dP) {
+  // Divisions sharing a denominator go through DivBy (the same bits).
   const double lr = L[0];
-  const double lq0 = L[1] / lr, lq1 = L[2] / lr, lq2 = L[3] / lr;
+  const DivBy dl = div_by(lr);
+  const double lq0 = div_sh(dl, L[1]), lq1 = div_sh(dl, L[2]), lq2 = div_sh(dl, L[3]);
   const double lVV = lq0 * lq0 + lq1 * lq1 + lq2 * lq2;
   const double lP = (L[4] - 0.5 * lr * lVV) * KM1;
-  const double lC = KG * lP / lr;
-  const double lH = 0.5 * lVV + lC / KM1;
+  const double lC = div_sh(dl, KG * lP);
+  const DivBy dk = div_by(KM1);
+  const double lH = 0.5 * lVV + div_sh(dk, lC);
   const double rr = R[0];
-  const double rq0 = R[1] / rr, rq1 = R[2] / rr, rq2 = R[3] / rr;
+  const DivBy dr = div_by(rr);
+  const double rq0 = div_sh(dr, R[1]), rq1 = div_sh(dr, R[2]), rq2 = div_sh(dr, R[3]);
   const double rVV = rq0 * rq0 + rq1 * rq1 + rq2 * rq2;
   const double rP = (R[4] - 0.5 * rr * rVV) * KM1;
-  const double rC = KG * rP / rr;
-  const double rH = 0.5 * rVV + rC / KM1;
+  const double rC = div_sh(dr, KG * rP);
+  const double rH = 0.5 * rVV + div_sh(dk, rC);
 
   const double t5 = sqrt(lr), t6 = sqrt(rr), t4 = t5 + t6;
   const double rho = sqrt(lr * rr);
-  const double q0 = (t5 * lq0 + t6 * rq0) / t4;
-  const double q1 = (t5 * lq1 + t6 * rq1) / t4;
-  const double q2 = (t5 * lq2 + t6 * rq2) / t4;
+  const DivBy d4 = div_by(t4);
+  const double q0 = div_sh(d4, t5 * lq0 + t6 * rq0);
+  const double q1 = div_sh(d4, t5 * lq1 + t6 * rq1);
+  const double q2 = div_sh(d4, t5 * lq2 + t6 * rq2);
   const double VV = q0 * q0 + q1 * q1 + q2 * q2;
-  const double H = (t5 * lH + t6 * rH) / t4;
+  const double H = div_sh(d4, t5 * lH + t6 * rH);
   const double C = (H - 0.5 * VV) * KM1;
-  const double beta = rmax(VV / C, e);
+  const DivBy dC = div_by(C);
+  const double beta = rmax(div_sh(dC, VV), e);
   const double un = D == 0 ? q0 : (D == 1 ? q1 : q2);
   const double lun = D == 0 ? lq0 : (D == 1 ? lq1 : lq2);
   const double run = D == 0 ? rq0 : (D == 1 ? rq1 : rq2);
   const double temp = 0.5 * (1 + beta) * un;
   const double S = 0.5 * sqrt(4 * beta * C + un * un * (1 - beta) * (1 - beta));
-  const double temp1 = (rP - lP) / rho / beta / C;
-  const double tS = temp / S;
+  const double temp1 = div_sh(dC, (rP - lP) / rho / beta);
+  const DivBy dS = div_by(S);
+  const double tS = div_sh(dS, temp);
   const double temp2 = tS * (run - lun);
-  const double temp3 = 0.5 * (1 - beta) * un * temp / S;
+  const double temp3 = div_sh(dS, 0.5 * (1 - beta) * un * temp);
   const double an = fabs(un);
   const double deltaU = (S - temp3 - beta * an) * temp1 + temp2;
   const double deltaP = tS * (rP - lP) + (S - an + temp3) * rho * (run - lun);
@@ -322,14 +363,15 @@ __global__ __launch_bounds__(WG) void k_flux(const double* __restrict__ U,
 
   // Residual with the dual-time source.
   double Rk[5];
+  const DivBy d2t = div_by(2 * deltaT);
   {
     const double uu0 = U1[b], uq0 = U1q[b];
-    Rk[0] = -(3 * u[0] - 4 * uu0 + uq0) / (2 * deltaT) + Rf[0];
+    Rk[0] = div_sh(d2t, -(3 * u[0] - 4 * uu0 + uq0)) + Rf[0];
   }
 #pragma unroll
   for (int v = 1; v < 5; ++v) {
     const double uu = U1[b + (size_t)v * PLANE], uq = U1q[b + (size_t)v * PLANE];
-    const double Rp = -(3 * u[v] - 4 * uu + uq) / (2 * deltaT);
+    const double Rp = div_sh(d2t, -(3 * u[v] - 4 * uu + uq));
     Rk[v] = Rp + Rf[v] + vF[v];
   }
 
@@ -346,32 +388,33 @@ __global__ __launch_bounds__(WG) void k_flux(const double* __restrict__ U,
   const double temp3 = H - H * KG - VV + KG * (T + VV);
   double out[5];
   if (IF == 1) {
-    const double d11 = (2 * deltaT * (-2 * (-KM1 * (H - VV) - temp3 * beta) * deltaT + 3 * KG * T * beta * deltaTau)) / temp;
-    const double d12 = (-4 * U_ * temp2) / temp;
-    const double d13 = (-4 * V_ * temp2) / temp;
-    const double d14 = (-4 * W_ * temp2) / temp;
-    const double d15 = (4 * temp2) / temp;
+    const DivBy dT = div_by(temp);
+    const double d11 = div_sh(dT, (2 * deltaT * (-2 * (-KM1 * (H - VV) - temp3 * beta) * deltaT + 3 * KG * T * beta * deltaTau)));
+    const double d12 = div_sh(dT, (-4 * U_ * temp2));
+    const double d13 = div_sh(dT, (-4 * V_ * temp2));
+    const double d14 = div_sh(dT, (-4 * W_ * temp2));
+    const double d15 = div_sh(dT, (4 * temp2));
     out[0] = deltaTau * (d11 * Rk[0] + d12 * Rk[1] + d13 * Rk[2] + d14 * Rk[3] + d15 * Rk[4]);
-    const double d21 = (4 * U_ * temp3 * (beta - 1) * deltaT * deltaT) / temp;
-    const double d22 = (2 * deltaT * (2 * KG * (T - U_ * U_ * (beta - 1)) * deltaT + 2 * U_ * U_ * (beta - 1) * deltaT + 3 * KG * T * beta * deltaTau)) / temp;
-    const double d23 = (-4 * U_ * V_ * temp2) / temp;
-    const double d24 = (-4 * U_ * W_ * temp2) / temp;
-    const double d25 = (4 * U_ * temp2) / temp;
+    const double d21 = div_sh(dT, (4 * U_ * temp3 * (beta - 1) * deltaT * deltaT));
+    const double d22 = div_sh(dT, (2 * deltaT * (2 * KG * (T - U_ * U_ * (beta - 1)) * deltaT + 2 * U_ * U_ * (beta - 1) * deltaT + 3 * KG * T * beta * deltaTau)));
+    const double d23 = div_sh(dT, (-4 * U_ * V_ * temp2));
+    const double d24 = div_sh(dT, (-4 * U_ * W_ * temp2));
+    const double d25 = div_sh(dT, (4 * U_ * temp2));
     out[1] = deltaTau * (d21 * Rk[0] + d22 * Rk[1] + d23 * Rk[2] + d24 * Rk[3] + d25 * Rk[4]);
-    const double d31 = (4 * V_ * temp3 * (beta - 1) * deltaT * deltaT) / temp;
-    const double d33 = (2 * deltaT * (2 * KG * (T - V_ * V_ * (beta - 1)) * deltaT + 2 * V_ * V_ * (beta - 1) * deltaT + 3 * KG * T * beta * deltaTau)) / temp;
-    const double d34 = (-4 * V_ * W_ * temp2) / temp;
-    const double d35 = (4 * V_ * temp2) / temp;
+    const double d31 = div_sh(dT, (4 * V_ * temp3 * (beta - 1) * deltaT * deltaT));
+    const double d33 = div_sh(dT, (2 * deltaT * (2 * KG * (T - V_ * V_ * (beta - 1)) * deltaT + 2 * V_ * V_ * (beta - 1) * deltaT + 3 * KG * T * beta * deltaTau)));
+    const double d34 = div_sh(dT, (-4 * V_ * W_ * temp2));
+    const double d35 = div_sh(dT, (4 * V_ * temp2));
     out[2] = deltaTau * (d31 * Rk[0] + d23 * Rk[1] + d33 * Rk[2] + d34 * Rk[3] + d35 * Rk[4]);
-    const double d41 = (4 * W_ * temp3 * (beta - 1) * deltaT * deltaT) / temp;
-    const double d44 = (2 * deltaT * (2 * KG * (T - W_ * W_ * (beta - 1)) * deltaT + 2 * W_ * W_ * (beta - 1) * deltaT + 3 * KG * T * beta * deltaTau)) / temp;
-    const double d45 = (4 * W_ * temp2) / temp;
+    const double d41 = div_sh(dT, (4 * W_ * temp3 * (beta - 1) * deltaT * deltaT));
+    const double d44 = div_sh(dT, (2 * deltaT * (2 * KG * (T - W_ * W_ * (beta - 1)) * deltaT + 2 * W_ * W_ * (beta - 1) * deltaT + 3 * KG * T * beta * deltaTau)));
+    const double d45 = div_sh(dT, (4 * W_ * temp2));
     out[3] = deltaTau * (d41 * Rk[0] + d24 * Rk[1] + d34 * Rk[2] + d44 * Rk[3] + d45 * Rk[4]);
-    const double d51 = (4 * H * temp3 * (beta - 1) * deltaT * deltaT) / temp;
-    const double d52 = (-4 * H * U_ * temp2) / temp;
-    const double d53 = (-4 * H * V_ * temp2) / temp;
-    const double d54 = (-4 * H * W_ * temp2) / temp;
-    const double d55 = (2 * deltaT * (2 * H * KM1 * (beta - 1) * deltaT + KG * T * (2 * deltaT + 3 * beta * deltaTau))) / temp;
+    const double d51 = div_sh(dT, (4 * H * temp3 * (beta - 1) * deltaT * deltaT));
+    const double d52 = div_sh(dT, (-4 * H * U_ * temp2));
+    const double d53 = div_sh(dT, (-4 * H * V_ * temp2));
+    const double d54 = div_sh(dT, (-4 * H * W_ * temp2));
+    const double d55 = div_sh(dT, (2 * deltaT * (2 * H * KM1 * (beta - 1) * deltaT + KG * T * (2 * deltaT + 3 * beta * deltaTau))));
     out[4] = deltaTau * (d51 * Rk[0] + d52 * Rk[1] + d53 * Rk[2] + d54 * Rk[3] + d55 * Rk[4]);
   } else {
 #pragma unroll
@@ -606,7 +649,8 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
         for (int v = 0; v < 5; ++v) q[v] = U[o + (size_t)v * PLANE];
       }
       const double r = q[0];
-      const double u = q[1] / r, v = q[2] / r, w = q[3] / r;
+      const DivBy dq = div_by(r);
+    const double u = div_sh(dq, q[1]), v = div_sh(dq, q[2]), w = div_sh(dq, q[3]);
       const double P = (q[4] - 0.5 * r * (u * u + v * v + w * w)) * KM1;
       pr[0][slot * PR_PLANE + n] = u;
       pr[1][slot * PR_PLANE + n] = v;
@@ -629,7 +673,8 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
 #pragma unroll
     for (int v = 0; v < 5; ++v) q[v] = 0.5 * (cpl[v * 400 + ea] + cpl[v * 400 + eb]);
     const double r = q[0];
-    const double u = q[1] / r, v = q[2] / r, w = q[3] / r;
+    const DivBy dq = div_by(r);
+    const double u = div_sh(dq, q[1]), v = div_sh(dq, q[2]), w = div_sh(dq, q[3]);
     const double P = (q[4] - 0.5 * r * (u * u + v * v + w * w)) * KM1;
     const int n = (jj - 1) * 18 + (kk - 1), slot = ip % 3;
     pr[0][slot * PR_PLANE + n] = u;
@@ -929,14 +974,15 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
 
     // ---- residual with the dual-time source, preconditioner, increment ----
     double Rk[5];
+    const DivBy d2t = div_by(2 * deltaT);
     {
       const double uu0 = tu1[0], uq0 = tuq[0];
-      Rk[0] = -(3 * u[0] - 4 * uu0 + uq0) / (2 * deltaT) + Rf[0];
+      Rk[0] = div_sh(d2t, -(3 * u[0] - 4 * uu0 + uq0)) + Rf[0];
     }
 #pragma unroll
     for (int v = 1; v < 5; ++v) {
       const double uu = tu1[v], uq = tuq[v];
-      const double Rp = -(3 * u[v] - 4 * uu + uq) / (2 * deltaT);
+      const double Rp = div_sh(d2t, -(3 * u[v] - 4 * uu + uq));
       Rk[v] = Rp + Rf[v] + vF[v];
     }
     double out[5] = {0, 0, 0, 0, 0};
@@ -951,32 +997,33 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
       const double temp = KG * T * (2 * deltaT + 3 * deltaTau) * (2 * deltaT + 3 * beta * deltaTau);
       const double temp2 = KM1 * (beta - 1) * deltaT * deltaT;
       const double temp3 = H - H * KG - VV + KG * (T + VV);
-      const double d11 = (2 * deltaT * (-2 * (-KM1 * (H - VV) - temp3 * beta) * deltaT + 3 * KG * T * beta * deltaTau)) / temp;
-      const double d12 = (-4 * U_ * temp2) / temp;
-      const double d13 = (-4 * V_ * temp2) / temp;
-      const double d14 = (-4 * W_ * temp2) / temp;
-      const double d15 = (4 * temp2) / temp;
+      const DivBy dT = div_by(temp);
+      const double d11 = div_sh(dT, (2 * deltaT * (-2 * (-KM1 * (H - VV) - temp3 * beta) * deltaT + 3 * KG * T * beta * deltaTau)));
+      const double d12 = div_sh(dT, (-4 * U_ * temp2));
+      const double d13 = div_sh(dT, (-4 * V_ * temp2));
+      const double d14 = div_sh(dT, (-4 * W_ * temp2));
+      const double d15 = div_sh(dT, (4 * temp2));
       out[0] = deltaTau * (d11 * Rk[0] + d12 * Rk[1] + d13 * Rk[2] + d14 * Rk[3] + d15 * Rk[4]);
-      const double d21 = (4 * U_ * temp3 * (beta - 1) * deltaT * deltaT) / temp;
-      const double d22 = (2 * deltaT * (2 * KG * (T - U_ * U_ * (beta - 1)) * deltaT + 2 * U_ * U_ * (beta - 1) * deltaT + 3 * KG * T * beta * deltaTau)) / temp;
-      const double d23 = (-4 * U_ * V_ * temp2) / temp;
-      const double d24 = (-4 * U_ * W_ * temp2) / temp;
-      const double d25 = (4 * U_ * temp2) / temp;
+      const double d21 = div_sh(dT, (4 * U_ * temp3 * (beta - 1) * deltaT * deltaT));
+      const double d22 = div_sh(dT, (2 * deltaT * (2 * KG * (T - U_ * U_ * (beta - 1)) * deltaT + 2 * U_ * U_ * (beta - 1) * deltaT + 3 * KG * T * beta * deltaTau)));
+      const double d23 = div_sh(dT, (-4 * U_ * V_ * temp2));
+      const double d24 = div_sh(dT, (-4 * U_ * W_ * temp2));
+      const double d25 = div_sh(dT, (4 * U_ * temp2));
       out[1] = deltaTau * (d21 * Rk[0] + d22 * Rk[1] + d23 * Rk[2] + d24 * Rk[3] + d25 * Rk[4]);
-      const double d31 = (4 * V_ * temp3 * (beta - 1) * deltaT * deltaT) / temp;
-      const double d33 = (2 * deltaT * (2 * KG * (T - V_ * V_ * (beta - 1)) * deltaT + 2 * V_ * V_ * (beta - 1) * deltaT + 3 * KG * T * beta * deltaTau)) / temp;
-      const double d34 = (-4 * V_ * W_ * temp2) / temp;
-      const double d35 = (4 * V_ * temp2) / temp;
+      const double d31 = div_sh(dT, (4 * V_ * temp3 * (beta - 1) * deltaT * deltaT));
+      const double d33 = div_sh(dT, (2 * deltaT * (2 * KG * (T - V_ * V_ * (beta - 1)) * deltaT + 2 * V_ * V_ * (beta - 1) * deltaT + 3 * KG * T * beta * deltaTau)));
+      const double d34 = div_sh(dT, (-4 * V_ * W_ * temp2));
+      const double d35 = div_sh(dT, (4 * V_ * temp2));
       out[2] = deltaTau * (d31 * Rk[0] + d23 * Rk[1] + d33 * Rk[2] + d34 * Rk[3] + d35 * Rk[4]);
-      const double d41 = (4 * W_ * temp3 * (beta - 1) * deltaT * deltaT) / temp;
-      const double d44 = (2 * deltaT * (2 * KG * (T - W_ * W_ * (beta - 1)) * deltaT + 2 * W_ * W_ * (beta - 1) * deltaT + 3 * KG * T * beta * deltaTau)) / temp;
-      const double d45 = (4 * W_ * temp2) / temp;
+      const double d41 = div_sh(dT, (4 * W_ * temp3 * (beta - 1) * deltaT * deltaT));
+      const double d44 = div_sh(dT, (2 * deltaT * (2 * KG * (T - W_ * W_ * (beta - 1)) * deltaT + 2 * W_ * W_ * (beta - 1) * deltaT + 3 * KG * T * beta * deltaTau)));
+      const double d45 = div_sh(dT, (4 * W_ * temp2));
       out[3] = deltaTau * (d41 * Rk[0] + d24 * Rk[1] + d34 * Rk[2] + d44 * Rk[3] + d45 * Rk[4]);
-      const double d51 = (4 * H * temp3 * (beta - 1) * deltaT * deltaT) / temp;
-      const double d52 = (-4 * H * U_ * temp2) / temp;
-      const double d53 = (-4 * H * V_ * temp2) / temp;
-      const double d54 = (-4 * H * W_ * temp2) / temp;
-      const double d55 = (2 * deltaT * (2 * H * KM1 * (beta - 1) * deltaT + KG * T * (2 * deltaT + 3 * beta * deltaTau))) / temp;
+      const double d51 = div_sh(dT, (4 * H * temp3 * (beta - 1) * deltaT * deltaT));
+      const double d52 = div_sh(dT, (-4 * H * U_ * temp2));
+      const double d53 = div_sh(dT, (-4 * H * V_ * temp2));
+      const double d54 = div_sh(dT, (-4 * H * W_ * temp2));
+      const double d55 = div_sh(dT, (2 * deltaT * (2 * H * KM1 * (beta - 1) * deltaT + KG * T * (2 * deltaT + 3 * beta * deltaTau))));
       out[4] = deltaTau * (d51 * Rk[0] + d52 * Rk[1] + d53 * Rk[2] + d54 * Rk[3] + d55 * Rk[4]);
     }
     // The pushing kernel reads none of Rku1, U1p1, U1p2 and Residual1 back;
@@ -2015,6 +2062,53 @@ __global__ __launch_bounds__(WG) void k_copy_d(const double* __restrict__ src,
                                                double* __restrict__ dst, size_t n) {
   for (size_t g = (size_t)blockIdx.x * WG + threadIdx.x; g < n; g += (size_t)gridDim.x * WG)
     dst[g] = src[g];
+}
+
+namespace {
+// DivBy/div_sh against a / b: per thread one denominator and four numerators
+// from a seeded hash (random signs, significands and exponents in
+// [-126, 126]; about one numerator in 16 is a signed zero).
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+__device__ __forceinline__ double test_value(unsigned long long r) {
+  const double m = 1.0 + (double)(r >> 12) * 0x1p-52;
+  const double v = ldexp(m, (int)((r >> 1) % 253) - 126);
+  return (r & 1) ? -v : v;
+}
+__global__ __launch_bounds__(WG) void k_div_selftest(unsigned long long seed, int n,
+                                                     unsigned long long* __restrict__ bad) {
+  const int g = blockIdx.x * WG + threadIdx.x;
+  if (g >= n) return;
+  const unsigned long long base = mix64(seed ^ ((unsigned long long)g << 20));
+  const double b = test_value(mix64(base));
+  double a[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    a[k] = test_value(mix64(base + 1 + k));
+    if ((mix64(base + 10 + k) & 15) == 0) a[k] = (k & 1) ? -0.0 : 0.0;
+  }
+  const DivBy d = div_by(b);
+  unsigned long long nb = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    nb += __double_as_longlong(div_sh(d, a[k])) != __double_as_longlong(a[k] / b);
+  if (nb) atomicAdd(bad, nb);
+}
+}  // namespace
+
+int div_selftest(unsigned long long seed, int n, unsigned long long* mismatches) {
+  unsigned long long* d = nullptr;
+  if (n < 0 || hipMalloc((void**)&d, sizeof(unsigned long long)) != hipSuccess) return 1;
+  int rc = hipMemset(d, 0, sizeof(unsigned long long)) != hipSuccess;
+  if (!rc && n > 0) hipLaunchKernelGGL(k_div_selftest, dim3((n + WG - 1) / WG), dim3(WG), 0, 0, seed, n, d);
+  rc = rc || hipGetLastError() != hipSuccess ||
+       hipMemcpy(mismatches, d, sizeof(unsigned long long), hipMemcpyDeviceToHost) != hipSuccess;
+  (void)hipFree(d);
+  return rc;
 }
 
 void launch_copy_d(hipStream_t s, const double* src, double* dst, size_t n) {

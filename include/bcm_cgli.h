@@ -280,6 +280,12 @@ int bcm_engine_ghost_stats(bcm_engine* e, int* stats3);
  * schedule (bitwise the same results).  Not a reference interface: a test
  * and diagnostic hook. */
 int bcm_engine_set_ghost_lds_limit(bcm_engine* e, int slots);
+/* The FP64 kernels divide several numerators by one denominator with the
+ * denominator's part of the correctly rounded division formed once.  This
+ * runs n threads of four such quotients on random operands against plain
+ * division on the current device; *mismatches receives the number of quotients
+ * whose bits differ (0 expected).  Not a reference interface: a self-test. */
+int bcm_selftest_division(unsigned long long seed, int n, unsigned long long* mismatches);
 
 /* ------------------------------------------------------------------------ */
 /* 4. Case pool: several independent cases resident on one device.           */
