@@ -986,6 +986,8 @@ int engine_autotune(Engine* e, int* variant, int* nseg, std::string& err) {
     // The mixed flux kernel in 1, 2 or 4 runs per cube (bitwise the same
     // results), each timed as one whole RK stage on the resident cases, once
     // per size class of the pool; the iterate the stages write is restored.
+    // The candidates are timed in three interleaved rounds and each keeps its
+    // fastest round, so a clock ramp at the start favours none of them.
     const int cls = size_class(e);
     auto known = e->nseg_by_class.find(cls);
     if (known == e->nseg_by_class.end()) {
@@ -1003,27 +1005,33 @@ int engine_autotune(Engine* e, int* variant, int* nseg, std::string& err) {
         HIP_TRY(hipMemcpyAsync(saved.p[a], live[a], bytes, hipMemcpyDeviceToDevice, e->stream));
       }
       const bool full = e->halo_full_next;
-      float best = 1e30f;
-      int bn = 1;
-      for (int cand : {1, 2, 4}) {
-        e->mixed_nseg = cand;
-        engine_stage(e, 2, 0, full);   // warm up
-        HIP_TRY(hipEventRecord(e->ev0, e->stream));
-        for (int r = 0; r < 3; ++r) engine_stage(e, 2, 0, full);
-        HIP_TRY(hipEventRecord(e->ev1, e->stream));
-        HIP_TRY(hipEventSynchronize(e->ev1));
-        float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
-        e->Uf = live[0];
-        e->Uf2 = live[1];
-        for (int a = 0; a < 2; ++a)
-          HIP_TRY(hipMemcpyAsync(live[a], saved.p[a], bytes, hipMemcpyDeviceToDevice, e->stream));
-        if (ms < best) {
-          best = ms;
-          bn = cand;
+      const int cands[3] = {1, 2, 4};
+      float fastest[3] = {1e30f, 1e30f, 1e30f};
+      for (int round = 0; round < 3; ++round)
+        for (int c = 0; c < 3; ++c) {
+          e->mixed_nseg = cands[c];
+          engine_stage(e, 2, 0, full);   // warm up
+          HIP_TRY(hipEventRecord(e->ev0, e->stream));
+          for (int r = 0; r < 3; ++r) engine_stage(e, 2, 0, full);
+          HIP_TRY(hipEventRecord(e->ev1, e->stream));
+          HIP_TRY(hipEventSynchronize(e->ev1));
+          float ms = 0;
+          HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+          e->Uf = live[0];
+          e->Uf2 = live[1];
+          for (int a = 0; a < 2; ++a)
+            HIP_TRY(
+                hipMemcpyAsync(live[a], saved.p[a], bytes, hipMemcpyDeviceToDevice, e->stream));
+          fastest[c] = std::min(fastest[c], ms);
         }
-      }
       HIP_TRY(hipStreamSynchronize(e->stream));
+      int bn = cands[0];
+      float best = fastest[0];
+      for (int c = 1; c < 3; ++c)
+        if (fastest[c] < best) {
+          best = fastest[c];
+          bn = cands[c];
+        }
       known = e->nseg_by_class.emplace(cls, bn).first;
     }
     if (e->mixed_nseg != known->second) e->graph_valid = false;
@@ -1041,8 +1049,9 @@ int engine_autotune(Engine* e, int* variant, int* nseg, std::string& err) {
   // (The plane kernel and the segmented marching kernel, selectable with
   // set_flux, reduce over per-plane rows and are not candidates.)  Each is
   // timed as one whole RK stage on the resident cases, as the inner
-  // iterations of a step run it; every array a stage writes is saved first
-  // and restored afterwards.
+  // iterations of a step run it, in three interleaved rounds of which each
+  // keeps its fastest; every array a stage writes is saved first and
+  // restored afterwards.
   const int cand[2][3] = {{1, 1, 1}, {1, 1, 2}};
   double* const live[6] = {e->U, e->U2, e->U1p1, e->U1p2, e->Rku, e->Res};
   struct Saved {
@@ -1058,37 +1067,34 @@ int engine_autotune(Engine* e, int* variant, int* nseg, std::string& err) {
                            e->stream));
   }
   const bool full = e->halo_full_next;
-  float best = 1e30f;
-  int bv = e->flux_variant, bs = e->flux_nseg;
-  bool bf = e->flux_fuse, bp = e->flux_push;
-  for (auto& cv : cand) {
-    e->flux_variant = cv[0];
-    e->flux_nseg = cv[1];
-    e->flux_fuse = cv[2] != 0;
-    e->flux_push = cv[2] == 2;
-    e->push_aux = false;
-    engine_stage(e, 2, 0, full);  // warm up
-    HIP_TRY(hipEventRecord(e->ev0, e->stream));
-    for (int r = 0; r < 3; ++r) engine_stage(e, 2, 0, full);
-    HIP_TRY(hipEventRecord(e->ev1, e->stream));
-    e->push_aux = true;
-    HIP_TRY(hipEventSynchronize(e->ev1));
-    float ms = 0;
-    HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
-    e->U = live[0];
-    e->U2 = live[1];
-    for (int a = 0; a < 6; ++a)
-      HIP_TRY(hipMemcpyAsync(live[a], saved.p[a], pool_bytes(e), hipMemcpyDeviceToDevice,
-                             e->stream));
-    if (ms < best) {
-      best = ms;
-      bv = cv[0];
-      bs = cv[1];
-      bf = cv[2] != 0;
-      bp = cv[2] == 2;
+  float fastest[2] = {1e30f, 1e30f};
+  for (int round = 0; round < 3; ++round)
+    for (int c = 0; c < 2; ++c) {
+      const int* cv = cand[c];
+      e->flux_variant = cv[0];
+      e->flux_nseg = cv[1];
+      e->flux_fuse = cv[2] != 0;
+      e->flux_push = cv[2] == 2;
+      e->push_aux = false;
+      engine_stage(e, 2, 0, full);  // warm up
+      HIP_TRY(hipEventRecord(e->ev0, e->stream));
+      for (int r = 0; r < 3; ++r) engine_stage(e, 2, 0, full);
+      HIP_TRY(hipEventRecord(e->ev1, e->stream));
+      e->push_aux = true;
+      HIP_TRY(hipEventSynchronize(e->ev1));
+      float ms = 0;
+      HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+      e->U = live[0];
+      e->U2 = live[1];
+      for (int a = 0; a < 6; ++a)
+        HIP_TRY(hipMemcpyAsync(live[a], saved.p[a], pool_bytes(e), hipMemcpyDeviceToDevice,
+                               e->stream));
+      fastest[c] = std::min(fastest[c], ms);
     }
-  }
   HIP_TRY(hipStreamSynchronize(e->stream));
+  const int* win = cand[fastest[1] < fastest[0] ? 1 : 0];
+  const int bv = win[0], bs = win[1];
+  const bool bf = win[2] != 0, bp = win[2] == 2;
   e->flux_push = bp;
   e->flux_variant = bv;
   e->flux_nseg = bs;
