@@ -7,6 +7,7 @@
 // BCM_Flux_XYZ_Viscous_Runge_kutta.cpp.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <type_traits>
 
 #include "bcm_kernels.h"
 
@@ -577,8 +578,9 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
   const DevPhys ph = pt.phys[cc];
   const int len = 16 / nseg;
   const int i0 = 2 + seg * len, i1 = i0 + len - 1;
-  const int t = threadIdx.x;
-  const int j = (t >> 4) + 2, k = (t & 15) + 2;
+  // t, j and k are re-formed at the top of every plane iteration (below).
+  int t = threadIdx.x;
+  int j = (t >> 4) + 2, k = (t & 15) + 2;
   const double dx = cube_size[c] / 16;
   const double dy = dx, dz = dx;
   const double inv = 1. / dx;
@@ -739,6 +741,15 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
   };
 #pragma unroll 1
   for (int i = i0; i <= i1; ++i) {
+    // The lane index passes through an empty asm statement, so every
+    // per-lane quantity derived from it (LDS addresses, face and flag
+    // indices) is computed inside the iteration that uses it rather than
+    // hoisted before the loop and held across it: held, they exceed the
+    // register budget of two waves per SIMD and spill to scratch memory,
+    // whose reloads sit on the plane's critical path.
+    asm volatile("" : "+v"(t));
+    j = (t >> 4) + 2;
+    k = (t & 15) + 2;
     // Plane i-4 was last read during iteration i-1 (backward-face stencil of
     // plane i-1 reaches i-4); the barrier that ended that iteration makes its
     // update safe now.
@@ -1252,6 +1263,10 @@ __global__ __launch_bounds__(WG) void k_reduce(const double* __restrict__ partia
 // .cpp:43-118), then Y (BCM_Y_boundary_condition.cpp:36-80), then Z
 // (BCM_Z_boundary_condition.cpp:42-92).  mask bits: 1 x-min, 2 x-max, 4 y-min,
 // 8 y-max, 16 z-min, 32 z-max.
+// In each of the three phases the cells read (the layers next to the ghost
+// layers) and the cells written (the ghost layers) are disjoint, so every
+// lane issues all its loads of a phase before its first store: one memory
+// round trip per phase instead of one per value.
 template <typename T>
 __device__ __forceinline__ void bc_cube(T* __restrict__ U, int c, int m, PhysTab pt) {
   const DevPhys ph = pt.phys[pt.cube_case[c]];
@@ -1259,17 +1274,23 @@ __device__ __forceinline__ void bc_cube(T* __restrict__ U, int c, int m, PhysTab
   const int t = threadIdx.x;
   {
     const int j = (t >> 4) + 2, k = (t & 15) + 2;
+    T lo[5], hi[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      if (m & 1) lo[v] = U[sidx(c, v, 2, j, k)];
+      if (m & 2) hi[v] = U[sidx(c, v, 17, j, k)];
+    }
     if (m & 1) {
-      const double rho = F.ld(U, sidx(c, 0, 2, j, k), 0);
-      const double uu = F.ld(U, sidx(c, 1, 2, j, k), 1) / rho;
-      const double vv = F.ld(U, sidx(c, 2, 2, j, k), 2) / rho;
-      const double ww = F.ld(U, sidx(c, 3, 2, j, k), 3) / rho;
+      const double rho = F.ld(lo, 0, 0);
+      const double uu = F.ld(lo, 1, 1) / rho;
+      const double vv = F.ld(lo, 2, 2) / rho;
+      const double ww = F.ld(lo, 3, 3) / rho;
       double VV = uu * uu + vv * vv + ww * ww;
-      const double P = (F.ld(U, sidx(c, 4, 2, j, k), 4) - 0.5 * rho * VV) * KM1;
+      const double P = (F.ld(lo, 4, 4) - 0.5 * rho * VV) * KM1;
       VV = ph.Uin * ph.Uin + ph.V0 * ph.V0 + ph.W0 * ph.W0;
       const double E = P / KM1 + 0.5 * rho * VV;
       for (int i = 0; i < 2; ++i) {
-        U[sidx(c, 0, i, j, k)] = U[sidx(c, 0, 2, j, k)];
+        U[sidx(c, 0, i, j, k)] = lo[0];
         F.st(U, sidx(c, 1, i, j, k), 1, rho * ph.Uin);
         F.st(U, sidx(c, 2, i, j, k), 2, rho * ph.V0);
         F.st(U, sidx(c, 3, i, j, k), 3, rho * ph.W0);
@@ -1277,52 +1298,84 @@ __device__ __forceinline__ void bc_cube(T* __restrict__ U, int c, int m, PhysTab
       }
     }
     if (m & 2)
+#pragma unroll
       for (int v = 0; v < 5; ++v) {
-        const double x = U[sidx(c, v, 17, j, k)];
-        U[sidx(c, v, 18, j, k)] = x;
-        U[sidx(c, v, 19, j, k)] = x;
+        U[sidx(c, v, 18, j, k)] = hi[v];
+        U[sidx(c, v, 19, j, k)] = hi[v];
       }
   }
   __syncthreads();
-  for (int w = t; w < 20 * 16; w += WG) {
-    const int i = w >> 4, k = (w & 15) + 2;
-    if (m & 4)
-      for (int v = 0; v < 5; ++v) {
-        const double x = U[sidx(c, v, i, 2, k)];
-        U[sidx(c, v, i, 1, k)] = x;
-        U[sidx(c, v, i, 0, k)] = x;
-      }
-    if (m & 8)
-      for (int v = 0; v < 5; ++v) {
-        const double x = U[sidx(c, v, i, 17, k)];
-        U[sidx(c, v, i, 18, k)] = x;
-        U[sidx(c, v, i, 19, k)] = x;
-      }
-  }
-  __syncthreads();
-  for (int w = t; w < 400; w += WG) {
-    const int i = w / 20, j = w % 20;
-    // Interior planes of a face marked ZBC_IN_FLUX get their z-BC from the
-    // flux kernel.
-    const bool inner = i >= 2 && i <= 17;
-    if ((m & 16) && !(inner && (m & ZBC_IN_FLUX)))
-      for (int v = 0; v < 5; ++v) {
-        if (v == 3) {
-          const double y = -F.ld(U, sidx(c, v, i, j, 2), 3);
-          F.st(U, sidx(c, v, i, j, 1), 3, y);
-          F.st(U, sidx(c, v, i, j, 0), 3, y);
-        } else {
-          const T x = U[sidx(c, v, i, j, 2)];
-          U[sidx(c, v, i, j, 1)] = x;
-          U[sidx(c, v, i, j, 0)] = x;
+  {
+    // Rows (i, k) of the y-faces, i = 0..19: w = t and t + 256 (w < 320).
+    T lo[2][5], hi[2][5];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int w = t + h * WG, i = w >> 4, k = (w & 15) + 2;
+      if (w < 20 * 16)
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          if (m & 4) lo[h][v] = U[sidx(c, v, i, 2, k)];
+          if (m & 8) hi[h][v] = U[sidx(c, v, i, 17, k)];
         }
-      }
-    if ((m & 32) && !(inner && (m & (ZBC_IN_FLUX << 1))))
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int w = t + h * WG, i = w >> 4, k = (w & 15) + 2;
+      if (w < 20 * 16)
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          if (m & 4) {
+            U[sidx(c, v, i, 1, k)] = lo[h][v];
+            U[sidx(c, v, i, 0, k)] = lo[h][v];
+          }
+          if (m & 8) {
+            U[sidx(c, v, i, 18, k)] = hi[h][v];
+            U[sidx(c, v, i, 19, k)] = hi[h][v];
+          }
+        }
+    }
+  }
+  __syncthreads();
+  {
+    // Rows (i, j) of the z-faces: w = t and t + 256 (w < 400).  Interior
+    // planes of a face marked ZBC_IN_FLUX get their z-BC from the flux
+    // kernel.
+    T lo[2][5], hi[2][5];
+    bool dlo[2], dhi[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int w = t + h * WG, i = w / 20, j = w % 20;
+      const bool inner = i >= 2 && i <= 17;
+      dlo[h] = w < 400 && (m & 16) && !(inner && (m & ZBC_IN_FLUX));
+      dhi[h] = w < 400 && (m & 32) && !(inner && (m & (ZBC_IN_FLUX << 1)));
+#pragma unroll
       for (int v = 0; v < 5; ++v) {
-        const double x = U[sidx(c, v, i, j, 17)];
-        U[sidx(c, v, i, j, 18)] = x;
-        U[sidx(c, v, i, j, 19)] = x;
+        if (dlo[h]) lo[h][v] = U[sidx(c, v, i, j, 2)];
+        if (dhi[h]) hi[h][v] = U[sidx(c, v, i, j, 17)];
       }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int w = t + h * WG, i = w / 20, j = w % 20;
+      if (dlo[h])
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          if (v == 3) {
+            const double y = -F.ld(lo[h], 3, 3);
+            F.st(U, sidx(c, v, i, j, 1), 3, y);
+            F.st(U, sidx(c, v, i, j, 0), 3, y);
+          } else {
+            U[sidx(c, v, i, j, 1)] = lo[h][v];
+            U[sidx(c, v, i, j, 0)] = lo[h][v];
+          }
+        }
+      if (dhi[h])
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          U[sidx(c, v, i, j, 18)] = hi[h][v];
+          U[sidx(c, v, i, j, 19)] = hi[h][v];
+        }
+    }
   }
 }
 
@@ -1635,45 +1688,57 @@ __global__ __launch_bounds__(WG) void k_ghost(T* __restrict__ U, GhostTables gt,
 
 // Edge-ghost averaging of one cube in the reference's three sequential passes
 // (Interface.cpp:5640-5732): i-edges, then j-edges, then k-edges.  Called by a
-// whole workgroup; the face ghosts it reads must be complete.
+// whole workgroup; the face ghosts it reads must be complete.  A pass reads
+// face ghost cells and writes edge cells only, so each lane loads both of its
+// items (w = t and t + 256) before storing either.
 template <typename T>
 __device__ __forceinline__ void edge_fill(T* __restrict__ U, int c) {
   const int t = threadIdx.x;
-  // i-edges at (j,k) in {1,18}^2, i = 2..17.
-  for (int w = t; w < 16 * 4 * 5; w += WG) {
-    const int i = (w >> 2) / 5 + 2, e = w & 3, v = (w >> 2) % 5;
-    double x;
-    if (e == 0) x = 0.5 * (U[sidx(c, v, i, 1, 2)] + U[sidx(c, v, i, 2, 1)]);
-    else if (e == 1) x = 0.5 * (U[sidx(c, v, i, 18, 2)] + U[sidx(c, v, i, 17, 1)]);
-    else if (e == 2) x = 0.5 * (U[sidx(c, v, i, 1, 17)] + U[sidx(c, v, i, 2, 18)]);
-    else x = 0.5 * (U[sidx(c, v, i, 18, 17)] + U[sidx(c, v, i, 17, 18)]);
-    const int jj = (e == 1 || e == 3) ? 18 : 1, kk = (e >= 2) ? 18 : 1;
-    U[sidx(c, v, i, jj, kk)] = x;
-  }
+  // Pass a (0: i-edges at (j,k) in {1,18}^2, i = 2..17; 1: j-edges at (i,k),
+  // j = 1..18; 2: k-edges at (i,j), k = 1..18).  Item w: edge e = w & 3,
+  // variable v = (w >> 2) % 5, position along the edge (w >> 2) / 5.
+  auto pass = [&](auto axis) {
+    constexpr int A = decltype(axis)::value;
+    constexpr int n = (A == 0 ? 16 : 18) * 4 * 5, p0 = A == 0 ? 2 : 1;
+    T xa[2], xb[2];
+    size_t dst[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int w = t + h * WG;
+      if (w >= n) continue;
+      const int q = (w >> 2) / 5 + p0, e = w & 3, v = (w >> 2) % 5;
+      size_t a, b;
+      if (A == 0) {
+        if (e == 0) a = sidx(c, v, q, 1, 2), b = sidx(c, v, q, 2, 1);
+        else if (e == 1) a = sidx(c, v, q, 18, 2), b = sidx(c, v, q, 17, 1);
+        else if (e == 2) a = sidx(c, v, q, 1, 17), b = sidx(c, v, q, 2, 18);
+        else a = sidx(c, v, q, 18, 17), b = sidx(c, v, q, 17, 18);
+        dst[h] = sidx(c, v, q, (e == 1 || e == 3) ? 18 : 1, (e >= 2) ? 18 : 1);
+      } else if (A == 1) {
+        if (e == 0) a = sidx(c, v, 1, q, 2), b = sidx(c, v, 2, q, 1);
+        else if (e == 1) a = sidx(c, v, 2, q, 18), b = sidx(c, v, 1, q, 17);
+        else if (e == 2) a = sidx(c, v, 17, q, 1), b = sidx(c, v, 18, q, 2);
+        else a = sidx(c, v, 17, q, 18), b = sidx(c, v, 18, q, 17);
+        dst[h] = sidx(c, v, (e >= 2) ? 18 : 1, q, (e == 1 || e == 3) ? 18 : 1);
+      } else {
+        if (e == 0) a = sidx(c, v, 1, 2, q), b = sidx(c, v, 2, 1, q);
+        else if (e == 1) a = sidx(c, v, 2, 18, q), b = sidx(c, v, 1, 17, q);
+        else if (e == 2) a = sidx(c, v, 17, 1, q), b = sidx(c, v, 18, 2, q);
+        else a = sidx(c, v, 17, 18, q), b = sidx(c, v, 18, 17, q);
+        dst[h] = sidx(c, v, (e >= 2) ? 18 : 1, (e == 1 || e == 3) ? 18 : 1, q);
+      }
+      xa[h] = U[a];
+      xb[h] = U[b];
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+      if (t + h * WG < n) U[dst[h]] = 0.5 * (xa[h] + xb[h]);
+  };
+  pass(std::integral_constant<int, 0>());
   __syncthreads();
-  // j-edges at (i,k) in {1,18}^2, j = 1..18.
-  for (int w = t; w < 18 * 4 * 5; w += WG) {
-    const int j = (w >> 2) / 5 + 1, e = w & 3, v = (w >> 2) % 5;
-    double x;
-    if (e == 0) x = 0.5 * (U[sidx(c, v, 1, j, 2)] + U[sidx(c, v, 2, j, 1)]);
-    else if (e == 1) x = 0.5 * (U[sidx(c, v, 2, j, 18)] + U[sidx(c, v, 1, j, 17)]);
-    else if (e == 2) x = 0.5 * (U[sidx(c, v, 17, j, 1)] + U[sidx(c, v, 18, j, 2)]);
-    else x = 0.5 * (U[sidx(c, v, 17, j, 18)] + U[sidx(c, v, 18, j, 17)]);
-    const int ii = (e >= 2) ? 18 : 1, kk = (e == 1 || e == 3) ? 18 : 1;
-    U[sidx(c, v, ii, j, kk)] = x;
-  }
+  pass(std::integral_constant<int, 1>());
   __syncthreads();
-  // k-edges at (i,j) in {1,18}^2, k = 1..18.
-  for (int w = t; w < 18 * 4 * 5; w += WG) {
-    const int k = (w >> 2) / 5 + 1, e = w & 3, v = (w >> 2) % 5;
-    double x;
-    if (e == 0) x = 0.5 * (U[sidx(c, v, 1, 2, k)] + U[sidx(c, v, 2, 1, k)]);
-    else if (e == 1) x = 0.5 * (U[sidx(c, v, 2, 18, k)] + U[sidx(c, v, 1, 17, k)]);
-    else if (e == 2) x = 0.5 * (U[sidx(c, v, 17, 1, k)] + U[sidx(c, v, 18, 2, k)]);
-    else x = 0.5 * (U[sidx(c, v, 17, 18, k)] + U[sidx(c, v, 18, 17, k)]);
-    const int ii = (e >= 2) ? 18 : 1, jj = (e == 1 || e == 3) ? 18 : 1;
-    U[sidx(c, v, ii, jj, k)] = x;
-  }
+  pass(std::integral_constant<int, 2>());
 }
 
 // ---------------------------------------------------------------------------
