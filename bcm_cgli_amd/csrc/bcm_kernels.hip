@@ -613,51 +613,60 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
     if (zhi && kk > 17 && cell >= 800 && cell < 7200) return cell - kk + 17;
     return cell;
   };
-  // The 68 cells beyond one per lane go to lanes 188..255, most of which
-  // stage no part of the plane.
-  auto fill_prims = [&](int ip) {
-    const int slot = ip % 3;
+  // Primitive cell n (j,k = n / 18 + 1, n % 18 + 1) of plane ip: its
+  // conserved values (fill_load), then u, v, w and T into the ring slot
+  // (fill_store).
+  auto fill_load = [&](int ip, int n, double (&q)[5]) {
     const bool gplane = ip == 1 || ip == 18;
+    const int jj = n / 18 + 1, kk = n % 18 + 1;
+    if (PUSH && gplane) {
+      int ea = 0, eb = 0;
+      if (!edge_sources(ip, jj, kk, ea, eb)) ea = eb = ip * 400 + jj * 20 + kk;
+      bool na, nb;
+      ea = zsrc(ea, na);
+      eb = zsrc(eb, nb);
+      const size_t cb = (size_t)c * CUBE_STRIDE;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        const double x = U[cb + (size_t)v * PLANE + ea], y = U[cb + (size_t)v * PLANE + eb];
+        q[v] = 0.5 * ((v == 3 && na ? -x : x) + (v == 3 && nb ? -y : y));
+      }
+    } else if (PUSH) {
+      bool na;
+      const int ea = zsrc(ip * 400 + jj * 20 + kk, na);
+      const size_t cb = (size_t)c * CUBE_STRIDE;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        const double x = U[cb + (size_t)v * PLANE + ea];
+        q[v] = v == 3 && na ? -x : x;
+      }
+    } else {
+      const size_t o = sidx(c, 0, ip, jj, kk);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) q[v] = U[o + (size_t)v * PLANE];
+    }
+  };
+  auto fill_store = [&](int ip, int n, const double (&q)[5]) {
+    const int slot = ip % 3;
+    const double r = q[0];
+    const DivBy dq = div_by(r);
+    const double u = div_sh(dq, q[1]), v = div_sh(dq, q[2]), w = div_sh(dq, q[3]);
+    const double P = (q[4] - 0.5 * r * (u * u + v * v + w * w)) * KM1;
+    pr[0][slot * PR_PLANE + n] = u;
+    pr[1][slot * PR_PLANE + n] = v;
+    pr[2][slot * PR_PLANE + n] = w;
+    pr[3][slot * PR_PLANE + n] = P / (r * RG);
+  };
+  // The whole 18x18 plane: the 68 cells beyond one per lane go to lanes
+  // 188..255, most of which stage no part of the plane.
+  auto fill_prims = [&](int ip) {
 #pragma unroll 1
     for (int rd = 0; rd < 2; ++rd) {
       const int n = rd == 0 ? t : WG + t - 188;
       if (rd == 1 && t < 188) continue;
-      const int jj = n / 18 + 1, kk = n % 18 + 1;
-      const size_t o = sidx(c, 0, ip, jj, kk);
       double q[5];
-      if (PUSH && gplane) {
-        int ea = 0, eb = 0;
-        if (!edge_sources(ip, jj, kk, ea, eb)) ea = eb = ip * 400 + jj * 20 + kk;
-        bool na, nb;
-        ea = zsrc(ea, na);
-        eb = zsrc(eb, nb);
-        const size_t cb = (size_t)c * CUBE_STRIDE;
-#pragma unroll
-        for (int v = 0; v < 5; ++v) {
-          const double x = U[cb + (size_t)v * PLANE + ea], y = U[cb + (size_t)v * PLANE + eb];
-          q[v] = 0.5 * ((v == 3 && na ? -x : x) + (v == 3 && nb ? -y : y));
-        }
-      } else if (PUSH) {
-        bool na;
-        const int ea = zsrc(ip * 400 + jj * 20 + kk, na);
-        const size_t cb = (size_t)c * CUBE_STRIDE;
-#pragma unroll
-        for (int v = 0; v < 5; ++v) {
-          const double x = U[cb + (size_t)v * PLANE + ea];
-          q[v] = v == 3 && na ? -x : x;
-        }
-      } else {
-#pragma unroll
-        for (int v = 0; v < 5; ++v) q[v] = U[o + (size_t)v * PLANE];
-      }
-      const double r = q[0];
-      const DivBy dq = div_by(r);
-    const double u = div_sh(dq, q[1]), v = div_sh(dq, q[2]), w = div_sh(dq, q[3]);
-      const double P = (q[4] - 0.5 * r * (u * u + v * v + w * w)) * KM1;
-      pr[0][slot * PR_PLANE + n] = u;
-      pr[1][slot * PR_PLANE + n] = v;
-      pr[2][slot * PR_PLANE + n] = w;
-      pr[3][slot * PR_PLANE + n] = P / (r * RG);
+      fill_load(ip, n, q);
+      fill_store(ip, n, q);
     }
   };
   // PUSH: the four corner cells of interior plane ip, formed from the staged
@@ -754,33 +763,42 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
     // plane i-1 reaches i-4); the barrier that ended that iteration makes its
     // update safe now.
     if (!PUSH && fuse && i - 4 >= i0) update_plane(i - 4);
-    fill_prims(i + 1);
-    // Plane i (conserved values and flags) into LDS for the y- and z-faces:
-    // two k-neighbours per lane in one 16-byte access, or cell by cell where
-    // the z-ghost cells are redirected.
-    if (!(zlo || zhi)) {
-      if (t < 200) {
+    {
+      // Plane i's conserved values go from memory straight into the staging
+      // buffer (global_load_lds: lane l of a wave writes the 16 bytes at the
+      // wave's base + 16 l), issued before the primitive fill so that the
+      // fill's loads and arithmetic run while they land; a wait on the
+      // vector memory counter before the barrier completes them.
+      const bool plain = !(zlo || zhi);
+      if (plain && t < 200) {
         const size_t o = sidx(c, 0, i, 0, 0) + 2 * t;
 #pragma unroll
         for (int v = 0; v < 5; ++v)
-          *reinterpret_cast<double2*>(&cpl[v * 400 + 2 * t]) =
-              *reinterpret_cast<const double2*>(&U[o + (size_t)v * PLANE]);
-        const int8_t* fr = fws + fidx(c, i, 0, 0) + 2 * t;
-        fpl[2 * t] = fr[0];
-        fpl[2 * t + 1] = fr[1];
+          __builtin_amdgcn_global_load_lds(
+              (const __attribute__((address_space(1))) void*)(U + o + (size_t)v * PLANE),
+              (__attribute__((address_space(3))) void*)(cpl + v * 400 + 2 * (t & ~63)), 16, 0, 0);
       }
-    } else {
-      for (int n = t; n < 400; n += WG) {
-        bool neg = false;
-        const int src = zsrc(i * 400 + n, neg) - i * 400;
-        const size_t o = sidx(c, 0, i, 0, 0) + src;
-#pragma unroll
-        for (int v = 0; v < 5; ++v) {
-          const double x = U[o + (size_t)v * PLANE];
-          cpl[v * 400 + n] = v == 3 && neg ? -x : x;
+      fill_prims(i + 1);
+      if (plain) {
+        if (t < 200) {
+          const int8_t* fr = fws + fidx(c, i, 0, 0) + 2 * t;
+          fpl[2 * t] = fr[0];
+          fpl[2 * t + 1] = fr[1];
         }
-        fpl[n] = fws[fidx(c, i, 0, 0) + n];
+      } else {
+        for (int n = t; n < 400; n += WG) {
+          bool neg = false;
+          const int src = zsrc(i * 400 + n, neg) - i * 400;
+          const size_t o = sidx(c, 0, i, 0, 0) + src;
+#pragma unroll
+          for (int v = 0; v < 5; ++v) {
+            const double x = U[o + (size_t)v * PLANE];
+            cpl[v * 400 + n] = v == 3 && neg ? -x : x;
+          }
+          fpl[n] = fws[fidx(c, i, 0, 0) + n];
+        }
       }
+      __builtin_amdgcn_s_waitcnt(0xF70);   // vmcnt(0)
     }
     __syncthreads();
     // y-faces (face f = (p-1)*16 + kk-2 joins j = p and p+1 at k = kk) and
