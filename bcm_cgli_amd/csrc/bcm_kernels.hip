@@ -769,8 +769,7 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
       // wave's base + 16 l), issued before the primitive fill so that the
       // fill's loads and arithmetic run while they land; a wait on the
       // vector memory counter before the barrier completes them.
-      const bool plain = !(zlo || zhi);
-      if (plain && t < 200) {
+      if (t < 200) {
         const size_t o = sidx(c, 0, i, 0, 0) + 2 * t;
 #pragma unroll
         for (int v = 0; v < 5; ++v)
@@ -778,24 +777,43 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
               (const __attribute__((address_space(1))) void*)(U + o + (size_t)v * PLANE),
               (__attribute__((address_space(3))) void*)(cpl + v * 400 + 2 * (t & ~63)), 16, 0, 0);
       }
-      fill_prims(i + 1);
-      if (plain) {
-        if (t < 200) {
-          const int8_t* fr = fws + fidx(c, i, 0, 0) + 2 * t;
-          fpl[2 * t] = fr[0];
-          fpl[2 * t + 1] = fr[1];
-        }
-      } else {
-        for (int n = t; n < 400; n += WG) {
-          bool neg = false;
-          const int src = zsrc(i * 400 + n, neg) - i * 400;
-          const size_t o = sidx(c, 0, i, 0, 0) + src;
+      // Primitives of plane i+1.  PUSH: the plane's four corners are never
+      // read from the fill (ghost planes are never the current plane, and an
+      // interior plane's corners come from corner_prims), so its 320 other
+      // cells fill the 256 lanes and wave 3.
+#pragma unroll 1
+      for (int rd = 0; rd < 2; ++rd) {
+        const int m = rd == 0 ? t : (PUSH ? WG + t - 192 : WG + t - 188);
+        if (rd == 1 && t < (PUSH ? 192 : 188)) continue;
+        const int n = PUSH ? m + 1 + (m >= 16) + (m >= 304) : m;
+        double q[5];
+        fill_load(i + 1, n, q);
+        fill_store(i + 1, n, q);
+      }
+      // ZBC_IN_FLUX faces: the staged plane's z-ghost cells (k = 0, 1 from
+      // k = 2 with w negated; k = 18, 19 from k = 17), 4 per row of 20, are
+      // loaded here and written over the copied plane once every wave's copy
+      // has landed (a barrier of their own).
+      double zq[5];
+      const int zrow = t >> 2, zk = t & 3;
+      const bool zfix = t < 80 && (zk < 2 ? zlo : zhi);
+      const int zdst = zrow * 20 + (zk < 2 ? zk : zk + 16), zs = zrow * 20 + (zk < 2 ? 2 : 17);
+      if (zfix) {
+        const size_t o = sidx(c, 0, i, 0, 0) + zs;
 #pragma unroll
-          for (int v = 0; v < 5; ++v) {
-            const double x = U[o + (size_t)v * PLANE];
-            cpl[v * 400 + n] = v == 3 && neg ? -x : x;
-          }
-          fpl[n] = fws[fidx(c, i, 0, 0) + n];
+        for (int v = 0; v < 5; ++v) zq[v] = U[o + (size_t)v * PLANE];
+      }
+      if (t < 200) {
+        const int8_t* fr = fws + fidx(c, i, 0, 0) + 2 * t;
+        fpl[2 * t] = fr[0];
+        fpl[2 * t + 1] = fr[1];
+      }
+      if (zlo || zhi) {
+        __builtin_amdgcn_s_waitcnt(0xF70);   // vmcnt(0)
+        __syncthreads();
+        if (zfix) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) cpl[v * 400 + zdst] = v == 3 && zk < 2 ? -zq[v] : zq[v];
         }
       }
       __builtin_amdgcn_s_waitcnt(0xF70);   // vmcnt(0)
