@@ -780,11 +780,11 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
       // Primitives of plane i+1.  PUSH: the plane's four corners are never
       // read from the fill (ghost planes are never the current plane, and an
       // interior plane's corners come from corner_prims), so its 320 other
-      // cells fill the 256 lanes and wave 3.
+      // cells fill the 256 lanes here and wave 3 in the face pass.
 #pragma unroll 1
-      for (int rd = 0; rd < 2; ++rd) {
-        const int m = rd == 0 ? t : (PUSH ? WG + t - 192 : WG + t - 188);
-        if (rd == 1 && t < (PUSH ? 192 : 188)) continue;
+      for (int rd = 0; rd < (PUSH ? 1 : 2); ++rd) {
+        const int m = rd == 0 ? t : WG + t - 188;
+        if (rd == 1 && t < 188) continue;
         const int n = PUSH ? m + 1 + (m >= 16) + (m >= 304) : m;
         double q[5];
         fill_load(i + 1, n, q);
@@ -851,6 +851,17 @@ __global__ __launch_bounds__(WG, 2) void k_flux_march(
         for (int v = 0; v < 5; ++v) fz[v][g] = F[v];
         fz[5][g] = dP;
       }
+    }
+    // PUSH: the fill's second round (wave 3, which has no boundary faces)
+    // runs in the face pass; plane i+1's primitives are first read after
+    // the next barrier, and its ring slot was last read before the previous
+    // one.
+    if (PUSH && t >= 192) {
+      const int m = WG + t - 192;
+      const int n = m + 1 + (m >= 16) + (m >= 304);
+      double q[5];
+      fill_load(i + 1, n, q);
+      fill_store(i + 1, n, q);
     }
     // The x-stencil of the cell's forward face, planes i-2..i+3, and the
     // flags it needs, loaded before the barrier so that they arrive while the
