@@ -217,6 +217,22 @@ def test_cfg2_one_physical_step_bitwise(B, O, engine):
     assert er_close(h, ho)
 
 
+def test_cfg2_three_physical_steps_bitwise(B, O, engine):
+    # Full-size case over three physical steps of 10 inner iterations (three
+    # graph replays, 90 pushed stages, the physical-time shift between steps,
+    # so the dual-time source of steps 2 and 3 comes from shifted levels)
+    # against the oracle on the host cores: every array the reference keeps
+    # bitwise, including the auxiliary arrays of the last inner iteration, and
+    # the norms of every inner iteration at 1e-12.
+    case = B.Case(B.case_params("cfg2"))
+    engine.load_case(case)
+    h = engine.run(3, 10)
+    orc = O.OracleRun(case, nthreads=16)
+    ho = orc.run(3, 10)
+    compare(engine, orc, STATE)
+    assert er_close(h, ho)
+
+
 def _dropin_run(B, case, n_steps, n_inner, output_dir=None):
     """Runs the reference main-loop call sequence (main.cpp:1146-1357) through
     the drop-in BCM_* entry points on host arrays bound to an engine; with
